@@ -1,0 +1,230 @@
+"""Benchmark: FP64 BCSR SpMV block-size sweep on one MI355X.
+
+Workload (the SpMV sweep the headline metric is quoted on): 9-point block
+stencil matrices (diagonally dominant, seed 0) for bs in {1,2,3,4,6,8,12,16,
+24,32}, square grid side n = round(sqrt(S/(9*bs^2))) with S = 4.4e8 stored
+values ("hbm" sizing, ~3.5-6.3 GB per matrix, all resident in HBM together) or
+S = 4.4e7 ("literal" sizing), x uniform[0,1].
+
+One step = one FP64 SpMV on every matrix of the sweep, each with its fastest
+kernel family/layout (chosen by timing every family first).  value = compulsory
+bytes of the step / step time (GB/s); the timed region is K steps between two
+HIP events on the work stream, inputs already resident in HBM.
+
+Also reported: per-(bs, layout, family) GB/s, the dominant kernel's roofline
+(HIP-event launch time; compulsory bytes per launch), and the CPU baseline:
+the oracle's single-threaded CSR SpMV (main.cpp:36-45 restated) on a bounded
+sample, timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import block_matrix_format_performance_amd as bm  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BS_SWEEP = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+FAMILY_NAMES = {
+    bm.BCSR_THREAD_PER_ROW: "thread_per_row",
+    bm.BCSR_GROUP_PER_ROW: "group_per_row",
+    bm.BCSR_SHUFFLE_X: "shuffle_x",
+    bm.BCSR_BLOCK_ROW_GROUP: "block_row_group",
+    bm.BCSR_LIBRARY: "rocsparse_bsrmv",
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def grid_side(bs, total_values):
+    return int(round(math.sqrt(total_values / (9.0 * bs * bs))))
+
+
+def build_case(bs, side, layouts):
+    rp, cols, vals = bm.gen_stencil9(side, side, bs, 0, seed=0)
+    n = side * side
+    mats = {}
+    for layout in layouts:
+        if layout == bm.ROW_MAJOR:
+            v = vals
+        else:
+            v = np.ascontiguousarray(vals.reshape(-1, bs, bs).transpose(0, 2, 1)).ravel()
+        mats[layout] = bm.Matrix.bcsr(rp, cols, v, n, n, bs, dtype=bm.F64, layout=layout)
+    return mats, rp, cols, vals
+
+
+def cpu_baseline(seconds):
+    """Oracle single-thread CSR SpMV on the bs=4 matrix of the literal sizing."""
+    import oracle
+    bs = 4
+    side = grid_side(bs, 4.4e7)
+    m = oracle.gen_stencil9(side, side, bs, 0, 0)
+    c = oracle.bcsr_to_csr(m)
+    x = oracle.rng_unit_vec(0, 4, c.n_cols) if c.n_cols < 2_000_000 else np.random.default_rng(0).uniform(0, 1, c.n_cols)
+    nbytes = 12.0 * c.nnz + 4.0 * (c.n_rows + 1) + 16.0 * c.n_rows
+    oracle.csr_spmv(c, x)  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.csr_spmv(c, x)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    per = el / reps
+    return {"value": round(nbytes / per / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle CSR SpMV (main.cpp:36-45 order), bs=4 stencil {side}x{side} "
+                      f"(nnz={c.nnz}), {reps} reps in {el:.1f} s, {per*1e3:.2f} ms each",
+            "seconds_per_spmv": per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--sizing", choices=("hbm", "literal"), default="hbm")
+    ap.add_argument("--bs", default=",".join(map(str, BS_SWEEP)))
+    ap.add_argument("--tune-reps", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--families", default="all", help="all | auto")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    bm.init(0)
+    dev = bm.device_info()
+    log(f"[bench] device {dev['name']} CUs {dev['compute_units']} mem {dev['total_mem']/2**30:.0f} GiB")
+    total_values = 4.4e8 if args.sizing == "hbm" else 4.4e7
+    bs_list = [int(b) for b in args.bs.split(",")]
+    stream = bm.Stream()
+
+    cases = []
+    sweep = {}
+    for bs in bs_list:
+        side = grid_side(bs, total_values)
+        t0 = time.perf_counter()
+        mats, _, _, _ = build_case(bs, side, (bm.ROW_MAJOR, bm.COL_MAJOR))
+        A0 = mats[bm.ROW_MAJOR]
+        x = bm.DeviceArray(A0.x_len)
+        x.fill(0, seed=0, stream_id=4)
+        y = bm.DeviceArray(A0.y_len)
+        nbytes, flops = A0.traffic()
+        log(f"[bench] bs={bs} grid {side}x{side} N={A0.n_rows} nnzb={A0.nnzb} bytes={nbytes/1e9:.3f} GB "
+            f"(built in {time.perf_counter()-t0:.1f} s)")
+        results = {}
+        best = None
+        for layout, A in mats.items():
+            fams = [bm.BCSR_THREAD_PER_ROW, bm.BCSR_GROUP_PER_ROW, bm.BCSR_SHUFFLE_X]
+            if layout == bm.COL_MAJOR:
+                fams.append(bm.BCSR_BLOCK_ROW_GROUP)
+            fams.append(bm.BCSR_LIBRARY)
+            if args.families == "auto":
+                fams = [A.auto_variant()]
+            for fam in fams:
+                t = A.spmv_timed(x, y, fam, warmup=5, reps=args.tune_reps, stream=stream.handle)
+                gbs = nbytes / t / 1e9
+                key = f"{'row' if layout == bm.ROW_MAJOR else 'col'}/{FAMILY_NAMES[fam]}"
+                results[key] = round(gbs, 1)
+                log(f"[bench]   {key:28s} {t*1e6:9.1f} us  {gbs:8.1f} GB/s  {gbs/HBM_PEAK_GBS*100:5.1f}%")
+                if fam != bm.BCSR_LIBRARY and (best is None or t < best[0]):
+                    best = (t, layout, fam, key)
+        sweep[str(bs)] = results
+        keep = mats[best[1]]
+        for layout in list(mats):
+            if layout != best[1]:
+                mats[layout].destroy()
+        cases.append({"bs": bs, "side": side, "A": keep, "family": best[2], "key": best[3], "x": x, "y": y,
+                      "bytes": nbytes, "flops": flops})
+
+    # ---- timed steps: every matrix once per step, best family each ----------
+    def step():
+        for c in cases:
+            c["A"].spmv(c["x"], c["y"], c["family"], stream=stream.handle)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    e0, e1 = bm.Event(), bm.Event()
+    e0.record(stream.handle)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream.handle)
+    total = e1.elapsed_since(e0)
+    stream.synchronize()
+    step_bytes = sum(c["bytes"] for c in cases)
+    ms_per_step = total / args.steps * 1e3
+    value = step_bytes * args.steps / total / 1e9
+
+    # ---- per-kernel launch time (HIP events on the work stream) -------------
+    per_kernel = []
+    for c in cases:
+        t = c["A"].spmv_timed(c["x"], c["y"], c["family"], warmup=args.warmup, reps=args.steps,
+                              stream=stream.handle)
+        per_kernel.append((t, c))
+    dom_t, dom = max(per_kernel, key=lambda p: p[0])
+    achieved = dom["bytes"] / dom_t / 1e9
+
+    out = {
+        "metric": "BCSR SpMV GB/s (% HBM peak); BiCGStab time to 1e-10 mixed vs FP64; p99 ms",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 9-point block stencil, x uniform[0,1])",
+        "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "config": {
+            "workload": f"FP64 BCSR SpMV block-size sweep, {args.sizing} sizing "
+                        f"({total_values:.1e} stored values per matrix), best kernel family per bs",
+            "bs": bs_list,
+            "grid_sides": [c["side"] for c in cases],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"bs={dom['bs']} {dom['key']}",
+            "launch_us": round(dom_t * 1e6, 2),
+            "algorithmic_bytes": dom["bytes"],
+        },
+        "per_bs_best": {str(c["bs"]): {"kernel": c["key"], "us": round(t * 1e6, 2),
+                                         "GB/s": round(c["bytes"] / t / 1e9, 1),
+                                         "frac": round(c["bytes"] / t / 1e9 / HBM_PEAK_GBS, 4)}
+                        for t, c in per_kernel},
+        "sweep_GBs": sweep,
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cb = cpu_baseline(args.cpu_seconds)
+        cb.pop("seconds_per_spmv")
+        cb["host_cpus"] = os.cpu_count()
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

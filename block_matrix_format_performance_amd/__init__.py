@@ -1,0 +1,371 @@
+"""Python binding of the gfx950 block-sparse SpMV / BiCGStab library (libbmfp.so).
+
+The product is the C ABI declared in include/bmfp.h, implemented by hand-written
+HIP kernels for gfx950.  This module is a thin ctypes layer over it for tests and
+the benchmark script; every compute call goes through libbmfp.so, and importing
+anything that needs the library fails loudly if it has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libbmfp.so")
+CPP_LIB_PATH = os.path.join(PKG_DIR, "libbmfp_cpp.so")
+MAIN_PATH = os.path.join(PKG_DIR, "bmfp_main")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "bmfp.h")
+
+# ---- constants mirrored from include/bmfp.h ---------------------------------
+OK = 0
+ERR_INVALID = -1
+ERR_UNSUPPORTED = -2
+ERR_NO_MEMORY = -3
+ERR_BREAKDOWN = -4
+ERR_NOT_CONVERGED = -5
+ERR_HIP = -6
+ERR_NONFINITE = -7
+ERR_BUDGET = -8
+ERR_NO_DEVICE = -9
+
+F64, F32, F16 = 0, 1, 2
+ROW_MAJOR, COL_MAJOR = 0, 1
+
+SPMV_AUTO = 0
+CSR_SCALAR = 1
+CSR_VECTOR = 2
+BCSR_THREAD_PER_ROW = 3
+BCSR_GROUP_PER_ROW = 4
+BCSR_SHUFFLE_X = 5
+BCSR_BLOCK_ROW_GROUP = 6
+BCSR_LIBRARY = 7
+
+BCSR_VARIANTS = (BCSR_THREAD_PER_ROW, BCSR_GROUP_PER_ROW, BCSR_SHUFFLE_X, BCSR_BLOCK_ROW_GROUP)
+PRECOMPILED_BS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+
+_VEC_NP = {F64: np.float64, F32: np.float32, F16: np.float32}
+
+
+class BmfpError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"bmfp status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads libbmfp.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "or `make -C block_matrix_format_performance_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    c_int, c_dbl, vp, sz = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.bmfp_last_error.restype = ctypes.c_char_p
+    L.bmfp_status_string.restype = ctypes.c_char_p
+    L.bmfp_status_string.argtypes = [c_int]
+    L.bmfp_version.restype = c_int
+    L.bmfp_init.argtypes = [c_int]
+    L.bmfp_device_count.argtypes = [ip]
+    L.bmfp_device_info.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz), ip, ip]
+    L.bmfp_malloc.argtypes = [ctypes.POINTER(vp), sz]
+    L.bmfp_free.argtypes = [vp]
+    L.bmfp_memcpy_h2d.argtypes = [vp, vp, sz]
+    L.bmfp_memcpy_d2h.argtypes = [vp, vp, sz]
+    L.bmfp_memcpy_d2d.argtypes = [vp, vp, sz]
+    L.bmfp_memset.argtypes = [vp, c_int, sz]
+    L.bmfp_stream_create.argtypes = [ctypes.POINTER(vp)]
+    L.bmfp_stream_destroy.argtypes = [vp]
+    L.bmfp_stream_synchronize.argtypes = [vp]
+    L.bmfp_event_create.argtypes = [ctypes.POINTER(vp)]
+    L.bmfp_event_destroy.argtypes = [vp]
+    L.bmfp_event_record.argtypes = [vp, vp]
+    L.bmfp_event_elapsed.argtypes = [vp, vp, dp]
+    L.bmfp_fill.argtypes = [vp, c_int, sz, c_int, ctypes.c_uint64, ctypes.c_uint64, c_dbl, vp]
+    L.bmfp_gen_n_diag_bcsr.argtypes = [c_int, c_int, c_int, vp, vp, vp]
+    L.bmfp_stencil_nnzb.restype = ctypes.c_int64
+    L.bmfp_stencil_nnzb.argtypes = [c_int, c_int]
+    L.bmfp_gen_stencil9.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_uint64, vp, vp, vp]
+    L.bmfp_bcsr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, c_int, c_int, vp, vp, vp]
+    L.bmfp_csr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, vp, vp, vp]
+    L.bmfp_matrix_convert.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp), vp]
+    L.bmfp_matrix_destroy.argtypes = [vp]
+    L.bmfp_matrix_shape.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
+    L.bmfp_spmv_traffic.argtypes = [vp, dp, dp]
+    L.bmfp_matrix_download_values.argtypes = [vp, vp]
+    L.bmfp_spmv.argtypes = [vp, c_int, vp, vp, vp]
+    L.bmfp_spmv_timed.argtypes = [vp, c_int, vp, vp, c_int, c_int, vp, dp]
+    L.bmfp_spmv_auto_variant.argtypes = [vp, ip]
+    _declare_solver(L)
+
+
+def _declare_solver(L):
+    """Solver / refinement / serving entry points (present once built)."""
+    c_int, c_dbl, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    if hasattr(L, "bmfp_bicgstab_create"):
+        L.bmfp_bicgstab_create.argtypes = [ctypes.POINTER(vp), vp, c_int]
+        L.bmfp_bicgstab_solve.argtypes = [vp, vp, vp, c_dbl, c_int, vp, ctypes.POINTER(SolveInfo)]
+        L.bmfp_bicgstab_destroy.argtypes = [vp]
+
+
+class SolveInfo(ctypes.Structure):
+    """Mirror of bmfp_solve_info (include/bmfp.h)."""
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("outer_iterations", ctypes.c_int),
+                ("inner_iterations", ctypes.c_int)]
+
+
+def check(status):
+    if status != OK:
+        raise BmfpError(status, lib().bmfp_last_error().decode(errors="replace"))
+    return status
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---- device plumbing -----------------------------------------------------------
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().bmfp_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def init(device=0):
+    check(lib().bmfp_init(device))
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    mem = ctypes.c_size_t(0)
+    cus = ctypes.c_int(0)
+    arch = ctypes.c_int(0)
+    check(lib().bmfp_device_info(name, 256, ctypes.byref(mem), ctypes.byref(cus), ctypes.byref(arch)))
+    return {"name": name.value.decode(), "total_mem": mem.value, "compute_units": cus.value,
+            "arch": arch.value}
+
+
+def synchronize():
+    check(lib().bmfp_device_synchronize())
+
+
+class DeviceArray:
+    """Owning device buffer of `n` elements of numpy dtype `dtype`."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().bmfp_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().bmfp_memcpy_h2d(self.ptr, _ptr(a), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().bmfp_memcpy_d2h(_ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def fill(self, kind, seed=0, stream_id=0, value=0.0, stream=None):
+        dt = F64 if self.dtype == np.float64 else F32
+        check(lib().bmfp_fill(self.ptr, dt, self.n, kind, seed, stream_id, value, stream))
+
+    def zero(self):
+        check(lib().bmfp_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().bmfp_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = ctypes.c_void_p()
+        check(lib().bmfp_stream_create(ctypes.byref(s)))
+        self.handle = s
+
+    def synchronize(self):
+        check(lib().bmfp_stream_synchronize(self.handle))
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().bmfp_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Event:
+    """hipEvent_t recorded on a stream (None = null stream)."""
+
+    def __init__(self):
+        e = ctypes.c_void_p()
+        check(lib().bmfp_event_create(ctypes.byref(e)))
+        self.handle = e
+
+    def record(self, stream=None):
+        check(lib().bmfp_event_record(self.handle, stream))
+
+    def elapsed_since(self, start):
+        t = ctypes.c_double(0)
+        check(lib().bmfp_event_elapsed(start.handle, self.handle, ctypes.byref(t)))
+        return t.value
+
+    def __del__(self):
+        try:
+            lib().bmfp_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+# ---- host generators (product code, libbmfp.so) ----------------------------------
+def gen_stencil9(nx, ny, bs, kind=0, seed=0):
+    """9-point block stencil (row-major blocks); returns (row_ptr, columns, values)."""
+    L = lib()
+    nnzb = int(L.bmfp_stencil_nnzb(nx, ny))
+    n = nx * ny
+    rp = np.zeros(n + 1, np.int32)
+    cols = np.zeros(nnzb, np.int32)
+    vals = np.zeros(nnzb * bs * bs, np.float64)
+    check(L.bmfp_gen_stencil9(nx, ny, bs, kind, seed, _ptr(rp), _ptr(cols), _ptr(vals)))
+    return rp, cols, vals
+
+
+def gen_n_diag_bcsr(n_rows, blocks_per_row, bs):
+    nnzb = n_rows * blocks_per_row
+    rp = np.zeros(n_rows + 1, np.int32)
+    cols = np.zeros(nnzb, np.int32)
+    vals = np.zeros(nnzb * bs * bs, np.float64)
+    check(lib().bmfp_gen_n_diag_bcsr(n_rows, blocks_per_row, bs, _ptr(rp), _ptr(cols), _ptr(vals)))
+    return rp, cols, vals
+
+
+# ---- matrices ---------------------------------------------------------------------
+class Matrix:
+    """Device-resident BCSR or CSR matrix (a bmfp_matrix handle)."""
+
+    def __init__(self, handle, keep=None):
+        self.handle = handle
+        self._keep = keep
+        vals = [ctypes.c_int(0) for _ in range(7)]
+        check(lib().bmfp_matrix_shape(handle, *[ctypes.byref(v) for v in vals]))
+        (self.is_bcsr, self.dtype, self.layout, self.n_rows, self.n_cols,
+         self.bs, self.nnzb) = [v.value for v in vals]
+
+    @classmethod
+    def bcsr(cls, row_ptr, columns, values, n_rows, n_cols, bs, dtype=F64, layout=ROW_MAJOR):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        cols = np.ascontiguousarray(columns, np.int32)
+        vals = np.ascontiguousarray(values, np.float64)
+        h = ctypes.c_void_p()
+        check(lib().bmfp_bcsr_create(ctypes.byref(h), dtype, layout, n_rows, n_cols, bs,
+                                     int(rp[-1]), _ptr(rp), _ptr(cols), _ptr(vals)))
+        return cls(h)
+
+    @classmethod
+    def csr(cls, row_ptr, columns, values, n_rows, n_cols, dtype=F64):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        cols = np.ascontiguousarray(columns, np.int32)
+        vals = np.ascontiguousarray(values, np.float64)
+        h = ctypes.c_void_p()
+        check(lib().bmfp_csr_create(ctypes.byref(h), dtype, n_rows, n_cols, int(rp[-1]),
+                                    _ptr(rp), _ptr(cols), _ptr(vals)))
+        return cls(h)
+
+    def convert(self, dtype, layout=None, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().bmfp_matrix_convert(self.handle, dtype, self.layout if layout is None else layout,
+                                        ctypes.byref(h), stream))
+        return Matrix(h)
+
+    @property
+    def vec_dtype(self):
+        return _VEC_NP[self.dtype]
+
+    @property
+    def x_len(self):
+        return self.n_cols * self.bs
+
+    @property
+    def y_len(self):
+        return self.n_rows * self.bs
+
+    def spmv(self, x, y, variant=SPMV_AUTO, stream=None):
+        check(lib().bmfp_spmv(self.handle, variant, x.ptr, y.ptr, stream))
+
+    def spmv_timed(self, x, y, variant=SPMV_AUTO, warmup=20, reps=200, stream=None):
+        t = ctypes.c_double(0)
+        check(lib().bmfp_spmv_timed(self.handle, variant, x.ptr, y.ptr, warmup, reps, stream,
+                                    ctypes.byref(t)))
+        return t.value
+
+    def traffic(self):
+        b, f = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().bmfp_spmv_traffic(self.handle, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+    def auto_variant(self):
+        v = ctypes.c_int(0)
+        check(lib().bmfp_spmv_auto_variant(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def download_values(self):
+        out = np.empty(self.nnzb * self.bs * self.bs, np.float64)
+        check(lib().bmfp_matrix_download_values(self.handle, _ptr(out)))
+        return out
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().bmfp_matrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def spmv_host(A, x, variant=SPMV_AUTO):
+    """Uploads x, runs one SpMV through the library, returns y on the host."""
+    dx = DeviceArray.from_host(np.ascontiguousarray(x, A.vec_dtype))
+    dy = DeviceArray(A.y_len, A.vec_dtype)
+    A.spmv(dx, dy, variant)
+    synchronize()
+    return dy.download()

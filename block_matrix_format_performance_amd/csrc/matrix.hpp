@@ -1,0 +1,42 @@
+// matrix.hpp -- device-resident sparse matrix handle (bmfp_matrix).
+#pragma once
+
+#include "common.hpp"
+
+struct bmfp_matrix_t {
+  int is_bcsr = 1;
+  int dtype = BMFP_F64;        // storage type of values
+  int layout = BMFP_ROW_MAJOR; // block layout (BCSR only)
+  int n_rows = 0;              // block rows (BCSR) or rows (CSR)
+  int n_cols = 0;              // block columns (BCSR) or columns (CSR)
+  int bs = 1;                  // 1 for CSR
+  int nnzb = 0;                // nonzero blocks (BCSR) or nonzeros (CSR)
+  int device = 0;
+  int max_row_blocks = 0;      // longest (block) row
+  double mean_row_blocks = 0;  // mean (block) row length
+  int csr_group = 0;           // lanes per row chosen for BMFP_CSR_VECTOR
+
+  std::shared_ptr<bmfp::DeviceBuffer> row_ptr;  // int[n_rows+1]  (shared by converted copies)
+  std::shared_ptr<bmfp::DeviceBuffer> columns;  // int[nnzb]
+  bmfp::DeviceBuffer values;                    // dtype[nnzb*bs*bs]
+  bmfp::DeviceBuffer scale;                     // float[n_rows] (BMFP_F16 only)
+
+  // rocSPARSE state for BMFP_BCSR_LIBRARY, created on first use.
+  void *lib_handle = nullptr;
+  void *lib_descr = nullptr;
+  ~bmfp_matrix_t();
+
+  int64_t n_vals() const { return (int64_t) nnzb * bs * bs; }
+  int64_t x_len() const { return (int64_t) n_cols * bs; }
+  int64_t y_len() const { return (int64_t) n_rows * bs; }
+};
+
+namespace bmfp {
+int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
+int launch_csr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
+int launch_library(bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
+void destroy_library(bmfp_matrix_t &A);
+int auto_variant(const bmfp_matrix_t &A);
+int csr_group_for(double mean_len);
+bool bcsr_supported(int variant, int dtype, int layout, int bs);
+}  // namespace bmfp

@@ -1,0 +1,541 @@
+// runtime.hip -- C ABI: errors, device/memory/stream plumbing, matrix handles,
+// SpMV entry points.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "matrix.hpp"
+
+namespace bmfp {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const char *fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+const char *get_error() { return g_last_error.c_str(); }
+
+int fail(int code, const char *fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+// ---- device fill -------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T *dst, int64_t n, int kind, uint64_t key, double value)
+{
+  const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double v = kind == 0 ? rng_unit(key, (uint64_t) i) : kind == 1 ? rng_sym(key, (uint64_t) i) : value;
+    dst[i] = (T) v;
+  }
+}
+
+// ---- value conversion (F64 -> F64/F32/F16, optional block transpose) --------
+// One wavefront per block row: first the row's max |entry| (F16 scale), then
+// a strided pass that writes every element in the target layout.
+template <class OT, bool SCALED>
+__global__ __launch_bounds__(256) void k_convert(int n_brows, int bs, int src_layout, int dst_layout,
+                                                 const int *__restrict__ rp, const double *__restrict__ src,
+                                                 OT *__restrict__ dst, float *__restrict__ scale)
+{
+  const int64_t br = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (br >= n_brows) return;
+  const int64_t bs2 = (int64_t) bs * bs;
+  const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  double inv = 1.0;
+  if (SCALED) {
+    double m = 0.0;
+    for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(src[e]));
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (m == 0.0) m = 1.0;
+    inv = 1.0 / m;
+    if (lane == 0) scale[br] = (float) m;
+  }
+  for (int64_t e = e0 + lane; e < e1; e += 64) {
+    const int64_t b = e / bs2;
+    const int w = (int) (e - b * bs2);
+    int r, c;
+    if (src_layout == BMFP_ROW_MAJOR) { r = w / bs; c = w % bs; }
+    else { c = w / bs; r = w % bs; }
+    const int64_t d = b * bs2 + (dst_layout == BMFP_ROW_MAJOR ? (int64_t) r * bs + c : (int64_t) c * bs + r);
+    const double v = SCALED ? src[e] * inv : src[e];
+    if constexpr (sizeof(OT) == 2) dst[d] = __float2half_rn((float) v);
+    else dst[d] = (OT) v;
+  }
+}
+
+}  // namespace bmfp
+
+using namespace bmfp;
+
+bmfp_matrix_t::~bmfp_matrix_t() { destroy_library(*this); }
+
+// ---- diagnostics -------------------------------------------------------------
+extern "C" const char *bmfp_last_error(void) { return get_error(); }
+
+extern "C" const char *bmfp_status_string(int status)
+{
+  switch (status) {
+  case BMFP_OK: return "ok";
+  case BMFP_ERR_INVALID: return "invalid argument";
+  case BMFP_ERR_UNSUPPORTED: return "unsupported";
+  case BMFP_ERR_NO_MEMORY: return "out of device memory";
+  case BMFP_ERR_BREAKDOWN: return "solver breakdown";
+  case BMFP_ERR_NOT_CONVERGED: return "not converged";
+  case BMFP_ERR_HIP: return "HIP runtime error";
+  case BMFP_ERR_NONFINITE: return "non-finite value";
+  case BMFP_ERR_BUDGET: return "memory budget exceeded";
+  case BMFP_ERR_NO_DEVICE: return "no device";
+  default: return "unknown status";
+  }
+}
+
+extern "C" int bmfp_version(void) { return BMFP_VERSION; }
+
+// ---- device / memory / streams ---------------------------------------------------
+extern "C" int bmfp_device_count(int *count)
+{
+  if (!count) return fail(BMFP_ERR_INVALID, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_init(int device)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(BMFP_ERR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(BMFP_ERR_INVALID, "device %d out of range [0,%d)", device, n);
+  BMFP_HIP(hipSetDevice(device));
+  hipDeviceProp_t p;
+  BMFP_HIP(hipGetDeviceProperties(&p, device));
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+    return fail(BMFP_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, p.gcnArchName);
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_device_info(char *name, size_t name_len, size_t *total_mem, int *cus, int *arch)
+{
+  int dev = 0;
+  BMFP_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  BMFP_HIP(hipGetDeviceProperties(&p, dev));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (total_mem) *total_mem = p.totalGlobalMem;
+  if (cus) *cus = p.multiProcessorCount;
+  if (arch) *arch = p.major * 100 + p.minor;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_malloc(void **ptr, size_t bytes)
+{
+  if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
+  BMFP_HIP(hipMalloc(ptr, bytes ? bytes : 256));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_free(void *ptr)
+{
+  if (ptr) BMFP_HIP(hipFree(ptr));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_memset(void *dst, int value, size_t bytes)
+{
+  if (bytes) BMFP_HIP(hipMemset(dst, value, bytes));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stream_create(bmfp_stream *stream)
+{
+  if (!stream) return fail(BMFP_ERR_INVALID, "stream is NULL");
+  hipStream_t s;
+  BMFP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stream_destroy(bmfp_stream stream)
+{
+  if (stream) BMFP_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stream_synchronize(bmfp_stream stream)
+{
+  BMFP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_device_synchronize(void)
+{
+  BMFP_HIP(hipDeviceSynchronize());
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_event_create(bmfp_event *event)
+{
+  if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
+  hipEvent_t e;
+  BMFP_HIP(hipEventCreate(&e));
+  *event = e;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_event_destroy(bmfp_event event)
+{
+  if (event) BMFP_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_event_record(bmfp_event event, bmfp_stream stream)
+{
+  if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
+  BMFP_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *seconds)
+{
+  if (!start || !stop || !seconds) return fail(BMFP_ERR_INVALID, "NULL argument");
+  BMFP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+  float ms = 0.f;
+  BMFP_HIP(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+  *seconds = ms * 1e-3;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_fill(void *dst, int dtype, size_t n, int kind, uint64_t seed, uint64_t stream_id,
+                         double value, bmfp_stream stream)
+{
+  if (n == 0) return BMFP_OK;
+  if (!dst || kind < 0 || kind > 2) return fail(BMFP_ERR_INVALID, "bmfp_fill: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t key = rng_key(seed, stream_id);
+  const unsigned groups = (unsigned) std::min<size_t>((n + 255) / 256, 8192);
+  if (dtype == BMFP_F64)
+    k_fill<double><<<groups, 256, 0, s>>>(static_cast<double *>(dst), (int64_t) n, kind, key, value);
+  else if (dtype == BMFP_F32)
+    k_fill<float><<<groups, 256, 0, s>>>(static_cast<float *>(dst), (int64_t) n, kind, key, value);
+  else
+    return fail(BMFP_ERR_INVALID, "bmfp_fill: dtype must be F64 or F32");
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+// ---- matrices -------------------------------------------------------------------
+namespace {
+
+int check_structure(int n_rows, int n_cols, int bs, int nnzb, const int *row_ptr, const int *columns,
+                    int *max_row)
+{
+  if (n_rows < 0 || n_cols < 0 || bs < 1 || nnzb < 0)
+    return fail(BMFP_ERR_INVALID, "negative shape (n_rows %d, n_cols %d, bs %d, nnzb %d)", n_rows, n_cols, bs, nnzb);
+  if ((int64_t) nnzb * bs * bs > INT_MAX || (int64_t) n_rows * bs > INT_MAX || (int64_t) n_cols * bs > INT_MAX)
+    return fail(BMFP_ERR_INVALID, "matrix too large for int32 indices");
+  if (!row_ptr || (nnzb && !columns)) return fail(BMFP_ERR_INVALID, "NULL index array");
+  if (row_ptr[0] != 0 || row_ptr[n_rows] != nnzb)
+    return fail(BMFP_ERR_INVALID, "row_ptr[0]=%d, row_ptr[n_rows]=%d, nnz=%d", row_ptr[0], row_ptr[n_rows], nnzb);
+  int mx = 0;
+  for (int i = 0; i < n_rows; i++) {
+    const int len = row_ptr[i + 1] - row_ptr[i];
+    if (len < 0) return fail(BMFP_ERR_INVALID, "row_ptr decreases at row %d", i);
+    mx = std::max(mx, len);
+  }
+  for (int k = 0; k < nnzb; k++)
+    if (columns[k] < 0 || columns[k] >= n_cols)
+      return fail(BMFP_ERR_INVALID, "column index %d at %d out of range [0,%d)", columns[k], k, n_cols);
+  *max_row = mx;
+  return BMFP_OK;
+}
+
+int upload_indices(bmfp_matrix_t &m, const int *row_ptr, const int *columns)
+{
+  m.row_ptr = std::make_shared<DeviceBuffer>();
+  m.columns = std::make_shared<DeviceBuffer>();
+  BMFP_TRY(m.row_ptr->alloc(sizeof(int) * ((size_t) m.n_rows + 1)));
+  BMFP_TRY(m.columns->alloc(sizeof(int) * (size_t) m.nnzb));
+  BMFP_HIP(hipMemcpy(m.row_ptr->ptr, row_ptr, sizeof(int) * ((size_t) m.n_rows + 1), hipMemcpyHostToDevice));
+  if (m.nnzb)
+    BMFP_HIP(hipMemcpy(m.columns->ptr, columns, sizeof(int) * (size_t) m.nnzb, hipMemcpyHostToDevice));
+  return BMFP_OK;
+}
+
+// Converts device F64 values `src` (layout src_layout) into m.values / m.scale.
+int convert_values(bmfp_matrix_t &m, const double *src, int src_layout, hipStream_t s)
+{
+  const int64_t nv = m.n_vals();
+  BMFP_TRY(m.values.alloc((size_t) nv * dtype_size(m.dtype)));
+  if (m.dtype == BMFP_F16) BMFP_TRY(m.scale.alloc(sizeof(float) * (size_t) std::max(m.n_rows, 1)));
+  if (m.n_rows == 0) return BMFP_OK;
+  const unsigned groups = (unsigned) (((int64_t) m.n_rows * 64 + 255) / 256);
+  const int *rp = m.row_ptr->as<int>();
+  switch (m.dtype) {
+  case BMFP_F64:
+    k_convert<double, false><<<groups, 256, 0, s>>>(m.n_rows, m.bs, src_layout, m.layout, rp, src,
+                                                   m.values.as<double>(), nullptr);
+    break;
+  case BMFP_F32:
+    k_convert<float, false><<<groups, 256, 0, s>>>(m.n_rows, m.bs, src_layout, m.layout, rp, src,
+                                                  m.values.as<float>(), nullptr);
+    break;
+  case BMFP_F16:
+    k_convert<__half, true><<<groups, 256, 0, s>>>(m.n_rows, m.bs, src_layout, m.layout, rp, src,
+                                                  m.values.as<__half>(), m.scale.as<float>());
+    break;
+  default: return fail(BMFP_ERR_INVALID, "bad dtype %d", m.dtype);
+  }
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+}  // namespace
+
+extern "C" int bmfp_bcsr_create(bmfp_matrix *out, int dtype, int layout, int n_rows, int n_cols, int bs, int nnzb,
+                                const int *row_ptr, const int *columns, const double *values)
+{
+  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
+  if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
+  int mx = 0;
+  BMFP_TRY(check_structure(n_rows, n_cols, bs, nnzb, row_ptr, columns, &mx));
+  if (nnzb && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
+  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+  m->is_bcsr = 1;
+  m->dtype = dtype;
+  m->layout = layout;
+  m->n_rows = n_rows;
+  m->n_cols = n_cols;
+  m->bs = bs;
+  m->nnzb = nnzb;
+  m->max_row_blocks = mx;
+  m->mean_row_blocks = n_rows ? (double) nnzb / n_rows : 0.0;
+  BMFP_HIP(hipGetDevice(&m->device));
+  BMFP_TRY(upload_indices(*m, row_ptr, columns));
+  const size_t nv = (size_t) m->n_vals();
+  if (dtype == BMFP_F64) {
+    BMFP_TRY(m->values.alloc(nv * sizeof(double)));
+    if (nv) BMFP_HIP(hipMemcpy(m->values.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    DeviceBuffer staging;
+    BMFP_TRY(staging.alloc(nv * sizeof(double)));
+    if (nv) BMFP_HIP(hipMemcpy(staging.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
+    BMFP_TRY(convert_values(*m, staging.as<double>(), layout, nullptr));
+    BMFP_HIP(hipDeviceSynchronize());
+  }
+  *out = m.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_cols, int nnz, const int *row_ptr,
+                               const int *columns, const double *values)
+{
+  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
+  int mx = 0;
+  BMFP_TRY(check_structure(n_rows, n_cols, 1, nnz, row_ptr, columns, &mx));
+  if (nnz && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
+  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+  m->is_bcsr = 0;
+  m->dtype = dtype;
+  m->n_rows = n_rows;
+  m->n_cols = n_cols;
+  m->bs = 1;
+  m->nnzb = nnz;
+  m->max_row_blocks = mx;
+  m->mean_row_blocks = n_rows ? (double) nnz / n_rows : 0.0;
+  m->csr_group = csr_group_for(m->mean_row_blocks);
+  BMFP_HIP(hipGetDevice(&m->device));
+  BMFP_TRY(upload_indices(*m, row_ptr, columns));
+  if (dtype == BMFP_F64) {
+    BMFP_TRY(m->values.alloc(sizeof(double) * (size_t) nnz));
+    if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, values, sizeof(double) * (size_t) nnz, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(values, values + nnz);
+    BMFP_TRY(m->values.alloc(sizeof(float) * (size_t) nnz));
+    if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, f.data(), sizeof(float) * (size_t) nnz, hipMemcpyHostToDevice));
+  }
+  *out = m.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_convert(bmfp_matrix src, int dtype, int layout, bmfp_matrix *out, bmfp_stream stream)
+{
+  if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (!src->is_bcsr || src->dtype != BMFP_F64)
+    return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_convert needs an F64 BCSR source");
+  if (dtype < BMFP_F64 || dtype > BMFP_F16 || (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR))
+    return fail(BMFP_ERR_INVALID, "bad dtype/layout");
+  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+  m->is_bcsr = 1;
+  m->dtype = dtype;
+  m->layout = layout;
+  m->n_rows = src->n_rows;
+  m->n_cols = src->n_cols;
+  m->bs = src->bs;
+  m->nnzb = src->nnzb;
+  m->max_row_blocks = src->max_row_blocks;
+  m->mean_row_blocks = src->mean_row_blocks;
+  m->device = src->device;
+  m->row_ptr = src->row_ptr;
+  m->columns = src->columns;
+  BMFP_TRY(convert_values(*m, src->values.as<double>(), src->layout, static_cast<hipStream_t>(stream)));
+  *out = m.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_destroy(bmfp_matrix m)
+{
+  delete m;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_shape(bmfp_matrix m, int *is_bcsr, int *dtype, int *layout, int *n_rows, int *n_cols,
+                                 int *bs, int *nnzb)
+{
+  if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
+  if (is_bcsr) *is_bcsr = m->is_bcsr;
+  if (dtype) *dtype = m->dtype;
+  if (layout) *layout = m->layout;
+  if (n_rows) *n_rows = m->n_rows;
+  if (n_cols) *n_cols = m->n_cols;
+  if (bs) *bs = m->bs;
+  if (nnzb) *nnzb = m->nnzb;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_spmv_traffic(bmfp_matrix m, double *bytes, double *flops)
+{
+  if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
+  const double nv = (double) m->n_vals();
+  const double vb = (double) dtype_size(m->dtype), xb = (double) vec_size(m->dtype);
+  double b = nv * vb + 4.0 * m->nnzb + 4.0 * (m->n_rows + 1.0) + xb * (double) m->x_len() + xb * (double) m->y_len();
+  if (m->dtype == BMFP_F16) b += 4.0 * m->n_rows;
+  if (bytes) *bytes = b;
+  if (flops) *flops = 2.0 * nv;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_download_values(bmfp_matrix m, double *values)
+{
+  if (!m || !values) return fail(BMFP_ERR_INVALID, "NULL argument");
+  const size_t nv = (size_t) m->n_vals();
+  if (!nv) return BMFP_OK;
+  if (m->dtype == BMFP_F64) {
+    BMFP_HIP(hipMemcpy(values, m->values.ptr, nv * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (m->dtype == BMFP_F32) {
+    std::vector<float> f(nv);
+    BMFP_HIP(hipMemcpy(f.data(), m->values.ptr, nv * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nv; i++) values[i] = f[i];
+  } else {
+    std::vector<__half> h(nv);
+    std::vector<float> sc((size_t) m->n_rows);
+    std::vector<int> rp((size_t) m->n_rows + 1);
+    BMFP_HIP(hipMemcpy(h.data(), m->values.ptr, nv * sizeof(__half), hipMemcpyDeviceToHost));
+    BMFP_HIP(hipMemcpy(sc.data(), m->scale.ptr, sc.size() * sizeof(float), hipMemcpyDeviceToHost));
+    BMFP_HIP(hipMemcpy(rp.data(), m->row_ptr->ptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    const size_t bs2 = (size_t) m->bs * m->bs;
+    for (int br = 0; br < m->n_rows; br++)
+      for (size_t e = (size_t) rp[br] * bs2; e < (size_t) rp[br + 1] * bs2; e++)
+        values[e] = (double) __half2float(h[e]) * (double) sc[br];
+  }
+  return BMFP_OK;
+}
+
+// ---- SpMV ---------------------------------------------------------------------------
+static int spmv_dispatch(bmfp_matrix A, int variant, const void *x, void *y, hipStream_t s)
+{
+  if (variant == BMFP_SPMV_AUTO) variant = auto_variant(*A);
+  if (variant == BMFP_BCSR_LIBRARY) return launch_library(*A, x, y, s);
+  if (A->is_bcsr) {
+    if (variant == BMFP_CSR_SCALAR || variant == BMFP_CSR_VECTOR)
+      return fail(BMFP_ERR_INVALID, "CSR variant on a BCSR matrix");
+    return launch_bcsr(*A, variant, x, y, s);
+  }
+  if (variant != BMFP_CSR_SCALAR && variant != BMFP_CSR_VECTOR)
+    return fail(BMFP_ERR_INVALID, "BCSR variant on a CSR matrix");
+  return launch_csr(*A, variant, x, y, s);
+}
+
+extern "C" int bmfp_spmv(bmfp_matrix A, int variant, const void *x, void *y, bmfp_stream stream)
+{
+  if (!A || (A->n_rows && (!x || !y))) return fail(BMFP_ERR_INVALID, "NULL argument");
+  return spmv_dispatch(A, variant, x, y, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y, int warmup, int reps,
+                               bmfp_stream stream, double *avg_seconds)
+{
+  if (!A || !avg_seconds || reps <= 0 || warmup < 0) return fail(BMFP_ERR_INVALID, "bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int i = 0; i < warmup; i++) BMFP_TRY(spmv_dispatch(A, variant, x, y, s));
+  hipEvent_t e0, e1;
+  BMFP_HIP(hipEventCreate(&e0));
+  BMFP_HIP(hipEventCreate(&e1));
+  int st = BMFP_OK;
+  if (hipEventRecord(e0, s) != hipSuccess) st = fail(BMFP_ERR_HIP, "hipEventRecord failed");
+  for (int i = 0; i < reps && st == BMFP_OK; i++) st = spmv_dispatch(A, variant, x, y, s);
+  float ms = 0.f;
+  if (st == BMFP_OK) {
+    hipError_t e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) st = hip_fail(e, "event timing", __FILE__, __LINE__);
+  }
+  (void) hipEventDestroy(e0);
+  (void) hipEventDestroy(e1);
+  if (st == BMFP_OK) *avg_seconds = ms * 1e-3 / reps;
+  return st;
+}
+
+extern "C" int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant)
+{
+  if (!A || !variant) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *variant = auto_variant(*A);
+  return BMFP_OK;
+}

@@ -1,0 +1,287 @@
+// spmv.hip -- instantiation tables and launchers for the SpMV kernels.
+#include <rocsparse/rocsparse.h>
+
+#include <cstdlib>
+
+#include "matrix.hpp"
+#include "spmv_kernels.hpp"
+
+namespace bmfp {
+
+namespace {
+
+// XCD-aware tile order; BMFP_XCD_REMAP=0 switches it off (for measurement).
+int xcd_remap_enabled()
+{
+  static int v = [] {
+    const char *e = std::getenv("BMFP_XCD_REMAP");
+    return e ? std::atoi(e) : 1;
+  }();
+  return v;
+}
+
+struct Grid {
+  unsigned groups;
+  int per;
+};
+
+Grid make_grid(int64_t threads)
+{
+  int64_t g = (threads + 255) / 256;
+  if (g < 1) g = 1;
+  Grid out{(unsigned) g, 0};
+  if (xcd_remap_enabled() && g >= 64) {
+    const int64_t g8 = (g + 7) / 8 * 8;
+    out.groups = (unsigned) g8;
+    out.per = (int) (g8 / 8);
+  }
+  return out;
+}
+
+// Lanes per row / per block row, by block size and value width.
+template <class VT> constexpr int gpr_vec(int bs)
+{
+  constexpr int W = 16 / (int) sizeof(VT);  // elements per 16-byte load
+  int v = 1;
+  while (v < W && bs % (2 * v) == 0) v *= 2;
+  return v;
+}
+constexpr int bgr_lanes(int bs) { return bs == 1 ? 16 : bs <= 3 ? 32 : 64; }
+
+template <class VT, class AT, bool SCALED>
+BcsrArgs<VT, AT> bcsr_args(const bmfp_matrix_t &A, const void *x, void *y, int per)
+{
+  BcsrArgs<VT, AT> a;
+  a.n_brows = A.n_rows;
+  a.xcd_per = per;
+  a.rp = A.row_ptr->as<int>();
+  a.cols = A.columns->as<int>();
+  a.vals = A.values.as<VT>();
+  a.scale = SCALED ? A.scale.as<float>() : nullptr;
+  a.x = static_cast<const AT *>(x);
+  a.y = static_cast<AT *>(y);
+  return a;
+}
+
+template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
+int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  const int64_t nb = A.n_rows;
+  switch (variant) {
+  case BMFP_BCSR_THREAD_PER_ROW: {
+    Grid g = make_grid(nb * BS);
+    k_bcsr_tpr<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    break;
+  }
+  case BMFP_BCSR_GROUP_PER_ROW: {
+    constexpr int VEC = gpr_vec<VT>(BS);
+    constexpr int G = pow2ceil(BS / VEC);
+    Grid g = make_grid(nb * BS * G);
+    k_bcsr_gpr<VT, AT, BS, VEC, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    break;
+  }
+  case BMFP_BCSR_SHUFFLE_X: {
+    constexpr int RPW = 64 / BS;
+    Grid g = make_grid((nb + RPW - 1) / RPW * 64);
+    k_bcsr_shfx<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    break;
+  }
+  case BMFP_BCSR_BLOCK_ROW_GROUP: {
+    if constexpr (LAYOUT != BMFP_COL_MAJOR)
+      return fail(BMFP_ERR_UNSUPPORTED, "BMFP_BCSR_BLOCK_ROW_GROUP needs column-major blocks");
+    else {
+      constexpr int GL = bgr_lanes(BS);
+      Grid g = make_grid(nb * GL);
+      k_bcsr_bgr<VT, AT, BS, GL, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    }
+    break;
+  }
+  default:
+    return fail(BMFP_ERR_INVALID, "variant %d is not a BCSR kernel family", variant);
+  }
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+template <class VT, class AT, bool SCALED, int LAYOUT>
+int launch_runtime_bs(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  if (variant != BMFP_BCSR_THREAD_PER_ROW)
+    return fail(BMFP_ERR_UNSUPPORTED, "block size %d has no precompiled kernel for variant %d", A.bs, variant);
+  Grid g = make_grid((int64_t) A.n_rows * A.bs);
+  k_bcsr_tpr_rt<VT, AT, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per), A.bs);
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+template <class VT, class AT, bool SCALED, int LAYOUT>
+int launch_layout(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+#define BMFP_BS_CASE(B) \
+  case B: return launch_fixed<VT, AT, SCALED, B, LAYOUT>(A, variant, x, y, s);
+  switch (A.bs) {
+    BMFP_BS_CASE(1)
+    BMFP_BS_CASE(2)
+    BMFP_BS_CASE(3)
+    BMFP_BS_CASE(4)
+    BMFP_BS_CASE(6)
+    BMFP_BS_CASE(8)
+    BMFP_BS_CASE(12)
+    BMFP_BS_CASE(16)
+    BMFP_BS_CASE(24)
+    BMFP_BS_CASE(32)
+  default:
+    return launch_runtime_bs<VT, AT, SCALED, LAYOUT>(A, variant, x, y, s);
+  }
+#undef BMFP_BS_CASE
+}
+
+template <class VT, class AT, bool SCALED>
+int launch_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  return A.layout == BMFP_ROW_MAJOR ? launch_layout<VT, AT, SCALED, BMFP_ROW_MAJOR>(A, variant, x, y, s)
+                                    : launch_layout<VT, AT, SCALED, BMFP_COL_MAJOR>(A, variant, x, y, s);
+}
+
+bool precompiled_bs(int bs)
+{
+  switch (bs) {
+  case 1: case 2: case 3: case 4: case 6: case 8: case 12: case 16: case 24: case 32: return true;
+  default: return false;
+  }
+}
+
+}  // namespace
+
+bool bcsr_supported(int variant, int dtype, int layout, int bs)
+{
+  if (bs < 1) return false;
+  if (variant == BMFP_BCSR_LIBRARY) return dtype != BMFP_F16;
+  if (!precompiled_bs(bs)) return variant == BMFP_BCSR_THREAD_PER_ROW;
+  if (variant == BMFP_BCSR_BLOCK_ROW_GROUP) return layout == BMFP_COL_MAJOR;
+  return variant == BMFP_BCSR_THREAD_PER_ROW || variant == BMFP_BCSR_GROUP_PER_ROW ||
+         variant == BMFP_BCSR_SHUFFLE_X;
+}
+
+int auto_variant(const bmfp_matrix_t &A)
+{
+  if (!A.is_bcsr) return A.mean_row_blocks >= 4.0 ? BMFP_CSR_VECTOR : BMFP_CSR_SCALAR;
+  if (!precompiled_bs(A.bs)) return BMFP_BCSR_THREAD_PER_ROW;
+  if (A.layout == BMFP_COL_MAJOR) return A.bs >= 2 ? BMFP_BCSR_BLOCK_ROW_GROUP : BMFP_BCSR_THREAD_PER_ROW;
+  return A.bs >= 4 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_THREAD_PER_ROW;
+}
+
+int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  if (A.n_rows == 0) return BMFP_OK;
+  if (!bcsr_supported(variant, A.dtype, A.layout, A.bs))
+    return fail(BMFP_ERR_UNSUPPORTED, "BCSR variant %d unsupported for dtype %d layout %d bs %d",
+                variant, A.dtype, A.layout, A.bs);
+  switch (A.dtype) {
+  case BMFP_F64: return launch_typed<double, double, false>(A, variant, x, y, s);
+  case BMFP_F32: return launch_typed<float, float, false>(A, variant, x, y, s);
+  case BMFP_F16: return launch_typed<__half, float, true>(A, variant, x, y, s);
+  default: return fail(BMFP_ERR_INVALID, "bad dtype %d", A.dtype);
+  }
+}
+
+// Lanes per row for the CSR lane-group kernel: the power of two nearest half
+// the mean row length, clamped to [2, 64].
+int csr_group_for(double mean_len)
+{
+  int g = 2;
+  while (g < 64 && 2.0 * g < mean_len) g *= 2;
+  return g;
+}
+
+template <class VT, class AT>
+int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  CsrArgs<VT, AT> a;
+  a.n_rows = A.n_rows;
+  a.rp = A.row_ptr->as<int>();
+  a.cols = A.columns->as<int>();
+  a.vals = A.values.as<VT>();
+  a.x = static_cast<const AT *>(x);
+  a.y = static_cast<AT *>(y);
+  if (variant == BMFP_CSR_SCALAR) {
+    Grid g = make_grid(A.n_rows);
+    a.xcd_per = g.per;
+    k_csr_scalar<VT, AT><<<g.groups, 256, 0, s>>>(a);
+  } else if (variant == BMFP_CSR_VECTOR) {
+    const int G = A.csr_group;
+    Grid g = make_grid((int64_t) A.n_rows * G);
+    a.xcd_per = g.per;
+    switch (G) {
+    case 2: k_csr_vector<VT, AT, 2><<<g.groups, 256, 0, s>>>(a); break;
+    case 4: k_csr_vector<VT, AT, 4><<<g.groups, 256, 0, s>>>(a); break;
+    case 8: k_csr_vector<VT, AT, 8><<<g.groups, 256, 0, s>>>(a); break;
+    case 16: k_csr_vector<VT, AT, 16><<<g.groups, 256, 0, s>>>(a); break;
+    case 32: k_csr_vector<VT, AT, 32><<<g.groups, 256, 0, s>>>(a); break;
+    default: k_csr_vector<VT, AT, 64><<<g.groups, 256, 0, s>>>(a); break;
+    }
+  } else {
+    return fail(BMFP_ERR_INVALID, "variant %d is not a CSR kernel family", variant);
+  }
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+int launch_csr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
+{
+  if (A.n_rows == 0) return BMFP_OK;
+  switch (A.dtype) {
+  case BMFP_F64: return launch_csr_typed<double, double>(A, variant, x, y, s);
+  case BMFP_F32: return launch_csr_typed<float, float>(A, variant, x, y, s);
+  default: return fail(BMFP_ERR_UNSUPPORTED, "CSR SpMV supports F64 and F32 storage");
+  }
+}
+
+// ---- rocSPARSE bsrmv: the library baseline row (reference: cuSPARSE) -------
+#define BMFP_RS(call)                                                                  \
+  do {                                                                                 \
+    rocsparse_status st_ = (call);                                                     \
+    if (st_ != rocsparse_status_success)                                               \
+      return fail(BMFP_ERR_HIP, "%s failed: rocsparse status %d", #call, (int) st_);   \
+  } while (0)
+
+int launch_library(bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
+{
+  if (!A.is_bcsr || A.dtype == BMFP_F16)
+    return fail(BMFP_ERR_UNSUPPORTED, "library bsrmv needs an F64/F32 BCSR matrix");
+  if (A.n_rows == 0) return BMFP_OK;
+  rocsparse_handle h = static_cast<rocsparse_handle>(A.lib_handle);
+  if (!h) {
+    BMFP_RS(rocsparse_create_handle(&h));
+    A.lib_handle = h;
+    rocsparse_mat_descr d;
+    BMFP_RS(rocsparse_create_mat_descr(&d));
+    A.lib_descr = d;
+  }
+  BMFP_RS(rocsparse_set_stream(h, s));
+  rocsparse_mat_descr d = static_cast<rocsparse_mat_descr>(A.lib_descr);
+  const rocsparse_direction dir =
+      A.layout == BMFP_ROW_MAJOR ? rocsparse_direction_row : rocsparse_direction_column;
+  if (A.dtype == BMFP_F64) {
+    const double one = 1.0, zero = 0.0;
+    BMFP_RS(rocsparse_dbsrmv(h, dir, rocsparse_operation_none, A.n_rows, A.n_cols, A.nnzb, &one, d,
+                             A.values.as<double>(), A.row_ptr->as<int>(), A.columns->as<int>(), A.bs,
+                             nullptr, static_cast<const double *>(x), &zero, static_cast<double *>(y)));
+  } else {
+    const float one = 1.0f, zero = 0.0f;
+    BMFP_RS(rocsparse_sbsrmv(h, dir, rocsparse_operation_none, A.n_rows, A.n_cols, A.nnzb, &one, d,
+                             A.values.as<float>(), A.row_ptr->as<int>(), A.columns->as<int>(), A.bs,
+                             nullptr, static_cast<const float *>(x), &zero, static_cast<float *>(y)));
+  }
+  return BMFP_OK;
+}
+
+void destroy_library(bmfp_matrix_t &A)
+{
+  if (A.lib_descr) rocsparse_destroy_mat_descr(static_cast<rocsparse_mat_descr>(A.lib_descr));
+  if (A.lib_handle) rocsparse_destroy_handle(static_cast<rocsparse_handle>(A.lib_handle));
+  A.lib_descr = nullptr;
+  A.lib_handle = nullptr;
+}
+
+}  // namespace bmfp

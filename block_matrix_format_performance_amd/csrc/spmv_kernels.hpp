@@ -1,0 +1,349 @@
+// spmv_kernels.hpp -- gfx950 SpMV kernels (BCSR and CSR), templated on the
+// value storage type VT, the vector/accumulator type AT and the block size BS.
+//
+// All kernels are HBM-bandwidth bound (0.14-0.25 flop/byte in FP64), so the
+// design goals are: every value byte read exactly once with wide (up to
+// 16-byte) loads, x re-reads served from L2/MALL, no atomics (results are
+// bitwise reproducible run to run), 64-lane wavefront reductions done with
+// cross-lane shuffles in a fixed order.
+//
+// Mapping of the reference's CUDA kernels (gpu_matrix_multiplier.cu) onto the
+// families here:
+//   k_bcsr_tpr   thread per scalar row       K4 :275-304, K8 :341-370, K9 :372-400
+//   k_bcsr_gpr   lane group per scalar row   K5 :306-339 (32 fixed lanes there)
+//   k_bcsr_shfx  thread per row, x segment   K10/K11 :402-469 (there: shared memory,
+//                shared by lane shuffles      wrong for bs<32; here: per block row)
+//   k_bcsr_bgr   lane group per block row    K12-K16 :485-673 (there: wrong for
+//                (column-major streaming)    bs in {3,5,6,9,10}; here: exact)
+//   k_csr_scalar thread per row              K2 :65-86
+//   k_csr_vector lane group per row          K3 :165-196
+#pragma once
+
+#include "common.hpp"
+
+namespace bmfp {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// ---- element conversions --------------------------------------------------
+template <class AT, class VT> __device__ __forceinline__ AT cvt(VT v) { return (AT) v; }
+template <> __device__ __forceinline__ float cvt<float, __half>(__half v) { return __half2float(v); }
+
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// ---- wide loads -------------------------------------------------------------
+constexpr int lowbit(int v) { return v & -v; }
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int pow2ceil(int v) { int p = 1; while (p < v) p *= 2; return p; }
+
+// Loads N consecutive VT from p, whose byte address is known to be a
+// multiple of ALIGN, in the widest chunks (<= 16 bytes) that alignment and
+// size allow.
+template <class VT, int N, int ALIGN>
+__device__ __forceinline__ void load_n(const VT *__restrict__ p, VT (&out)[N])
+{
+  constexpr int BYTES = N * (int) sizeof(VT);
+  constexpr int CH = cmin(cmin(ALIGN, 16), lowbit(BYTES));
+  const char *src = reinterpret_cast<const char *>(p);
+  char *dst = reinterpret_cast<char *>(out);
+  if constexpr (CH == 16) {
+#pragma unroll
+    for (int i = 0; i < BYTES / 16; ++i) {
+      u32x4 v = *reinterpret_cast<const u32x4 *>(src + 16 * i);
+      __builtin_memcpy(dst + 16 * i, &v, 16);
+    }
+  } else if constexpr (CH == 8) {
+#pragma unroll
+    for (int i = 0; i < BYTES / 8; ++i) {
+      u32x2 v = *reinterpret_cast<const u32x2 *>(src + 8 * i);
+      __builtin_memcpy(dst + 8 * i, &v, 8);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = p[i];
+  }
+}
+
+// Maps the hardware workgroup id so that each XCD (workgroups are dealt to
+// the 8 XCDs round robin) processes one contiguous run of tiles: neighbouring
+// block rows then share x segments, column indices and row pointers in the
+// same L2.  `per` = launched groups / 8 (launch a multiple of 8); 0 = off.
+__device__ __forceinline__ int64_t tile_id(int per)
+{
+  const int64_t i = blockIdx.x;
+  if (per == 0) return i;
+  return (i & 7) * (int64_t) per + (i >> 3);
+}
+
+// ---- shared launch arguments --------------------------------------------------
+template <class VT, class AT>
+struct BcsrArgs {
+  int n_brows;
+  int xcd_per;
+  const int *__restrict__ rp;
+  const int *__restrict__ cols;
+  const VT *__restrict__ vals;
+  const float *__restrict__ scale;   // per block row, F16 storage only
+  const AT *__restrict__ x;
+  AT *__restrict__ y;
+};
+
+// Chunk of columns processed per step in the thread-per-row kernels: keeps
+// the register footprint bounded for large BS.
+template <int BS> constexpr int col_chunk() { return BS % 8 == 0 ? 8 : BS; }
+
+// ============================================================================
+// Thread per scalar row.  Row-major: the thread's row of each block is BS
+// contiguous values, loaded in 16-byte pieces; the BS threads of a block row
+// together read the whole block contiguously.  Column-major: for each column
+// c the BS threads of a block row read one contiguous column.
+// ============================================================================
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_tpr(BcsrArgs<VT, AT> a)
+{
+  const int64_t t = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  const int64_t br64 = t / BS;
+  if (br64 >= a.n_brows) return;
+  const int br = (int) br64;
+  const int r = (int) (t - br64 * BS);
+  const int b0 = a.rp[br], b1 = a.rp[br + 1];
+  constexpr int CC = col_chunk<BS>();
+  constexpr int XALIGN = lowbit((int) sizeof(AT) * BS);
+  constexpr int VALIGN = lowbit((int) sizeof(VT) * BS);
+  AT acc = 0;
+#pragma unroll 3
+  for (int b = b0; b < b1; ++b) {
+    const AT *xs = a.x + (int64_t) a.cols[b] * BS;
+    const VT *blk = a.vals + (int64_t) b * (BS * BS);
+#pragma unroll
+    for (int c0 = 0; c0 < BS; c0 += CC) {
+      AT xv[CC];
+      load_n<AT, CC, cmin(XALIGN, lowbit((int) sizeof(AT) * CC))>(xs + c0, xv);
+      if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+        VT v[CC];
+        load_n<VT, CC, cmin(VALIGN, lowbit((int) sizeof(VT) * CC))>(blk + r * BS + c0, v);
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(blk[(c0 + c) * BS + r]), xv[c], acc);
+      }
+    }
+  }
+  if constexpr (SCALED) acc *= (AT) a.scale[br];
+  a.y[t] = acc;
+}
+
+// ============================================================================
+// Lane group per scalar row.  G = pow2ceil(BS/VEC) lanes; lane g < BS/VEC
+// owns columns [g*VEC, g*VEC+VEC) of every block in the row.  Partial sums
+// are combined by a butterfly over the G lanes.
+// ============================================================================
+template <class VT, class AT, int BS, int VEC, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_gpr(BcsrArgs<VT, AT> a)
+{
+  constexpr int GA = BS / VEC;
+  constexpr int G = pow2ceil(GA);
+  static_assert(BS % VEC == 0 && G <= 64, "bad group shape");
+  const int64_t t = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  const int g = (int) (t % G);
+  const int64_t row = t / G;
+  const int64_t br64 = row / BS;
+  if (br64 >= a.n_brows) return;  // whole groups leave together
+  const int br = (int) br64;
+  const int r = (int) (row - br64 * BS);
+  const int b0 = a.rp[br], b1 = a.rp[br + 1];
+  constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * BS), lowbit((int) sizeof(AT) * VEC));
+  constexpr int VALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
+  AT acc = 0;
+  if (g < GA) {
+#pragma unroll 3
+    for (int b = b0; b < b1; ++b) {
+      const AT *xs = a.x + (int64_t) a.cols[b] * BS + g * VEC;
+      const VT *blk = a.vals + (int64_t) b * (BS * BS);
+      AT xv[VEC];
+      load_n<AT, VEC, XALIGN>(xs, xv);
+      if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+        VT v[VEC];
+        load_n<VT, VEC, VALIGN>(blk + r * BS + g * VEC, v);
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+      } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(blk[(g * VEC + c) * BS + r]), xv[c], acc);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
+  if (g == 0) {
+    if constexpr (SCALED) acc *= (AT) a.scale[br];
+    a.y[row] = acc;
+  }
+}
+
+// ============================================================================
+// Thread per scalar row with the x segment shared by shuffles: the BS lanes
+// of a block row each load one element of x[col*BS .. col*BS+BS) (one
+// coalesced load instead of BS broadcast loads per lane) and read the others
+// from their neighbours' registers.  RPW = 64/BS block rows per wavefront;
+// the 64 - RPW*BS spare lanes idle.
+// ============================================================================
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_shfx(BcsrArgs<VT, AT> a)
+{
+  constexpr int RPW = 64 / BS;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int lbr = lane / BS;
+  const int r = lane - lbr * BS;
+  const int64_t br64 = wave * RPW + lbr;
+  if (lane >= RPW * BS || br64 >= a.n_brows) return;  // whole block rows leave together
+  const int br = (int) br64;
+  const int b0 = a.rp[br], b1 = a.rp[br + 1];
+  const int base = lbr * BS;
+  constexpr int VALIGN = lowbit((int) sizeof(VT) * BS);
+  constexpr int CC = col_chunk<BS>();
+  AT acc = 0;
+  for (int b = b0; b < b1; ++b) {
+    const AT xr = a.x[(int64_t) a.cols[b] * BS + r];
+    const VT *blk = a.vals + (int64_t) b * (BS * BS);
+#pragma unroll
+    for (int c0 = 0; c0 < BS; c0 += CC) {
+      if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+        VT v[CC];
+        load_n<VT, CC, cmin(VALIGN, lowbit((int) sizeof(VT) * CC))>(blk + r * BS + c0, v);
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(v[c]), __shfl(xr, base + c0 + c), acc);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CC; ++c)
+          acc = fmadd(cvt<AT>(blk[(c0 + c) * BS + r]), __shfl(xr, base + c0 + c), acc);
+      }
+    }
+  }
+  if constexpr (SCALED) acc *= (AT) a.scale[br];
+  a.y[(int64_t) br * BS + r] = acc;
+}
+
+// ============================================================================
+// Lane group per block row, column-major blocks.  The block row's values are
+// one contiguous run of nblk*BS*BS elements; GL lanes stream it in strides of
+// L = (GL/BS)*BS elements (one coalesced load per lane per step).  Because L
+// is a multiple of BS, lane j always meets row r = j % BS; the M = GL/BS
+// lanes of each row are then summed by a fixed-order shuffle tree.  This is
+// the reference's "column-by-column" kernel with the lane accounting made
+// exact for every BS (the reference folds duplicate columns in for BS that
+// do not divide 32).
+// ============================================================================
+template <class VT, class AT, int BS, int GL, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_bgr(BcsrArgs<VT, AT> a)
+{
+  static_assert(GL == 16 || GL == 32 || GL == 64, "GL");
+  static_assert(BS <= GL, "BS > GL");
+  constexpr int M = GL / BS;
+  constexpr int L = M * BS;
+  constexpr int BS2 = BS * BS;
+  const int64_t t = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  const int j = (int) (t % GL);
+  const int64_t br64 = t / GL;
+  if (br64 >= a.n_brows) return;  // whole groups leave together
+  const int br = (int) br64;
+  const int b0 = a.rp[br], b1 = a.rp[br + 1];
+  const int nelem = (b1 - b0) * BS2;
+  const VT *vals = a.vals + (int64_t) b0 * BS2;
+  const int *cols = a.cols + b0;
+  AT acc = 0;
+  if (j < L) {
+#pragma unroll 4
+    for (int e = j; e < nelem; e += L) {
+      const int k = e / BS2;
+      const int c = (e - k * BS2) / BS;
+      acc = fmadd(cvt<AT>(vals[e]), a.x[(int64_t) cols[k] * BS + c], acc);
+    }
+  }
+  // lanes r, r+BS, ..., r+(M-1)*BS hold row r's partial sums
+  const int q = j / BS;
+#pragma unroll
+  for (int s = 1; s < M; s *= 2) {
+    const AT o = __shfl_down(acc, s * BS, GL);
+    if ((q % (2 * s)) == 0 && q + s < M) acc += o;
+  }
+  if (j < BS) {
+    if constexpr (SCALED) acc *= (AT) a.scale[br];
+    a.y[(int64_t) br * BS + j] = acc;
+  }
+}
+
+// ============================================================================
+// Runtime block size (any bs), thread per scalar row; used for block sizes
+// without a precompiled instantiation.
+// ============================================================================
+template <class VT, class AT, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_tpr_rt(BcsrArgs<VT, AT> a, int bs)
+{
+  const int64_t t = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  const int64_t br64 = t / bs;
+  if (br64 >= a.n_brows) return;
+  const int br = (int) br64;
+  const int r = (int) (t - br64 * bs);
+  const int b0 = a.rp[br], b1 = a.rp[br + 1];
+  const int64_t bs2 = (int64_t) bs * bs;
+  AT acc = 0;
+  for (int b = b0; b < b1; ++b) {
+    const AT *xs = a.x + (int64_t) a.cols[b] * bs;
+    const VT *blk = a.vals + (int64_t) b * bs2;
+    for (int c = 0; c < bs; ++c) {
+      const VT v = LAYOUT == BMFP_ROW_MAJOR ? blk[r * bs + c] : blk[c * bs + r];
+      acc = fmadd(cvt<AT>(v), xs[c], acc);
+    }
+  }
+  if constexpr (SCALED) acc *= (AT) a.scale[br];
+  a.y[t] = acc;
+}
+
+// ============================================================================
+// CSR.
+// ============================================================================
+template <class VT, class AT>
+struct CsrArgs {
+  int n_rows;
+  int xcd_per;
+  const int *__restrict__ rp;
+  const int *__restrict__ cols;
+  const VT *__restrict__ vals;
+  const AT *__restrict__ x;
+  AT *__restrict__ y;
+};
+
+template <class VT, class AT>
+__global__ __launch_bounds__(256) void k_csr_scalar(CsrArgs<VT, AT> a)
+{
+  const int64_t row = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  if (row >= a.n_rows) return;
+  const int e0 = a.rp[row], e1 = a.rp[row + 1];
+  AT acc = 0;
+#pragma unroll 4
+  for (int e = e0; e < e1; ++e) acc = fmadd(cvt<AT>(a.vals[e]), a.x[a.cols[e]], acc);
+  a.y[row] = acc;
+}
+
+template <class VT, class AT, int G>
+__global__ __launch_bounds__(256) void k_csr_vector(CsrArgs<VT, AT> a)
+{
+  const int64_t t = tile_id(a.xcd_per) * 256 + threadIdx.x;
+  const int g = (int) (t % G);
+  const int64_t row = t / G;
+  if (row >= a.n_rows) return;  // whole groups leave together
+  const int e0 = a.rp[row], e1 = a.rp[row + 1];
+  AT acc = 0;
+#pragma unroll 4
+  for (int e = e0 + g; e < e1; e += G) acc = fmadd(cvt<AT>(a.vals[e]), a.x[a.cols[e]], acc);
+#pragma unroll
+  for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
+  if (g == 0) a.y[row] = acc;
+}
+
+}  // namespace bmfp

@@ -1,0 +1,171 @@
+/*
+ * bmfp.h -- C ABI of the MI355X (gfx950) block-sparse SpMV / BiCGStab library.
+ *
+ * The reference exposes C++ templates with CUDA inside them; this header is the
+ * plain-C boundary underneath the drop-in C++ headers in include/bmfp/.  Each
+ * entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: BMFP_OK (0) or a negative code.
+ *     No exception crosses the ABI and nothing calls exit().
+ *     bmfp_last_error() returns the calling thread's last error text.
+ *   - Device pointers are caller-owned unless a function says otherwise.
+ *   - Calls taking a bmfp_stream are ordered on that stream (NULL = the
+ *     device's null stream) and return without synchronising unless noted.
+ *   - Indices are int32 (the reference's index_type = int); n_rows, nnz and
+ *     nnzb*bs*bs must fit in int32.
+ */
+#ifndef BMFP_H
+#define BMFP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMFP_VERSION 100
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+  BMFP_OK = 0,
+  BMFP_ERR_INVALID = -1,        /* bad argument / shape                      */
+  BMFP_ERR_UNSUPPORTED = -2,    /* e.g. a block size with no kernel          */
+  BMFP_ERR_NO_MEMORY = -3,      /* device allocation failed                  */
+  BMFP_ERR_BREAKDOWN = -4,      /* BiCGStab breakdown ((r^,v)=0, rho=0, ...) */
+  BMFP_ERR_NOT_CONVERGED = -5,  /* max_iterations reached                    */
+  BMFP_ERR_HIP = -6,            /* HIP runtime error                         */
+  BMFP_ERR_NONFINITE = -7,      /* NaN/Inf in inputs or iterates             */
+  BMFP_ERR_BUDGET = -8,         /* memory-budget admission refused           */
+  BMFP_ERR_NO_DEVICE = -9       /* no usable gfx950 device                   */
+};
+
+/* ---- element types and layouts ----------------------------------------- */
+enum { BMFP_F64 = 0, BMFP_F32 = 1, BMFP_F16 = 2 };
+enum { BMFP_ROW_MAJOR = 0, BMFP_COL_MAJOR = 1 };
+
+/* ---- SpMV kernel families (reference kernel names in brackets) ---------- */
+enum {
+  BMFP_SPMV_AUTO = 0,            /* best family for (format, layout, bs)             */
+  BMFP_CSR_SCALAR = 1,           /* [csr_spmv_kernel] gpu_matrix_multiplier.cu:65-86  */
+  BMFP_CSR_VECTOR = 2,           /* [csr_spmv_vector_kernel] :165-196; lane groups    */
+                                 /*  of 2..64 lanes chosen from the mean row length   */
+  BMFP_BCSR_THREAD_PER_ROW = 3,  /* [..._thread_per_row_{row,column}_major_matrix     */
+                                 /*  (_template)] :275-304, :341-400                  */
+  BMFP_BCSR_GROUP_PER_ROW = 4,   /* [..._warp_per_row_row_major_matrix] :306-339      */
+  BMFP_BCSR_SHUFFLE_X = 5,       /* [..._coal_x(_template)] :402-469, x segments      */
+                                 /*  shared across a block row by lane shuffles       */
+  BMFP_BCSR_BLOCK_ROW_GROUP = 6, /* [..._column_by_column(_fastdiv,_po2,_template)]   */
+                                 /*  :485-673, lane group per block row               */
+  BMFP_BCSR_LIBRARY = 7          /* [cusparse?bsrmv] :676-734 -> rocSPARSE bsrmv      */
+};
+
+typedef struct bmfp_matrix_t *bmfp_matrix;
+typedef struct bmfp_solver_t *bmfp_solver;
+typedef void *bmfp_stream;     /* a hipStream_t */
+
+/* ---- diagnostics --------------------------------------------------------- */
+const char *bmfp_last_error(void);
+const char *bmfp_status_string(int status);
+int bmfp_version(void);
+
+/* ---- device, memory, streams -------------------------------------------
+ * Replace the raw cudaSetDevice / cudaMalloc / cudaMemcpy / cudaFree /
+ * cudaDeviceSynchronize calls made inline by main.cpp:388 and
+ * gpu_matrix_multiplier.cu:88-163 (and every other wrapper). */
+int bmfp_init(int device);
+int bmfp_device_count(int *count);
+int bmfp_device_info(char *name, size_t name_len, size_t *total_mem_bytes,
+                     int *compute_units, int *gcn_arch_major_minor);
+int bmfp_malloc(void **ptr, size_t bytes);
+int bmfp_free(void *ptr);
+int bmfp_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int bmfp_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int bmfp_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int bmfp_memset(void *dst, int value, size_t bytes);
+int bmfp_stream_create(bmfp_stream *stream);
+int bmfp_stream_destroy(bmfp_stream stream);
+int bmfp_stream_synchronize(bmfp_stream stream);
+int bmfp_device_synchronize(void);
+/* Timing events on a stream (hipEvent_t underneath).  bmfp_event_elapsed
+ * waits for `stop` and returns the seconds between the two records. */
+typedef void *bmfp_event;
+int bmfp_event_create(bmfp_event *event);
+int bmfp_event_destroy(bmfp_event event);
+int bmfp_event_record(bmfp_event event, bmfp_stream stream);
+int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *seconds);
+/* Deterministic device fill with the library's counter-based RNG:
+ * kind 0 uniform[0,1), kind 1 uniform[-1,1), kind 2 constant `value`.
+ * Element i of a vector filled with (seed, stream_id) is identical to the
+ * host generator's element i.  dtype is BMFP_F64 or BMFP_F32. */
+int bmfp_fill(void *dst, int dtype, size_t n, int kind, uint64_t seed,
+              uint64_t stream_id, double value, bmfp_stream stream);
+
+/* ---- host-side synthetic matrices ---------------------------------------
+ * gen_n_diag_bcsr: matrix_converters.h:157-196 (values double, row-major).
+ * Arrays: row_ptr[n_rows+1], columns[n_rows*blocks_per_row],
+ * values[n_rows*blocks_per_row*bs*bs]. */
+int bmfp_gen_n_diag_bcsr(int n_rows, int blocks_per_row, int bs,
+                         int *row_ptr, int *columns, double *values);
+/* 9-point block stencil on an nx*ny node grid (no reference counterpart:
+ * the synthetic workloads' generator).  kind 0: diagonally dominant,
+ * nonsymmetric; kind 1: convective -(K+0.3C) off-diagonals.  Values are
+ * row-major inside blocks.  nnzb = bmfp_stencil_nnzb(nx, ny). */
+int64_t bmfp_stencil_nnzb(int nx, int ny);
+int bmfp_gen_stencil9(int nx, int ny, int bs, int kind, uint64_t seed,
+                      int *row_ptr, int *columns, double *values);
+
+/* ---- device-resident matrices -------------------------------------------
+ * Replace the per-call cudaMalloc + cudaMemcpy of matrix arrays inside
+ * gpu_csr_spmv / gpu_csr_vector_spmv / gpu_bcsr_spmv
+ * (gpu_matrix_multiplier.cu:88-163, :198-273, :736-770) and the uploads in
+ * gpu_bicgstab::solve (bicgstab.cu:403-405).  The handle owns its device
+ * copies; host arrays can be freed after the call returns.
+ *
+ * bmfp_bcsr_create: `values` are host doubles in `layout`; `dtype` is the
+ * device storage type.  BMFP_F16 stores each block row scaled by one FP32
+ * factor (max |entry| of the block row maps to 1.0). */
+int bmfp_bcsr_create(bmfp_matrix *out, int dtype, int layout, int n_rows,
+                     int n_cols, int bs, int nnzb, const int *row_ptr,
+                     const int *columns, const double *values);
+int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_cols,
+                    int nnz, const int *row_ptr, const int *columns,
+                    const double *values);
+/* Device-side copy of a BCSR matrix into another storage type / layout
+ * (indices shared by reference count).  Used for mixed precision. */
+int bmfp_matrix_convert(bmfp_matrix src, int dtype, int layout,
+                        bmfp_matrix *out, bmfp_stream stream);
+int bmfp_matrix_destroy(bmfp_matrix m);
+/* Shape query: any pointer may be NULL.  For CSR, bs = 1 and nnzb = nnz. */
+int bmfp_matrix_shape(bmfp_matrix m, int *is_bcsr, int *dtype, int *layout,
+                      int *n_rows, int *n_cols, int *bs, int *nnzb);
+/* Compulsory HBM bytes and flops of one SpMV on this matrix: values once,
+ * indices once, x once, y once (never the reference's CPU model, which
+ * counts x per nonzero). */
+int bmfp_spmv_traffic(bmfp_matrix m, double *bytes, double *flops);
+/* Download values (as doubles, in the handle's layout, unscaled). */
+int bmfp_matrix_download_values(bmfp_matrix m, double *values);
+
+/* ---- SpMV ----------------------------------------------------------------
+ * y = A x.  x, y are device vectors of n_cols*bs / n_rows*bs elements in the
+ * vector type of A (F64 for F64 matrices, F32 for F32 and F16 matrices).
+ * variant: BMFP_SPMV_AUTO or a family valid for A's format
+ * (CSR: BMFP_CSR_*; BCSR: BMFP_BCSR_*).  Replaces the kernel launches of
+ * gpu_csr_spmv (:132-139), gpu_csr_vector_spmv (:242-249) and the 12 BCSR
+ * variants of gpu_bcsr_spmv (:784-1305). */
+int bmfp_spmv(bmfp_matrix A, int variant, const void *x, void *y,
+              bmfp_stream stream);
+/* `warmup` untimed, then `reps` SpMVs between two hipEvents recorded on
+ * `stream`; *avg_seconds = elapsed / reps.  Replaces the single-launch
+ * cudaEvent timing in every reference wrapper (e.g. :125-149). */
+int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y,
+                    int warmup, int reps, bmfp_stream stream,
+                    double *avg_seconds);
+/* The family BMFP_SPMV_AUTO resolves to for A. */
+int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

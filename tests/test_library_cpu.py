@@ -1,0 +1,87 @@
+"""CPU-side checks of the product library (no GPU needed): it loads, exports
+every symbol include/bmfp.h declares, its host generators agree bit-exactly
+with the oracle, and argument errors come back as status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import block_matrix_format_performance_amd as bm
+import oracle
+
+
+def declared_symbols():
+    with open(bm.HEADER_PATH) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bmfp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = bm.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert missing == []
+
+
+def test_cpp_layer_exports():
+    if not os.path.exists(bm.CPP_LIB_PATH):
+        pytest.skip("C++ layer not built")
+    L = ctypes.CDLL(bm.CPP_LIB_PATH)
+    assert hasattr(L, "bmfp_cpp_abi_version")
+
+
+def test_status_strings_and_version():
+    L = bm.lib()
+    assert L.bmfp_version() == 100
+    assert L.bmfp_status_string(bm.OK) == b"ok"
+    assert L.bmfp_status_string(bm.ERR_BREAKDOWN) == b"solver breakdown"
+    assert L.bmfp_status_string(-1234) == b"unknown status"
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("nx,ny,bs", [(1, 1, 1), (1, 6, 2), (5, 1, 3), (7, 5, 4), (4, 4, 6), (3, 2, 8), (9, 8, 2)])
+def test_stencil_generator_matches_oracle_bitwise(kind, nx, ny, bs):
+    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, kind, seed=kind + 7)
+    m = oracle.gen_stencil9(nx, ny, bs, kind, kind + 7)
+    np.testing.assert_array_equal(rp, m.row_ptr)
+    np.testing.assert_array_equal(cols, m.columns)
+    assert np.array_equal(vals.view(np.uint64), m.values.view(np.uint64))
+
+
+def test_stencil_generator_multithreaded_matches_oracle():
+    # large enough to take the threaded path
+    rp, cols, vals = bm.gen_stencil9(61, 47, 12, 0, seed=3)
+    m = oracle.gen_stencil9(61, 47, 12, 0, 3)
+    assert np.array_equal(vals.view(np.uint64), m.values.view(np.uint64))
+    np.testing.assert_array_equal(cols, m.columns)
+
+
+def test_n_diag_generator_matches_oracle():
+    for N, bpr, bs in [(7, 3, 1), (50, 6, 4), (9, 6, 3)]:
+        rp, cols, vals = bm.gen_n_diag_bcsr(N, bpr, bs)
+        m = oracle.gen_n_diag_bcsr(N, bpr, bs)
+        np.testing.assert_array_equal(rp, m.row_ptr)
+        np.testing.assert_array_equal(cols, m.columns)
+        assert np.array_equal(vals, m.values)
+
+
+def test_generator_argument_errors():
+    L = bm.lib()
+    z = np.zeros(16, np.int32)
+    v = np.zeros(16)
+    st = L.bmfp_gen_stencil9(0, 3, 2, 0, 0, z.ctypes.data, z.ctypes.data, v.ctypes.data)
+    assert st == bm.ERR_INVALID
+    assert b"gen_stencil9" in L.bmfp_last_error()
+    st = L.bmfp_gen_stencil9(3, 3, 2, 5, 0, z.ctypes.data, z.ctypes.data, v.ctypes.data)
+    assert st == bm.ERR_INVALID
+    assert L.bmfp_gen_n_diag_bcsr(3, 5, 2, z.ctypes.data, z.ctypes.data, v.ctypes.data) == bm.ERR_INVALID
+
+
+def test_init_without_device_reports_no_device():
+    if bm.device_count() > 0:
+        pytest.skip("a device is present")
+    assert bm.lib().bmfp_init(0) == bm.ERR_NO_DEVICE

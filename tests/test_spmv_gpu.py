@@ -1,0 +1,228 @@
+"""GPU parity of every SpMV kernel family against the CPU oracle.
+
+Bar (FP64): per row |y_gpu - y_oracle| <= 1e-12 * (|A||x|)_i.
+FP32 / FP16+scale storage: the oracle runs on the device's own rounded values
+(downloaded) and x rounded to FP32, so only accumulation error remains:
+|y_gpu - y_oracle| <= (row_len + 2) * 2^-24 * (|A||x|)_i.
+"""
+import numpy as np
+import pytest
+
+import block_matrix_format_performance_amd as bm
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+U32 = 2.0 ** -24
+
+
+@pytest.fixture(autouse=True)
+def _name_the_case(request, device):
+    print(f"\n[gpu-case] {request.node.nodeid}", flush=True)
+    yield
+
+
+def stencil_case(nx, ny, bs, kind=0, seed=0):
+    m = oracle.gen_stencil9(nx, ny, bs, kind, seed)
+    x = oracle.rng_unit_vec(seed, 4, m.n_cols * bs)
+    return m, x
+
+
+def ragged_case(n_rows, n_cols, bs, max_len, seed, empty_every=5):
+    """Random BCSR with ragged rows (0..max_len blocks), sorted columns."""
+    rng = np.random.default_rng(seed)
+    rp = [0]
+    cols = []
+    for r in range(n_rows):
+        ln = 0 if (empty_every and r % empty_every == 3) else int(rng.integers(1, max_len + 1))
+        ln = min(ln, n_cols)
+        cols.extend(sorted(rng.choice(n_cols, ln, replace=False).tolist()))
+        rp.append(len(cols))
+    nnzb = len(cols)
+    vals = rng.uniform(-1, 1, nnzb * bs * bs)
+    m = oracle.BCSR(n_rows, n_cols, bs, np.array(rp), np.array(cols, dtype=np.int32), vals)
+    x = rng.uniform(0, 1, n_cols * bs)
+    return m, x
+
+
+def row_lengths(m):
+    return np.repeat(np.diff(m.row_ptr) * m.bs, m.bs)
+
+
+def check_bcsr(m, x, layout, variant, dtype=bm.F64):
+    vals = m.values if layout == bm.ROW_MAJOR else oracle.transpose_blocks(m)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, vals, m.n_rows, m.n_cols, m.bs, dtype=dtype, layout=layout)
+    y = bm.spmv_host(A, x, variant)
+    if dtype == bm.F64:
+        want = oracle.bcsr_spmv(m, x, layout=layout, values=vals)
+        scale = oracle.bcsr_absmv(m, x, layout=layout, values=vals)
+        tol = F64_TOL * scale
+    else:
+        dv = A.download_values()
+        x32 = x.astype(np.float32).astype(np.float64)
+        want = oracle.bcsr_spmv(m, x32, layout=layout, values=dv)
+        scale = oracle.bcsr_absmv(m, x32, layout=layout, values=dv)
+        tol = (row_lengths(m) + 2) * U32 * scale
+    err = np.abs(y.astype(np.float64) - want)
+    bad = np.nonzero(err > tol)[0]
+    assert bad.size == 0, (f"variant {variant} layout {layout} bs {m.bs} dtype {dtype}: "
+                           f"{bad.size} rows off, first {bad[:5]}, err {err[bad[:5]]}, tol {tol[bad[:5]]}")
+    return A, y
+
+
+VARIANTS_ROW = [bm.BCSR_THREAD_PER_ROW, bm.BCSR_GROUP_PER_ROW, bm.BCSR_SHUFFLE_X]
+VARIANTS_COL = VARIANTS_ROW + [bm.BCSR_BLOCK_ROW_GROUP]
+
+
+@pytest.mark.parametrize("bs", bm.PRECOMPILED_BS)
+@pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
+def test_bcsr_f64_stencil_all_variants(bs, layout):
+    m, x = stencil_case(13, 11, bs, kind=0, seed=bs)
+    for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+        check_bcsr(m, x, layout, v)
+    check_bcsr(m, x, layout, bm.SPMV_AUTO)
+
+
+@pytest.mark.parametrize("bs", [1, 3, 4, 8, 24])
+@pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
+def test_bcsr_f64_ragged_empty_rows(bs, layout):
+    m, x = ragged_case(97, 83, bs, max_len=40, seed=100 + bs)
+    for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+        check_bcsr(m, x, layout, v)
+
+
+@pytest.mark.parametrize("bs", [1, 4, 16])
+def test_bcsr_f64_xcd_sized_grid(bs):
+    # big enough that the launch uses the XCD tile order (>= 64 workgroups)
+    m, x = stencil_case(72, 70, bs, kind=1, seed=3)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+            check_bcsr(m, x, layout, v)
+
+
+@pytest.mark.parametrize("bs", [5, 7, 40])
+def test_bcsr_runtime_block_size(bs):
+    m, x = stencil_case(6, 5, bs, kind=0, seed=1)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        check_bcsr(m, x, layout, bm.BCSR_THREAD_PER_ROW)
+        check_bcsr(m, x, layout, bm.SPMV_AUTO)
+        A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, bs, layout=layout)
+        with pytest.raises(bm.BmfpError) as e:
+            bm.spmv_host(A, x, bm.BCSR_GROUP_PER_ROW)
+        assert e.value.status == bm.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("bs", [2, 4, 8, 12, 32])
+@pytest.mark.parametrize("dtype", [bm.F32, bm.F16])
+def test_bcsr_low_precision_storage(bs, dtype):
+    m, x = stencil_case(9, 8, bs, kind=0, seed=2)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+            check_bcsr(m, x, layout, v, dtype=dtype)
+
+
+def test_bcsr_identity_blocks_give_x():
+    bs, n = 4, 50
+    rp = np.arange(n + 1)
+    cols = np.arange(n)
+    vals = np.tile(np.eye(bs).ravel(), n)
+    x = oracle.rng_sym_vec(0, 4, n * bs)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        A = bm.Matrix.bcsr(rp, cols, vals, n, n, bs, layout=layout)
+        for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+            np.testing.assert_array_equal(bm.spmv_host(A, x, v), x)
+
+
+def test_bcsr_single_block_is_dense_gemv():
+    bs = 32
+    rng = np.random.default_rng(1)
+    blk = rng.uniform(-1, 1, (bs, bs))
+    x = rng.uniform(-1, 1, bs)
+    want = blk @ x
+    for layout, vals in ((bm.ROW_MAJOR, blk.ravel()), (bm.COL_MAJOR, blk.T.ravel())):
+        A = bm.Matrix.bcsr([0, 1], [0], vals, 1, 1, bs, layout=layout)
+        for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+            y = bm.spmv_host(A, x, v)
+            assert np.all(np.abs(y - want) <= 1e-13 * (np.abs(blk) @ np.abs(x)))
+
+
+def test_bcsr_empty_matrix_is_noop():
+    A = bm.Matrix.bcsr([0], [], [], 0, 0, 4)
+    dx = bm.DeviceArray(1)
+    dy = bm.DeviceArray(1)
+    A.spmv(dx, dy, bm.BCSR_THREAD_PER_ROW)
+    bm.synchronize()
+
+
+def test_bcsr_results_are_bitwise_reproducible():
+    m, x = stencil_case(40, 33, 6, kind=1, seed=9)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
+            _, y1 = check_bcsr(m, x, layout, v)
+            _, y2 = check_bcsr(m, x, layout, v)
+            assert np.array_equal(y1, y2)
+
+
+def test_bcsr_library_baseline_rows():
+    m, x = stencil_case(20, 17, 4, kind=0, seed=4)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        check_bcsr(m, x, layout, bm.BCSR_LIBRARY)
+
+
+def test_invalid_structures_rejected():
+    with pytest.raises(bm.BmfpError) as e:
+        bm.Matrix.bcsr([0, 2], [0, 5], np.zeros(8), 1, 2, 2)   # column out of range
+    assert e.value.status == bm.ERR_INVALID
+    with pytest.raises(bm.BmfpError):
+        bm.Matrix.bcsr([0, 2, 1], [0, 1], np.zeros(8), 2, 2, 2)  # row_ptr decreases
+    m, x = stencil_case(4, 4, 2)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 2)
+    with pytest.raises(bm.BmfpError):
+        bm.spmv_host(A, x, bm.CSR_SCALAR)
+
+
+# ---- CSR ----------------------------------------------------------------------
+def check_csr(c, x, variant, dtype=bm.F64):
+    A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols, dtype=dtype)
+    y = bm.spmv_host(A, x, variant).astype(np.float64)
+    if dtype == bm.F64:
+        want = oracle.csr_spmv(c, x)
+        tol = F64_TOL * oracle.csr_absmv(c, x)
+    else:
+        v32 = c.values.astype(np.float32).astype(np.float64)
+        c32 = oracle.CSR(c.n_rows, c.n_cols, c.row_ptr, c.columns, v32)
+        x32 = x.astype(np.float32).astype(np.float64)
+        want = oracle.csr_spmv(c32, x32)
+        tol = (np.diff(c.row_ptr) + 2) * U32 * oracle.csr_absmv(c32, x32)
+    bad = np.nonzero(np.abs(y - want) > tol)[0]
+    assert bad.size == 0, (variant, bad[:5])
+
+
+@pytest.mark.parametrize("bs", [1, 2, 4, 8, 32])
+def test_csr_stencil_expansion(bs):
+    m, x = stencil_case(12, 10, bs, kind=0, seed=bs)
+    c = oracle.bcsr_to_csr(m)
+    for v in (bm.CSR_SCALAR, bm.CSR_VECTOR, bm.SPMV_AUTO):
+        check_csr(c, x, v)
+    check_csr(c, x, bm.CSR_VECTOR, dtype=bm.F32)
+
+
+def test_csr_ragged_rows_every_group_width():
+    # mean row lengths from ~1 to ~300 cover lane groups of 2 .. 64
+    for max_len in (2, 8, 24, 60, 130, 300):
+        m, x = ragged_case(150, 400, 1, max_len=max_len, seed=max_len)
+        c = oracle.bcsr_to_csr(m)
+        for v in (bm.CSR_SCALAR, bm.CSR_VECTOR):
+            check_csr(c, x, v)
+
+
+def test_fill_matches_host_rng():
+    n = 1000
+    d = bm.DeviceArray(n)
+    d.fill(0, seed=5, stream_id=4)
+    assert np.array_equal(d.download(), oracle.rng_unit_vec(5, 4, n))
+    d.fill(1, seed=5, stream_id=6)
+    assert np.array_equal(d.download(), oracle.rng_sym_vec(5, 6, n))
+    d.fill(2, value=2.5)
+    assert np.all(d.download() == 2.5)
