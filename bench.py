@@ -62,6 +62,26 @@ def build_case(bs, side, layouts):
     return mats, rp, cols, vals
 
 
+def measure_copy(nbytes, reps, stream):
+    """Plain streaming copy kernel: the measured-bandwidth yardstick."""
+    nbytes = int(nbytes) // 16 * 16
+    src = bm.DeviceArray(nbytes // 8)
+    dst = bm.DeviceArray(nbytes // 8)
+    src.fill(2, value=1.0)
+    L = bm.lib()
+    for _ in range(3):
+        bm.check(L.bmfp_stream_copy(dst.ptr, src.ptr, nbytes, stream.handle))
+    e0, e1 = bm.Event(), bm.Event()
+    e0.record(stream.handle)
+    for _ in range(reps):
+        bm.check(L.bmfp_stream_copy(dst.ptr, src.ptr, nbytes, stream.handle))
+    e1.record(stream.handle)
+    t = e1.elapsed_since(e0) / reps
+    src.free()
+    dst.free()
+    return {"bytes_moved": 2 * nbytes, "us": round(t * 1e6, 2), "GB/s": round(2 * nbytes / t / 1e9, 1)}
+
+
 def cpu_baseline(seconds):
     """Oracle single-thread CSR SpMV on the bs=4 matrix of the literal sizing."""
     import oracle
@@ -86,6 +106,26 @@ def cpu_baseline(seconds):
             "seconds_per_spmv": per}
 
 
+def run_only(args, total_values, stream):
+    bs_s, lay_s, fam_s = args.only.split(":")
+    bs = int(bs_s)
+    layout = bm.ROW_MAJOR if lay_s == "row" else bm.COL_MAJOR
+    fam = {v: k for k, v in FAMILY_NAMES.items()}[fam_s]
+    side = grid_side(bs, total_values)
+    mats, _, _, _ = build_case(bs, side, (layout,))
+    A = mats[layout]
+    x = bm.DeviceArray(A.x_len)
+    x.fill(0, seed=0, stream_id=4)
+    y = bm.DeviceArray(A.y_len)
+    nbytes, _ = A.traffic()
+    t = A.spmv_timed(x, y, fam, warmup=args.warmup, reps=args.steps, stream=stream.handle)
+    res = {"kernel": args.only, "side": side, "launches": args.warmup + args.steps, "us": round(t * 1e6, 2),
+           "algorithmic_bytes": nbytes, "GB/s": round(nbytes / t / 1e9, 1)}
+    if args.copy_mb > 0:
+        res["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 5, stream)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +137,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--families", default="all", help="all | auto")
+    ap.add_argument("--only", default=None,
+                    help="BS:LAYOUT:FAMILY (e.g. 4:row:group_per_row): run just that kernel "
+                         "--steps times (for counter collection) and print its timing")
+    ap.add_argument("--copy-mb", type=float, default=2048.0,
+                    help="size of the streaming-copy yardstick (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +155,10 @@ def main():
     total_values = 4.4e8 if args.sizing == "hbm" else 4.4e7
     bs_list = [int(b) for b in args.bs.split(",")]
     stream = bm.Stream()
+
+    if args.only:
+        run_only(args, total_values, stream)
+        return
 
     cases = []
     sweep = {}
@@ -215,6 +264,8 @@ def main():
                         for t, c in per_kernel},
         "sweep_GBs": sweep,
     }
+    if args.copy_mb > 0:
+        out["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 50, stream)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         cb = cpu_baseline(args.cpu_seconds)

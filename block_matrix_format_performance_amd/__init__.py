@@ -93,6 +93,7 @@ def _declare(L):
     L.bmfp_event_destroy.argtypes = [vp]
     L.bmfp_event_record.argtypes = [vp, vp]
     L.bmfp_event_elapsed.argtypes = [vp, vp, dp]
+    L.bmfp_stream_copy.argtypes = [vp, vp, sz, vp]
     L.bmfp_fill.argtypes = [vp, c_int, sz, c_int, ctypes.c_uint64, ctypes.c_uint64, c_dbl, vp]
     L.bmfp_gen_n_diag_bcsr.argtypes = [c_int, c_int, c_int, vp, vp, vp]
     L.bmfp_stencil_nnzb.restype = ctypes.c_int64
@@ -114,10 +115,10 @@ def _declare(L):
 def _declare_solver(L):
     """Solver / refinement / serving entry points (present once built)."""
     c_int, c_dbl, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
-    if hasattr(L, "bmfp_bicgstab_create"):
-        L.bmfp_bicgstab_create.argtypes = [ctypes.POINTER(vp), vp, c_int]
-        L.bmfp_bicgstab_solve.argtypes = [vp, vp, vp, c_dbl, c_int, vp, ctypes.POINTER(SolveInfo)]
-        L.bmfp_bicgstab_destroy.argtypes = [vp]
+    L.bmfp_bicgstab_create.argtypes = [ctypes.POINTER(vp), vp, c_int]
+    L.bmfp_bicgstab_solve.argtypes = [vp, vp, vp, c_dbl, c_int, vp, ctypes.POINTER(SolveInfo)]
+    L.bmfp_bicgstab_destroy.argtypes = [vp]
+    L.bmfp_bicgstab_workspace_bytes.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_size_t)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -369,3 +370,44 @@ def spmv_host(A, x, variant=SPMV_AUTO):
     A.spmv(dx, dy, variant)
     synchronize()
     return dy.download()
+
+
+# ---- solvers ------------------------------------------------------------------------
+SOLVE_STATUSES = (OK, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_NONFINITE)
+
+
+class BiCGStab:
+    """bmfp_bicgstab_* handle: BiCGStab on a device-resident BCSR matrix."""
+
+    def __init__(self, A, use_jacobi=False):
+        self.A = A
+        h = ctypes.c_void_p()
+        check(lib().bmfp_bicgstab_create(ctypes.byref(h), A.handle, 1 if use_jacobi else 0))
+        self.handle = h
+
+    def solve(self, b, x, eps, max_iterations, stream=None):
+        """b, x: DeviceArray.  Returns SolveInfo; raises only for bad arguments."""
+        info = SolveInfo()
+        st = lib().bmfp_bicgstab_solve(self.handle, b.ptr, x.ptr, eps, max_iterations, stream,
+                                       ctypes.byref(info))
+        if st not in SOLVE_STATUSES:
+            check(st)
+        return info
+
+    def solve_host(self, b, eps=1e-10, max_iterations=10000):
+        vt = self.A.vec_dtype
+        db = DeviceArray.from_host(np.ascontiguousarray(b, vt))
+        dx = DeviceArray(self.A.y_len, vt)
+        info = self.solve(db, dx, eps, max_iterations)
+        return dx.download(), info
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().bmfp_bicgstab_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,326 @@
+// bicgstab.hip -- BiCGStab solver handle (bmfp_bicgstab_*): resident Krylov
+// vectors, device-side scalars, fused kernels replayed from a HIP graph.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "matrix.hpp"
+#include "solver.hpp"
+#include "solver_kernels.hpp"
+
+namespace bmfp {
+
+namespace {
+
+constexpr int GRAPH_ITERS = 8;  // iterations captured per graph launch
+
+template <class VT, class AT, bool SCALED>
+MatArgs<VT, AT> mat_args(const bmfp_matrix_t &A)
+{
+  MatArgs<VT, AT> m;
+  m.n_brows = A.n_rows;
+  m.rp = A.row_ptr->as<int>();
+  m.cols = A.columns->as<int>();
+  m.vals = A.values.as<VT>();
+  m.scale = SCALED ? A.scale.as<float>() : nullptr;
+  return m;
+}
+
+template <class AT>
+SolverVecs<AT> vec_args(const bmfp_solver_t &S, const void *b)
+{
+  SolverVecs<AT> V;
+  AT *base = S.vecs.as<AT>();
+  const int64_t n = S.n;
+  V.b = static_cast<const AT *>(b);
+  V.x = base + 0 * n;
+  V.r = base + 1 * n;
+  V.rh = base + 2 * n;
+  V.p[0] = base + 3 * n;
+  V.p[1] = base + 4 * n;
+  V.v[0] = base + 5 * n;
+  V.v[1] = base + 6 * n;
+  V.s = base + 7 * n;
+  V.t = base + 8 * n;
+  V.pinv = S.use_jacobi ? base + 9 * n : nullptr;
+  double *parts = S.parts.as<double>();
+  V.partA = parts;
+  V.partB = parts + SOLVER_MAX_GROUPS;
+  V.partC = parts + 3 * SOLVER_MAX_GROUPS;
+  V.st = S.state.as<SolverState>();
+  V.n = n;
+  return V;
+}
+
+// Lane-group shape for the fused SpMVs: thread per row for small blocks and
+// for column-major blocks; 16-byte column slices per lane for row-major
+// blocks from bs = 8 up.
+template <class VT> constexpr int solver_vec(int bs, int layout)
+{
+  if (layout != BMFP_ROW_MAJOR || bs < 8) return bs;
+  constexpr int W = 16 / (int) sizeof(VT);
+  int v = 1;
+  while (v < W && bs % (2 * v) == 0) v *= 2;
+  return v;
+}
+
+template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
+struct Ops {
+  static constexpr int VEC = solver_vec<VT>(BS, LAYOUT);
+  static constexpr int G = pow2ceil(BS / VEC);
+
+  static int groups(const bmfp_matrix_t &A)
+  {
+    const int64_t t = (int64_t) A.n_rows * BS * G;
+    return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
+  }
+
+  static int init(const bmfp_solver_t &S, const void *b, double eps, int max_iter, hipStream_t s)
+  {
+    k_bicg_init<AT><<<S.np, 256, 0, s>>>(vec_args<AT>(S, b), eps, max_iter);
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
+
+  static int iterate(const bmfp_solver_t &S, hipStream_t s)
+  {
+    const MatArgs<VT, AT> M = mat_args<VT, AT, SCALED>(*S.A);
+    const SolverVecs<AT> V = vec_args<AT>(S, nullptr);
+    k_bicg_a<VT, AT, BS, G, VEC, LAYOUT, SCALED><<<S.np, 256, 0, s>>>(M, V);
+    k_bicg_b<VT, AT, BS, G, VEC, LAYOUT, SCALED><<<S.np, 256, 0, s>>>(M, V);
+    k_bicg_c<AT><<<S.np, 256, 0, s>>>(V);
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
+
+  static int jacobi(const bmfp_solver_t &S, hipStream_t s)
+  {
+    const MatArgs<VT, AT> M = mat_args<VT, AT, SCALED>(*S.A);
+    const unsigned g = (unsigned) ((S.n + 255) / 256);
+    k_jacobi<VT, AT, LAYOUT, SCALED><<<g, 256, 0, s>>>(M, S.A->bs, S.vecs.as<AT>() + 9 * S.n);
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
+};
+
+template <class VT, class AT, bool SCALED, int LAYOUT>
+bool pick_bs(bmfp_solver_t &S)
+{
+#define BMFP_SOLVER_BS(B)                                        \
+  case B:                                                        \
+    S.ops_init = Ops<VT, AT, SCALED, B, LAYOUT>::init;           \
+    S.ops_iterate = Ops<VT, AT, SCALED, B, LAYOUT>::iterate;     \
+    S.ops_jacobi = Ops<VT, AT, SCALED, B, LAYOUT>::jacobi;       \
+    S.np = Ops<VT, AT, SCALED, B, LAYOUT>::groups(*S.A);         \
+    return true;
+  switch (S.A->bs) {
+    BMFP_SOLVER_BS(1)
+    BMFP_SOLVER_BS(2)
+    BMFP_SOLVER_BS(3)
+    BMFP_SOLVER_BS(4)
+    BMFP_SOLVER_BS(6)
+    BMFP_SOLVER_BS(8)
+    BMFP_SOLVER_BS(12)
+    BMFP_SOLVER_BS(16)
+    BMFP_SOLVER_BS(24)
+    BMFP_SOLVER_BS(32)
+  default:
+    return false;
+  }
+#undef BMFP_SOLVER_BS
+}
+
+template <class VT, class AT, bool SCALED>
+bool pick_layout(bmfp_solver_t &S)
+{
+  return S.A->layout == BMFP_ROW_MAJOR ? pick_bs<VT, AT, SCALED, BMFP_ROW_MAJOR>(S)
+                                       : pick_bs<VT, AT, SCALED, BMFP_COL_MAJOR>(S);
+}
+
+// ||b - y||^2 partial sums (one per workgroup) for the true residual.
+template <class AT>
+__global__ __launch_bounds__(256) void k_diff_norm2(const AT *__restrict__ b, const AT *__restrict__ y, int64_t n,
+                                                    double *__restrict__ part)
+{
+  __shared__ double lds[2][4];
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) {
+    const double d = (double) b[i] - (double) y[i];
+    v[0] += d * d;
+    v[1] += (double) b[i] * (double) b[i];
+  }
+  write_partials<2>(part, v, lds);
+}
+
+}  // namespace
+
+size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi)
+{
+  const int64_t n = A.y_len();
+  return (size_t) n * vec_size(A.dtype) * (use_jacobi ? 10 : 9) + sizeof(double) * 5 * SOLVER_MAX_GROUPS +
+         sizeof(SolverState) + 256;
+}
+
+int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
+{
+  S.A = A;
+  S.use_jacobi = use_jacobi ? 1 : 0;
+  S.n = A->y_len();
+  if (A->n_rows != A->n_cols) return fail(BMFP_ERR_INVALID, "BiCGStab needs a square matrix");
+  if (!A->is_bcsr) return fail(BMFP_ERR_UNSUPPORTED, "BiCGStab runs on BCSR handles (CSR: use bs = 1)");
+  bool ok = false;
+  switch (A->dtype) {
+  case BMFP_F64: ok = pick_layout<double, double, false>(S); break;
+  case BMFP_F32: ok = pick_layout<float, float, false>(S); break;
+  case BMFP_F16: ok = pick_layout<__half, float, true>(S); break;
+  }
+  if (!ok) return fail(BMFP_ERR_UNSUPPORTED, "no BiCGStab kernels for bs %d", A->bs);
+  const size_t vb = vec_size(A->dtype);
+  BMFP_TRY(S.vecs.alloc((size_t) std::max<int64_t>(S.n, 1) * vb * (S.use_jacobi ? 10 : 9)));
+  BMFP_TRY(S.parts.alloc(sizeof(double) * 5 * SOLVER_MAX_GROUPS));
+  BMFP_TRY(S.state.alloc(sizeof(SolverState)));
+  BMFP_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.h_state), sizeof(SolverState), hipHostMallocDefault));
+  BMFP_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+  BMFP_HIP(hipEventCreate(&S.ev0));
+  BMFP_HIP(hipEventCreate(&S.ev1));
+  BMFP_HIP(hipEventCreateWithFlags(&S.ev_sync, hipEventDisableTiming));
+  if (S.use_jacobi && S.n) {
+    BMFP_TRY(S.ops_jacobi(S, S.stream));
+    BMFP_HIP(hipStreamSynchronize(S.stream));
+  }
+  return BMFP_OK;
+}
+
+static int build_graph(bmfp_solver_t &S)
+{
+  if (S.graph) return BMFP_OK;
+  hipGraph_t g = nullptr;
+  BMFP_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+  int st = BMFP_OK;
+  for (int i = 0; i < GRAPH_ITERS && st == BMFP_OK; i++) st = S.ops_iterate(S, S.stream);
+  hipError_t e = hipStreamEndCapture(S.stream, &g);
+  if (st != BMFP_OK) {
+    if (g) (void) hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+  (void) hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    S.graph = nullptr;
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  return BMFP_OK;
+}
+
+int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
+               bmfp_solve_info *info)
+{
+  if (!(eps > 0.0) || max_iterations < 0) return fail(BMFP_ERR_INVALID, "eps must be > 0, max_iterations >= 0");
+  if (S.n == 0) {
+    if (info) *info = bmfp_solve_info{0, BMFP_OK, 0.0, 0.0, 0.0, 0, 0};
+    return BMFP_OK;
+  }
+  if (!b || !x) return fail(BMFP_ERR_INVALID, "NULL vector");
+  BMFP_TRY(build_graph(S));
+  hipStream_t s = S.stream;
+  // order after the caller's prior work on its stream
+  BMFP_HIP(hipEventRecord(S.ev_sync, user));
+  BMFP_HIP(hipStreamWaitEvent(s, S.ev_sync, 0));
+  BMFP_HIP(hipEventRecord(S.ev0, s));
+  BMFP_TRY(S.ops_init(S, b, eps, max_iterations, s));
+  // Launch graph chunks; read the 64-byte state back between batches.  The
+  // batch grows while the solve runs so that the host round trip stays a
+  // small fraction; kernels after the stop are no-ops.
+  int batch = 1;
+  for (;;) {
+    for (int k = 0; k < batch; k++) BMFP_HIP(hipGraphLaunch(S.graph, s));
+    BMFP_HIP(hipMemcpyAsync(S.h_state, S.state.ptr, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    BMFP_HIP(hipStreamSynchronize(s));
+    if (S.h_state->done) break;
+    batch = std::min(batch * 2, 8);
+  }
+  const size_t vb = vec_size(S.A->dtype);
+  BMFP_HIP(hipMemcpyAsync(x, S.vecs.ptr, (size_t) S.n * vb, hipMemcpyDeviceToDevice, s));
+  BMFP_HIP(hipEventRecord(S.ev1, s));
+  BMFP_HIP(hipEventSynchronize(S.ev1));
+  float ms = 0.f;
+  BMFP_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  const SolverState &st = *S.h_state;
+  bmfp_solve_info out{};
+  out.iterations = st.iter;
+  out.status = st.status;
+  out.rel_residual = st.norm_b2 > 0 ? std::sqrt(st.rr) / std::sqrt(st.norm_b2) : 0.0;
+  out.seconds = ms * 1e-3;
+  // true residual ||b - A x|| / ||b|| (after the timed region)
+  {
+    void *y = S.vecs.as<char>() + (size_t) 8 * S.n * vb;  // t buffer
+    BMFP_TRY(launch_bcsr(*S.A, auto_variant(*S.A), x, y, s));
+    const int ng = std::min<int64_t>((S.n + 255) / 256, SOLVER_MAX_GROUPS);
+    double *part = S.parts.as<double>();
+    if (S.A->dtype == BMFP_F64)
+      k_diff_norm2<double><<<ng, 256, 0, s>>>(static_cast<const double *>(b), static_cast<const double *>(y), S.n, part);
+    else
+      k_diff_norm2<float><<<ng, 256, 0, s>>>(static_cast<const float *>(b), static_cast<const float *>(y), S.n, part);
+    BMFP_LAUNCHED();
+    std::vector<double> h((size_t) 2 * ng);
+    BMFP_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    BMFP_HIP(hipStreamSynchronize(s));
+    double rr = 0, bb = 0;
+    for (int g = 0; g < ng; g++) {
+      rr += h[2 * g];
+      bb += h[2 * g + 1];
+    }
+    out.true_rel_residual = bb > 0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
+  }
+  if (info) *info = out;
+  return out.status;
+}
+
+}  // namespace bmfp
+
+using namespace bmfp;
+
+bmfp_solver_t::~bmfp_solver_t()
+{
+  if (graph) (void) hipGraphExecDestroy(graph);
+  if (h_state) (void) hipHostFree(h_state);
+  if (ev0) (void) hipEventDestroy(ev0);
+  if (ev1) (void) hipEventDestroy(ev1);
+  if (ev_sync) (void) hipEventDestroy(ev_sync);
+  if (stream) (void) hipStreamDestroy(stream);
+}
+
+extern "C" int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes)
+{
+  if (!A || !bytes) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *bytes = solver_workspace_bytes(*A, use_jacobi);
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_bicgstab_create(bmfp_solver *out, bmfp_matrix A, int use_jacobi)
+{
+  if (!out || !A) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
+  BMFP_TRY(solver_create(*S, A, use_jacobi));
+  *out = S.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_bicgstab_solve(bmfp_solver s, const void *b, void *x, double eps, int max_iterations,
+                                   bmfp_stream stream, bmfp_solve_info *info)
+{
+  if (!s) return fail(BMFP_ERR_INVALID, "solver is NULL");
+  const int st = solver_run(*s, b, x, eps, max_iterations, static_cast<hipStream_t>(stream), info);
+  if (st == BMFP_ERR_NOT_CONVERGED) set_error("BiCGStab: max_iterations (%d) reached", max_iterations);
+  else if (st == BMFP_ERR_BREAKDOWN) set_error("BiCGStab breakdown");
+  else if (st == BMFP_ERR_NONFINITE) set_error("BiCGStab: non-finite value in the iteration");
+  return st;
+}
+
+extern "C" int bmfp_bicgstab_destroy(bmfp_solver s)
+{
+  delete s;
+  return BMFP_OK;
+}

@@ -48,6 +48,15 @@ __global__ __launch_bounds__(256) void k_fill(T *dst, int64_t n, int kind, uint6
   }
 }
 
+// ---- streaming copy (bandwidth yardstick) -------------------------------------
+typedef unsigned int u32x4c __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(u32x4c *__restrict__ dst, const u32x4c *__restrict__ src,
+                                                     int64_t n16)
+{
+  const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 // ---- value conversion (F64 -> F64/F32/F16, optional block transpose) --------
 // One wavefront per block row: first the row's max |entry| (F16 scale), then
 // a strided pass that writes every element in the target layout.
@@ -244,6 +253,19 @@ extern "C" int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *sec
   float ms = 0.f;
   BMFP_HIP(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
   *seconds = ms * 1e-3;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stream_copy(void *dst, const void *src, size_t bytes, bmfp_stream stream)
+{
+  if (bytes == 0) return BMFP_OK;
+  if (!dst || !src || bytes % 16 || ((uintptr_t) dst | (uintptr_t) src) % 16)
+    return fail(BMFP_ERR_INVALID, "bmfp_stream_copy: needs 16-byte aligned pointers and size");
+  const int64_t n16 = (int64_t) (bytes / 16);
+  const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
+  k_stream_copy<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<u32x4c *>(dst),
+                                                                        static_cast<const u32x4c *>(src), n16);
+  BMFP_LAUNCHED();
   return BMFP_OK;
 }
 

@@ -1,0 +1,34 @@
+// solver.hpp -- BiCGStab solver handle shared by the plain solver, the
+// mixed-precision refinement loop and the serving loop.
+#pragma once
+
+#include "matrix.hpp"
+
+namespace bmfp {
+struct SolverState;
+}
+
+struct bmfp_solver_t {
+  bmfp_matrix_t *A = nullptr;   // borrowed
+  int use_jacobi = 0;
+  int np = 1;                   // persistent workgroups of every solver kernel
+  int64_t n = 0;                // scalar rows
+  bmfp::DeviceBuffer vecs;      // x r r^ p0 p1 v0 v1 s t [pinv]
+  bmfp::DeviceBuffer parts;     // partial sums
+  bmfp::DeviceBuffer state;     // SolverState
+  bmfp::SolverState *h_state = nullptr;  // pinned mirror of the device state
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_sync = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int (*ops_init)(const bmfp_solver_t &, const void *, double, int, hipStream_t) = nullptr;
+  int (*ops_iterate)(const bmfp_solver_t &, hipStream_t) = nullptr;
+  int (*ops_jacobi)(const bmfp_solver_t &, hipStream_t) = nullptr;
+  ~bmfp_solver_t();
+};
+
+namespace bmfp {
+size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi);
+int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi);
+int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
+               bmfp_solve_info *info);
+}  // namespace bmfp

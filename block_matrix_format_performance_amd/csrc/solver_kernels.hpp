@@ -1,0 +1,447 @@
+// solver_kernels.hpp -- fused BiCGStab kernels for gfx950.
+//
+// One BiCGStab iteration (bicgstab.cu:426-513 without per-step host syncs) is
+// three kernels:
+//   A  v = A q,  q = P (r + beta (p_prev - omega_prev v_prev))   (p formed on
+//      the fly at every gathered index and written for owned rows), partial
+//      sums of (r^, v)
+//   B  t = A z,  z = P (r - alpha v)  (s written for owned rows), partial sums
+//      of (t, s) and (t, t)
+//   C  r = s - omega t,  x = (x + alpha P p) + omega P s, partial sums of
+//      (r^, r) and (r, r)
+// P is the point-Jacobi inverse diagonal (bicgstab.cu:201-238) or identity.
+//
+// Scalars never leave the device.  Every kernel runs a persistent grid of the
+// same NP workgroups; each workgroup first reduces the previous kernel's NP
+// partial sums itself, in one fixed order, so all workgroups derive
+// bit-identical alpha/beta/omega without atomics or grid barriers.  The
+// iteration counter and the stop flag travel through SolverState fields that
+// only earlier kernels write.
+#pragma once
+
+#include "spmv_kernels.hpp"
+
+namespace bmfp {
+
+constexpr int SOLVER_MAX_GROUPS = 2048;
+
+struct SolverState {
+  double rho[2], alpha[2], omega[2], tt[2];  // by iteration parity
+  double norm_b2;      // ||b||^2
+  double rr;           // ||r||^2 after the last completed iteration
+  double eps;          // relative tolerance on ||r||/||b||
+  int iter;            // completed iterations (written by C)
+  int iter_b, iter_c;  // hand-off copies (written by A, B)
+  int done;            // 1 once the solve has stopped
+  int status;          // BMFP_OK / _NOT_CONVERGED / _BREAKDOWN / _NONFINITE
+  int max_iter;
+  int pad;
+};
+
+template <class AT>
+struct SolverVecs {
+  const AT *__restrict__ b;
+  AT *__restrict__ x;
+  AT *__restrict__ r;
+  AT *__restrict__ rh;
+  AT *__restrict__ p[2];
+  AT *__restrict__ v[2];
+  AT *__restrict__ s;
+  AT *__restrict__ t;
+  const AT *__restrict__ pinv;   // Jacobi inverse diagonal, or null
+  double *__restrict__ partA;    // [NP]
+  double *__restrict__ partB;    // [NP][2]
+  double *__restrict__ partC;    // [NP][2]
+  SolverState *st;
+  int64_t n;                     // scalar rows
+};
+
+// ---- fixed-order reductions ----------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Sum of one value per thread over a 256-thread workgroup; result valid in
+// every thread.  Order fixed by thread index.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[4])
+{
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = wave_sum(v[k]);
+    if (lane == 0) lds[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = (lds[k][0] + lds[k][1]) + (lds[k][2] + lds[k][3]);
+  __syncthreads();
+}
+
+// Reduces K interleaved partial sums part[g*K + k], g < np, in a fixed order.
+template <int K>
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, int np, double (&out)[K],
+                                                double (*lds)[4])
+{
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = 0.0;
+  for (int g = threadIdx.x; g < np; g += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] += part[g * K + k];
+  block_sum<K>(out, lds);
+}
+
+template <int K>
+__device__ __forceinline__ void write_partials(double *__restrict__ part, double (&v)[K], double (*lds)[4])
+{
+  block_sum<K>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[blockIdx.x * K + k] = v[k];
+}
+
+// Reads the stop flag once per workgroup: WG 0 of the running kernel may set
+// it concurrently, and every thread of a workgroup must take the same branch.
+__device__ __forceinline__ bool solver_stopped(const SolverState *st)
+{
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = st->done;
+  __syncthreads();
+  return s_done != 0;
+}
+
+// ---- operands formed on the fly ---------------------------------------------------
+// The same explicit fused multiply-adds are used where an owned row's value is
+// stored and where neighbours gather it, so both see identical bits.
+template <class AT> __device__ __forceinline__ AT p_update(AT r, AT p, AT v, AT beta, AT omega)
+{
+  return fmadd(beta, fmadd(-omega, v, p), r);
+}
+template <class AT> __device__ __forceinline__ AT s_update(AT r, AT v, AT alpha) { return fmadd(-alpha, v, r); }
+
+// Kernel A operand: q_i = P_i (r_i + beta (p_i - omega v_i)).  `raw` returns
+// the unpreconditioned p_i that is stored for owned rows.
+template <class AT>
+struct OperandP {
+  const AT *__restrict__ r;
+  const AT *__restrict__ p;
+  const AT *__restrict__ v;
+  const AT *__restrict__ pinv;
+  AT beta, omega;
+  __device__ __forceinline__ AT raw(int64_t i) const { return p_update(r[i], p[i], v[i], beta, omega); }
+  __device__ __forceinline__ AT operator()(int64_t i) const
+  {
+    const AT q = raw(i);
+    return pinv ? pinv[i] * q : q;
+  }
+  template <int N, int ALIGN>
+  __device__ __forceinline__ void load(int64_t i, AT (&out)[N]) const
+  {
+    AT rr[N], pp[N], vv[N];
+    load_n<AT, N, ALIGN>(r + i, rr);
+    load_n<AT, N, ALIGN>(p + i, pp);
+    load_n<AT, N, ALIGN>(v + i, vv);
+#pragma unroll
+    for (int c = 0; c < N; ++c) out[c] = p_update(rr[c], pp[c], vv[c], beta, omega);
+    if (pinv) {
+      AT d[N];
+      load_n<AT, N, ALIGN>(pinv + i, d);
+#pragma unroll
+      for (int c = 0; c < N; ++c) out[c] = d[c] * out[c];
+    }
+  }
+};
+
+// Kernel B operand: z_i = P_i (r_i - alpha v_i); raw = s_i.
+template <class AT>
+struct OperandS {
+  const AT *__restrict__ r;
+  const AT *__restrict__ v;
+  const AT *__restrict__ pinv;
+  AT alpha;
+  __device__ __forceinline__ AT raw(int64_t i) const { return s_update(r[i], v[i], alpha); }
+  __device__ __forceinline__ AT operator()(int64_t i) const
+  {
+    const AT q = raw(i);
+    return pinv ? pinv[i] * q : q;
+  }
+  template <int N, int ALIGN>
+  __device__ __forceinline__ void load(int64_t i, AT (&out)[N]) const
+  {
+    AT rr[N], vv[N];
+    load_n<AT, N, ALIGN>(r + i, rr);
+    load_n<AT, N, ALIGN>(v + i, vv);
+#pragma unroll
+    for (int c = 0; c < N; ++c) out[c] = s_update(rr[c], vv[c], alpha);
+    if (pinv) {
+      AT d[N];
+      load_n<AT, N, ALIGN>(pinv + i, d);
+#pragma unroll
+      for (int c = 0; c < N; ++c) out[c] = d[c] * out[c];
+    }
+  }
+};
+
+// Residual refinement operand: xn_i = x_i + d_i (FP64 outer update).
+struct OperandXD {
+  const double *__restrict__ x;
+  const float *__restrict__ d;
+  __device__ __forceinline__ double raw(int64_t i) const { return x[i] + (double) d[i]; }
+  __device__ __forceinline__ double operator()(int64_t i) const { return raw(i); }
+  template <int N, int ALIGN>
+  __device__ __forceinline__ void load(int64_t i, double (&out)[N]) const
+  {
+    double xx[N];
+    float dd[N];
+    load_n<double, N, ALIGN>(x + i, xx);
+    load_n<float, N, cmin(ALIGN / 2 > 0 ? ALIGN / 2 : 1, 16)>(d + i, dd);
+#pragma unroll
+    for (int c = 0; c < N; ++c) out[c] = xx[c] + (double) dd[c];
+  }
+};
+
+// ---- fused SpMV body -------------------------------------------------------------------
+// Lane group of G lanes per scalar row (G = 1: thread per row); lane g owns
+// columns [g*VEC, (g+1)*VEC) of every block.  Returns, for lane 0 of the
+// group, y_row = sum_b A_(row,b) op(x_b) and the owned operand op.raw(row).
+template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED, class OP>
+__device__ __forceinline__ AT fused_row(const int *__restrict__ rp, const int *__restrict__ cols,
+                                        const VT *__restrict__ vals, const float *__restrict__ scale,
+                                        const OP &op, int br, int r, int g)
+{
+  constexpr int GA = BS / VEC;
+  constexpr int CC = VEC % 8 == 0 ? 8 : VEC;  // columns per step (bounds registers)
+  constexpr int VALIGN =
+      cmin(cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC)), lowbit((int) sizeof(VT) * CC));
+  constexpr int XALIGN =
+      cmin(cmin(lowbit((int) sizeof(AT) * BS), lowbit((int) sizeof(AT) * VEC)), lowbit((int) sizeof(AT) * CC));
+  AT acc = 0;
+  if (g < GA) {
+    const int b0 = rp[br], b1 = rp[br + 1];
+#pragma unroll 2
+    for (int b = b0; b < b1; ++b) {
+      const int64_t x0 = (int64_t) cols[b] * BS + g * VEC;
+      const VT *blk = vals + (int64_t) b * (BS * BS);
+#pragma unroll
+      for (int c0 = 0; c0 < VEC; c0 += CC) {
+        AT xv[CC];
+        op.template load<CC, XALIGN>(x0 + c0, xv);
+        if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+          VT v[CC];
+          load_n<VT, CC, VALIGN>(blk + r * BS + g * VEC + c0, v);
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+        } else {
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(blk[(g * VEC + c0 + c) * BS + r]), xv[c], acc);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
+  if constexpr (SCALED) acc *= (AT) scale[br];
+  return acc;
+}
+
+// ---- kernels ---------------------------------------------------------------------------
+template <class VT, class AT>
+struct MatArgs {
+  int n_brows;
+  const int *__restrict__ rp;
+  const int *__restrict__ cols;
+  const VT *__restrict__ vals;
+  const float *__restrict__ scale;
+};
+
+// Init: x = 0, r = r^ = b, p = v = 0 (both parities); partC = ((b,b), (b,b)).
+template <class AT>
+__global__ __launch_bounds__(256) void k_bicg_init(SolverVecs<AT> V, double eps, int max_iter)
+{
+  __shared__ double lds[2][4];
+  double part[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < V.n; i += (int64_t) gridDim.x * 256) {
+    const AT bi = V.b[i];
+    V.x[i] = 0;
+    V.r[i] = bi;
+    V.rh[i] = bi;
+    V.p[0][i] = 0;
+    V.p[1][i] = 0;
+    V.v[0][i] = 0;
+    V.v[1][i] = 0;
+    part[0] += (double) bi * (double) bi;
+  }
+  part[1] = part[0];
+  write_partials<2>(V.partC, part, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    SolverState *st = V.st;
+    for (int k = 0; k < 2; ++k) st->rho[k] = st->alpha[k] = st->omega[k] = st->tt[k] = 1.0;
+    st->norm_b2 = 0.0;
+    st->rr = 0.0;
+    st->eps = eps;
+    st->iter = st->iter_b = st->iter_c = 0;
+    st->done = 0;
+    st->status = BMFP_OK;
+    st->max_iter = max_iter;
+  }
+}
+
+template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  SolverState *st = V.st;
+  if (solver_stopped(st)) return;
+  const int it = st->iter;
+  double red[2];
+  reduce_partials<2>(V.partC, gridDim.x, red, lds);
+  const double rho_c = red[0], rr = red[1];
+  const double nb2 = it == 0 ? rr : st->norm_b2;
+  int stop = 1;  // 1: continue; otherwise the final status
+  if (it == 0 && rr == 0.0) stop = BMFP_OK;                         // b = 0: x = 0 is exact
+  else if (!isfinite(rr) || !isfinite(rho_c)) stop = BMFP_ERR_NONFINITE;
+  else if (it > 0 && sqrt(rr) / sqrt(nb2) < st->eps) stop = BMFP_OK;
+  else if (it > 0 && (st->tt[(it - 1) & 1] == 0.0 || st->omega[(it - 1) & 1] == 0.0)) stop = BMFP_ERR_BREAKDOWN;
+  else if (it >= st->max_iter) stop = BMFP_ERR_NOT_CONVERGED;
+  else if (rho_c == 0.0) stop = BMFP_ERR_BREAKDOWN;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rr;
+    if (it == 0) st->norm_b2 = rr;
+    if (stop != 1) {
+      st->status = stop;
+      st->done = 1;
+    } else {
+      st->rho[it & 1] = rho_c;
+      st->iter_b = it;
+    }
+  }
+  if (stop != 1) return;
+  const int pv = (it - 1) & 1, cu = it & 1;
+  const double rho_p = it == 0 ? 1.0 : st->rho[pv];
+  const double alpha_p = it == 0 ? 1.0 : st->alpha[pv];
+  const double omega_p = it == 0 ? 1.0 : st->omega[pv];
+  const double beta = rho_c / rho_p * alpha_p / omega_p;
+  OperandP<AT> op{V.r, V.p[pv], V.v[pv], V.pinv, (AT) beta, (AT) omega_p};
+  double part[1] = {0.0};
+  const int64_t threads = (int64_t) M.n_brows * BS * G;
+  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
+    const int64_t tt = base + threadIdx.x;
+    if (tt < threads) {
+      const int g = (int) (tt % G);
+      const int64_t row = tt / G;
+      const int br = (int) (row / BS), r = (int) (row % BS);
+      const AT y = fused_row<VT, AT, BS, G, VEC, LAYOUT, SCALED>(M.rp, M.cols, M.vals, M.scale, op, br, r, g);
+      if (g == 0) {
+        V.p[cu][row] = op.raw(row);
+        V.v[cu][row] = y;
+        part[0] += (double) V.rh[row] * (double) y;
+      }
+    }
+  }
+  write_partials<1>(V.partA, part, lds);
+}
+
+template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bicg_b(MatArgs<VT, AT> M, SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  SolverState *st = V.st;
+  if (solver_stopped(st)) return;
+  const int it = st->iter_b;
+  double red[1];
+  reduce_partials<1>(V.partA, gridDim.x, red, lds);
+  const double rhv = red[0];
+  const double rho_c = st->rho[it & 1];
+  const int stop = !isfinite(rhv) ? BMFP_ERR_NONFINITE : rhv == 0.0 ? BMFP_ERR_BREAKDOWN : 1;
+  const double alpha = rho_c / rhv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (stop != 1) {
+      st->status = stop;
+      st->done = 1;
+    } else {
+      st->alpha[it & 1] = alpha;
+      st->iter_c = it;
+    }
+  }
+  if (stop != 1) return;
+  OperandS<AT> op{V.r, V.v[it & 1], V.pinv, (AT) alpha};
+  double part[2] = {0.0, 0.0};
+  const int64_t threads = (int64_t) M.n_brows * BS * G;
+  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
+    const int64_t tt = base + threadIdx.x;
+    if (tt < threads) {
+      const int g = (int) (tt % G);
+      const int64_t row = tt / G;
+      const int br = (int) (row / BS), r = (int) (row % BS);
+      const AT y = fused_row<VT, AT, BS, G, VEC, LAYOUT, SCALED>(M.rp, M.cols, M.vals, M.scale, op, br, r, g);
+      if (g == 0) {
+        const AT s = op.raw(row);
+        V.s[row] = s;
+        V.t[row] = y;
+        part[0] += (double) y * (double) s;
+        part[1] += (double) y * (double) y;
+      }
+    }
+  }
+  write_partials<2>(V.partB, part, lds);
+}
+
+template <class AT>
+__global__ __launch_bounds__(256) void k_bicg_c(SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  SolverState *st = V.st;
+  if (solver_stopped(st)) return;
+  const int it = st->iter_c;
+  double red[2];
+  reduce_partials<2>(V.partB, gridDim.x, red, lds);
+  const double ts = red[0], tt = red[1];
+  const double omega = tt == 0.0 ? 0.0 : ts / tt;
+  const double alpha = st->alpha[it & 1];
+  const AT a = (AT) alpha, w = (AT) omega;
+  const AT *__restrict__ p = V.p[it & 1];
+  double part[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < V.n; i += (int64_t) gridDim.x * 256) {
+    const AT si = V.s[i];
+    const AT q = V.pinv ? V.pinv[i] * p[i] : p[i];
+    const AT z = V.pinv ? V.pinv[i] * si : si;
+    const AT h = fmadd(a, q, V.x[i]);
+    const AT rn = fmadd(-w, V.t[i], si);
+    V.x[i] = fmadd(w, z, h);
+    V.r[i] = rn;
+    part[0] += (double) V.rh[i] * (double) rn;
+    part[1] += (double) rn * (double) rn;
+  }
+  write_partials<2>(V.partC, part, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->omega[it & 1] = omega;
+    st->tt[it & 1] = tt;
+    st->iter = it + 1;
+  }
+}
+
+// Point-Jacobi inverse diagonal from BCSR diagonal blocks (bicgstab.cu:201-225:
+// 1/a_ii, or 1 when |a_ii| < 1e-20 or the diagonal block is absent).
+template <class VT, class AT, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_jacobi(MatArgs<VT, AT> M, int bs, AT *__restrict__ pinv)
+{
+  const int64_t row = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (row >= (int64_t) M.n_brows * bs) return;
+  const int br = (int) (row / bs), r = (int) (row % bs);
+  double d = 0.0;
+  bool found = false;
+  for (int b = M.rp[br]; b < M.rp[br + 1]; ++b)
+    if (M.cols[b] == br) {
+      const VT v = M.vals[(int64_t) b * bs * bs + (int64_t) r * bs + r];
+      d = (double) cvt<AT>(v);
+      if (SCALED) d *= (double) M.scale[br];
+      found = true;
+      break;
+    }
+  pinv[row] = (AT) ((!found || fabs(d) < 1e-20) ? 1.0 : 1.0 / d);
+}
+
+}  // namespace bmfp

@@ -95,6 +95,10 @@ int bmfp_event_create(bmfp_event *event);
 int bmfp_event_destroy(bmfp_event event);
 int bmfp_event_record(bmfp_event event, bmfp_stream stream);
 int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *seconds);
+/* Device-to-device copy by a plain 16-byte-per-lane streaming kernel (not
+ * the DMA engines): the measured-bandwidth yardstick for the SpMV kernels.
+ * bytes must be a multiple of 16 and both pointers 16-byte aligned. */
+int bmfp_stream_copy(void *dst, const void *src, size_t bytes, bmfp_stream stream);
 /* Deterministic device fill with the library's counter-based RNG:
  * kind 0 uniform[0,1), kind 1 uniform[-1,1), kind 2 constant `value`.
  * Element i of a vector filled with (seed, stream_id) is identical to the
@@ -164,6 +168,40 @@ int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y,
                     double *avg_seconds);
 /* The family BMFP_SPMV_AUTO resolves to for A. */
 int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant);
+
+/* ---- BiCGStab ------------------------------------------------------------
+ * Replaces gpu_bicgstab<T,C> (bicgstab.h:12-47, bicgstab.cu:23-526).  The
+ * solver keeps its Krylov vectors and scalars resident; each iteration is
+ * three fused kernels (two SpMVs with the vector updates formed on the fly,
+ * one update kernel) whose dot products are reduced on the device in a fixed
+ * order, so results are bitwise reproducible.  Convergence test as in the
+ * reference: recurrence ||r||/||b|| < eps (bicgstab.cu:512).  Breakdowns
+ * ((r^,v) = 0, rho = 0, omega = 0) return BMFP_ERR_BREAKDOWN with x the last
+ * valid iterate instead of NaNs; t = 0 (s = 0, exact solution) converges.
+ * The vector type follows the matrix: F64 matrices solve in FP64, F32 and F16
+ * (per-block-row scaled) matrices in FP32 vectors with FP64 reductions. */
+typedef struct {
+  int iterations;           /* BiCGStab iterations (total inner ones for refinement)  */
+  int status;               /* BMFP_OK, _NOT_CONVERGED, _BREAKDOWN or _NONFINITE       */
+  double rel_residual;      /* recurrence ||r||/||b|| at exit                          */
+  double true_rel_residual; /* ||b - A x||/||b|| recomputed in FP64 after the solve    */
+  double seconds;           /* device time of the solve (hipEvents on its stream)      */
+  int outer_iterations;     /* refinement steps (0 for plain BiCGStab)                 */
+  int inner_iterations;     /* inner BiCGStab iterations summed over refinement steps  */
+} bmfp_solve_info;
+
+/* use_jacobi: point-Jacobi preconditioning (bicgstab.cu:201-238). */
+int bmfp_bicgstab_create(bmfp_solver *out, bmfp_matrix A, int use_jacobi);
+/* b, x: device vectors of A's vector type; x is output only (x0 = 0, as
+ * bicgstab_init_kernel, bicgstab.cu:257-284).  Synchronous: returns when the
+ * solve has stopped.  Returns info->status (BMFP_OK when converged). */
+int bmfp_bicgstab_solve(bmfp_solver s, const void *b, void *x, double eps,
+                        int max_iterations, bmfp_stream stream,
+                        bmfp_solve_info *info);
+int bmfp_bicgstab_destroy(bmfp_solver s);
+/* Device bytes the solver for A holds beyond A itself (the reference's
+ * ctor memory estimate, bicgstab.cu:30-45). */
+int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes);
 
 #ifdef __cplusplus
 }

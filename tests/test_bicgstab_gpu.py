@@ -1,0 +1,145 @@
+"""GPU BiCGStab (bmfp_bicgstab_*) against the CPU oracle restatement of
+bicgstab.cu:388-513.
+
+Bar: same stop reason; iteration counts within a small window (the device
+reduces dot products in a different, fixed order, so trajectories agree to
+rounding); solution within 1e-9 (relative 2-norm) of the oracle's solution at
+eps = 1e-10 and of the manufactured solution where the system is well
+conditioned; bitwise-identical results run to run.
+"""
+import numpy as np
+import pytest
+
+import block_matrix_format_performance_amd as bm
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _name_the_case(request, device):
+    print(f"\n[gpu-case] {request.node.nodeid}", flush=True)
+    yield
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def gpu_matrix(m, layout=bm.ROW_MAJOR, dtype=bm.F64):
+    vals = m.values if layout == bm.ROW_MAJOR else oracle.transpose_blocks(m)
+    return bm.Matrix.bcsr(m.row_ptr, m.columns, vals, m.n_rows, m.n_cols, m.bs, dtype=dtype, layout=layout)
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_config0_like_b_is_A_times_ones(jacobi):
+    # diagonally dominant 9-point stencil, bs=2, b = A*1, x0 = 0
+    m = oracle.gen_stencil9(48, 40, 2, 0, 0)
+    c = oracle.bcsr_to_csr(m)
+    b = oracle.csr_spmv(c, np.ones(c.n_rows))
+    A = gpu_matrix(m)
+    x, info = bm.BiCGStab(A, use_jacobi=jacobi).solve_host(b, eps=1e-10, max_iterations=1000)
+    xo, oi = oracle.bicgstab(c, b, 1e-10, 1000, use_jacobi=jacobi)
+    assert info.status == bm.OK and oi.status == 0
+    assert abs(info.iterations - oi.iterations) <= 2, (info.iterations, oi.iterations)
+    assert info.rel_residual < 1e-10
+    assert info.true_rel_residual < 1e-9
+    assert rel(x, xo) < 1e-9
+    assert rel(x, np.ones_like(x)) < 1e-9
+
+
+@pytest.mark.parametrize("bs", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32])
+@pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
+def test_convective_system_all_block_sizes(bs, layout):
+    # config-2 type matrix (-(K+0.3C) off-diagonal blocks), x* uniform[-1,1]
+    side = max(3, 48 // bs)
+    m = oracle.gen_stencil9(side, side - 1, bs, 1, 0)
+    c = oracle.bcsr_to_csr(m)
+    xs = oracle.rng_sym_vec(1, 5, c.n_rows)
+    b = oracle.csr_spmv(c, xs)
+    x, info = bm.BiCGStab(gpu_matrix(m, layout)).solve_host(b, eps=1e-10, max_iterations=5000)
+    xo, oi = oracle.bicgstab(c, b, 1e-10, 5000)
+    assert info.status == bm.OK and oi.status == 0
+    assert abs(info.iterations - oi.iterations) <= max(3, oi.iterations // 20)
+    assert rel(x, xo) < 1e-8
+    assert info.true_rel_residual < 1e-9
+
+
+def test_results_bitwise_reproducible():
+    m = oracle.gen_stencil9(30, 30, 4, 1, 3)
+    c = oracle.bcsr_to_csr(m)
+    b = oracle.csr_spmv(c, oracle.rng_sym_vec(1, 5, c.n_rows))
+    S = bm.BiCGStab(gpu_matrix(m))
+    x1, i1 = S.solve_host(b, 1e-10, 2000)
+    x2, i2 = S.solve_host(b, 1e-10, 2000)
+    assert i1.iterations == i2.iterations
+    assert np.array_equal(x1, x2)
+
+
+def test_identity_converges_in_one_iteration():
+    n = 37
+    A = bm.Matrix.bcsr(np.arange(n + 1), np.arange(n), np.ones(n), n, n, 1)
+    b = oracle.rng_sym_vec(0, 6, n)
+    x, info = bm.BiCGStab(A).solve_host(b, 1e-12, 10)
+    assert info.status == bm.OK and info.iterations == 1
+    np.testing.assert_array_equal(x, b)
+
+
+def test_breakdown_reported_not_nan():
+    A = bm.Matrix.bcsr([0, 1, 2], [1, 0], [1.0, -1.0], 2, 2, 1)
+    x, info = bm.BiCGStab(A).solve_host(np.array([1.0, 0.0]), 1e-12, 10)
+    assert info.status == bm.ERR_BREAKDOWN
+    assert np.all(np.isfinite(x))
+
+
+def test_max_iterations_reported():
+    m = oracle.gen_stencil9(20, 20, 2, 1, 0)
+    c = oracle.bcsr_to_csr(m)
+    b = oracle.csr_spmv(c, np.ones(c.n_rows))
+    x, info = bm.BiCGStab(gpu_matrix(m)).solve_host(b, 1e-14, 3)
+    xo, oi = oracle.bicgstab(c, b, 1e-14, 3)
+    assert info.status == bm.ERR_NOT_CONVERGED and oi.status == 1
+    assert info.iterations == 3
+    assert rel(x, xo) < 1e-10
+
+
+def test_zero_rhs_gives_zero():
+    m = oracle.gen_stencil9(10, 10, 3, 0, 0)
+    x, info = bm.BiCGStab(gpu_matrix(m)).solve_host(np.zeros(300), 1e-10, 100)
+    assert info.status == bm.OK and info.iterations == 0
+    assert np.all(x == 0)
+
+
+def test_nonfinite_rhs_reported():
+    m = oracle.gen_stencil9(10, 10, 2, 0, 0)
+    b = np.ones(200)
+    b[17] = np.nan
+    x, info = bm.BiCGStab(gpu_matrix(m)).solve_host(b, 1e-10, 100)
+    assert info.status == bm.ERR_NONFINITE
+
+
+@pytest.mark.parametrize("dtype,tol", [(bm.F32, 1e-5), (bm.F16, 1e-3)])
+def test_low_precision_solves_reach_their_tolerance(dtype, tol):
+    m = oracle.gen_stencil9(40, 36, 4, 0, 0)
+    c = oracle.bcsr_to_csr(m)
+    xs = oracle.rng_sym_vec(1, 5, c.n_rows)
+    A = gpu_matrix(m, dtype=dtype)
+    dv = A.download_values()   # the stored (rounded / scaled) matrix
+    cl = oracle.bcsr_to_csr(oracle.BCSR(m.n_rows, m.n_cols, m.bs, m.row_ptr, m.columns, dv))
+    b = oracle.csr_spmv(cl, xs).astype(np.float32)
+    x, info = bm.BiCGStab(A).solve_host(b, tol, 2000)
+    assert info.status == bm.OK
+    r = b.astype(np.float64) - oracle.csr_spmv(cl, x.astype(np.float64))
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 10 * tol
+
+
+def test_solver_rejects_bad_arguments():
+    m = oracle.gen_stencil9(4, 4, 2, 0, 0)
+    S = bm.BiCGStab(gpu_matrix(m))
+    db = bm.DeviceArray(32)
+    dx = bm.DeviceArray(32)
+    with pytest.raises(bm.BmfpError) as e:
+        S.solve(db, dx, -1.0, 10)
+    assert e.value.status == bm.ERR_INVALID
+    with pytest.raises(bm.BmfpError):
+        bm.BiCGStab(bm.Matrix.bcsr([0, 1], [1], np.ones(4), 1, 2, 2))  # not square
