@@ -173,6 +173,19 @@ extern "C" int bmfp_free(void *ptr)
   return BMFP_OK;
 }
 
+extern "C" int bmfp_malloc_host(void **ptr, size_t bytes)
+{
+  if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
+  BMFP_HIP(hipHostMalloc(ptr, bytes ? bytes : 64, hipHostMallocDefault));
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_free_host(void *ptr)
+{
+  if (ptr) BMFP_HIP(hipHostFree(ptr));
+  return BMFP_OK;
+}
+
 extern "C" int bmfp_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
   if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));

@@ -10,12 +10,14 @@ namespace bmfp {
 
 namespace {
 
-// XCD-aware tile order; BMFP_XCD_REMAP=0 switches it off (for measurement).
+// XCD-contiguous tile order (tile_id).  Off by default: measured on the
+// 9-point stencil sweep it is neutral to slightly slower than the hardware's
+// round-robin order.  BMFP_XCD_REMAP=1 switches it on.
 int xcd_remap_enabled()
 {
   static int v = [] {
     const char *e = std::getenv("BMFP_XCD_REMAP");
-    return e ? std::atoi(e) : 1;
+    return e ? std::atoi(e) : 0;
   }();
   return v;
 }
@@ -38,12 +40,15 @@ Grid make_grid(int64_t threads)
   return out;
 }
 
-// Lanes per row / per block row, by block size and value width.
+// Columns per lane in the lane-group-per-row kernel: one 16-byte load, or
+// two for FP64 blocks of 12 and more columns (fewer lanes per row, more bytes
+// in flight per lane: +3-4 points of HBM peak measured at bs 12-32).
 template <class VT> constexpr int gpr_vec(int bs)
 {
   constexpr int W = 16 / (int) sizeof(VT);  // elements per 16-byte load
   int v = 1;
   while (v < W && bs % (2 * v) == 0) v *= 2;
+  if (sizeof(VT) == 8 && bs >= 12 && bs % 4 == 0) v = 4;
   return v;
 }
 constexpr int bgr_lanes(int bs) { return bs == 1 ? 16 : bs <= 3 ? 32 : 64; }
@@ -166,9 +171,13 @@ bool bcsr_supported(int variant, int dtype, int layout, int bs)
 int auto_variant(const bmfp_matrix_t &A)
 {
   if (!A.is_bcsr) return A.mean_row_blocks >= 4.0 ? BMFP_CSR_VECTOR : BMFP_CSR_SCALAR;
+  // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
+  // matrices (FP64, ~4.4e8 stored values): batched thread-per-row for small
+  // blocks; 16-byte lane slices (row-major) or shuffle-shared x segments
+  // (column-major) from bs = 4 up.
   if (!precompiled_bs(A.bs)) return BMFP_BCSR_THREAD_PER_ROW;
-  if (A.layout == BMFP_COL_MAJOR) return A.bs >= 2 ? BMFP_BCSR_BLOCK_ROW_GROUP : BMFP_BCSR_THREAD_PER_ROW;
-  return A.bs >= 4 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_THREAD_PER_ROW;
+  if (A.bs <= 3) return BMFP_BCSR_THREAD_PER_ROW;
+  return A.layout == BMFP_COL_MAJOR ? BMFP_BCSR_SHUFFLE_X : BMFP_BCSR_GROUP_PER_ROW;
 }
 
 int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)

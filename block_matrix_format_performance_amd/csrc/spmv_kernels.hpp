@@ -94,6 +94,60 @@ struct BcsrArgs {
 // the register footprint bounded for large BS.
 template <int BS> constexpr int col_chunk() { return BS % 8 == 0 ? 8 : BS; }
 
+// Blocks whose loads a thread issues together before using them (thread-per-
+// row kernels).  Small blocks carry few bytes per block, so a whole 9-point
+// row is fetched at once to keep enough bytes in flight; 0 = plain loop.
+template <int BS> constexpr int tpr_batch() { return BS <= 2 ? 9 : BS == 3 ? 3 : 0; }
+
+// Row r of one block (BS values in the thread's layout).
+template <class VT, int BS, int LAYOUT>
+__device__ __forceinline__ void load_block_row(const VT *__restrict__ blk, int r, VT (&v)[BS])
+{
+  if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+    load_n<VT, BS, lowbit((int) sizeof(VT) * BS)>(blk + r * BS, v);
+  } else {
+#pragma unroll
+    for (int c = 0; c < BS; ++c) v[c] = blk[c * BS + r];
+  }
+}
+
+// Sum over blocks [b0, b1) of row r of each block times its x segment.  Whole
+// batches of U blocks issue all their loads (indices, values, x) before any
+// is used; the remaining < U blocks go one at a time.  Unpredicated batches
+// matter: per-block guards inside a batch serialise the loads.
+template <class VT, class AT, int BS, int LAYOUT, int U>
+__device__ __forceinline__ AT tpr_batched(const int *__restrict__ cols, const VT *__restrict__ vals,
+                                          const AT *__restrict__ x, int b0, int b1, int r)
+{
+  constexpr int XALIGN = lowbit((int) sizeof(AT) * BS);
+  AT acc = 0;
+  int b = b0;
+  for (; b + U <= b1; b += U) {
+    int col[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) col[k] = cols[b + k];
+    VT v[U][BS];
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_block_row<VT, BS, LAYOUT>(vals + (int64_t) (b + k) * (BS * BS), r, v[k]);
+    AT xv[U][BS];
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_n<AT, BS, XALIGN>(x + (int64_t) col[k] * BS, xv[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[k][c]), xv[k][c], acc);
+  }
+  for (; b < b1; ++b) {
+    VT v[BS];
+    AT xv[BS];
+    load_block_row<VT, BS, LAYOUT>(vals + (int64_t) b * (BS * BS), r, v);
+    load_n<AT, BS, XALIGN>(x + (int64_t) cols[b] * BS, xv);
+#pragma unroll
+    for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+  }
+  return acc;
+}
+
 // ============================================================================
 // Thread per scalar row.  Row-major: the thread's row of each block is BS
 // contiguous values, loaded in 16-byte pieces; the BS threads of a block row
@@ -109,26 +163,30 @@ __global__ __launch_bounds__(256) void k_bcsr_tpr(BcsrArgs<VT, AT> a)
   const int br = (int) br64;
   const int r = (int) (t - br64 * BS);
   const int b0 = a.rp[br], b1 = a.rp[br + 1];
-  constexpr int CC = col_chunk<BS>();
-  constexpr int XALIGN = lowbit((int) sizeof(AT) * BS);
-  constexpr int VALIGN = lowbit((int) sizeof(VT) * BS);
   AT acc = 0;
+  if constexpr (tpr_batch<BS>() > 0) {
+    acc = tpr_batched<VT, AT, BS, LAYOUT, tpr_batch<BS>()>(a.cols, a.vals, a.x, b0, b1, r);
+  } else {
+    constexpr int CC = col_chunk<BS>();
+    constexpr int XALIGN = lowbit((int) sizeof(AT) * BS);
+    constexpr int VALIGN = lowbit((int) sizeof(VT) * BS);
 #pragma unroll 3
-  for (int b = b0; b < b1; ++b) {
-    const AT *xs = a.x + (int64_t) a.cols[b] * BS;
-    const VT *blk = a.vals + (int64_t) b * (BS * BS);
+    for (int b = b0; b < b1; ++b) {
+      const AT *xs = a.x + (int64_t) a.cols[b] * BS;
+      const VT *blk = a.vals + (int64_t) b * (BS * BS);
 #pragma unroll
-    for (int c0 = 0; c0 < BS; c0 += CC) {
-      AT xv[CC];
-      load_n<AT, CC, cmin(XALIGN, lowbit((int) sizeof(AT) * CC))>(xs + c0, xv);
-      if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
-        VT v[CC];
-        load_n<VT, CC, cmin(VALIGN, lowbit((int) sizeof(VT) * CC))>(blk + r * BS + c0, v);
+      for (int c0 = 0; c0 < BS; c0 += CC) {
+        AT xv[CC];
+        load_n<AT, CC, cmin(XALIGN, lowbit((int) sizeof(AT) * CC))>(xs + c0, xv);
+        if constexpr (LAYOUT == BMFP_ROW_MAJOR) {
+          VT v[CC];
+          load_n<VT, CC, cmin(VALIGN, lowbit((int) sizeof(VT) * CC))>(blk + r * BS + c0, v);
 #pragma unroll
-        for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
-      } else {
+          for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+        } else {
 #pragma unroll
-        for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(blk[(c0 + c) * BS + r]), xv[c], acc);
+          for (int c = 0; c < CC; ++c) acc = fmadd(cvt<AT>(blk[(c0 + c) * BS + r]), xv[c], acc);
+        }
       }
     }
   }

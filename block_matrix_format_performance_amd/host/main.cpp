@@ -1,0 +1,264 @@
+// main.cpp -- benchmark driver, drop-in for the reference's main.cpp:
+// generates a matrix, times the single-threaded CPU CSR SpMV and every GPU
+// SpMV family, prints a speed-up table and writes result.json
+// ({ "<bs>": { "<format>": mean_seconds } }).  Without arguments it runs the
+// reference's published workload (FP32, N = 50,000 block rows, 6 blocks per
+// row, bs in {2,4,8,16,32}, x = 1; main.cpp:386-398).
+//
+// Options (all optional):
+//   --type float|double        value type (default float)
+//   --bs 2,4,8,16,32           block sizes
+//   --matrix ndiag|stencil     gen_n_diag_bcsr (default) or the seeded
+//                              9-point block stencil (kind 0, seed 0)
+//   --n-rows N --bpr K         ndiag shape (default 50000, 6)
+//   --values V                 stencil: grid side round(sqrt(V/(9 bs^2)))
+//   --x ones|random            x = 1 (reference) or uniform[0,1] (seed 0)
+//   --reps R                   repetitions averaged (default 10)
+//   --out FILE                 JSON output (default result.json)
+//   --solve                    also run the FP64 BiCGStab of the 256x256
+//                              bs=2 stencil (b = A*1, x0 = 0, eps 1e-8) and
+//                              report its iterations, error and time
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <optional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "bmfp/bicgstab.h"
+#include "bmfp/gpu_matrix_multiplier.h"
+#include "bmfp/matrix_converters.h"
+#include "bmfp/measurement_class.h"
+
+namespace {
+
+struct Options {
+  std::string type = "float";
+  std::vector<int> bs = {2, 4, 8, 16, 32};
+  std::string matrix = "ndiag";
+  int n_rows = 50000;
+  int bpr = 6;
+  double values = 4.4e7;
+  std::string x = "ones";
+  int reps = 10;
+  std::string out = "result.json";
+  bool solve = false;
+};
+
+// main.cpp:21-63 with x as an input (the reference overwrites x with 1):
+// y_row = sum over the row, in CSR order, of a*x[col]; bytes counted the
+// reference's way (x per nonzero, main.cpp:50-56).
+template <typename T, typename I>
+measurement_class cpu_csr_spmv_single_thread_naive(const csr_matrix_class<T, I> &m, const T *x, T *y)
+{
+  std::fill_n(y, m.n_rows, T(0));
+  const I *rp = m.row_ptr.get();
+  const I *cols = m.columns.get();
+  const T *vals = m.values.get();
+  const auto begin = std::chrono::steady_clock::now();
+  for (I row = 0; row < m.n_rows; row++) {
+    T dot = 0;
+    for (I e = rp[row]; e < rp[row + 1]; e++) dot += vals[e] * x[cols[e]];
+    y[row] = dot;
+  }
+  const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
+  const double bytes = (double) m.nnz * sizeof(T) * 2 + (double) m.nnz * sizeof(I) + 2.0 * m.n_rows * sizeof(I) +
+                       (double) m.n_rows * sizeof(T);
+  return measurement_class("CPU CSR", elapsed, bytes, 2.0 * m.nnz);
+}
+
+// main.cpp:70-113 without colours.
+class time_printer
+{
+  double reference;
+
+public:
+  explicit time_printer(double reference_time) : reference(reference_time) {}
+  void print_time(const measurement_class &m) const
+  {
+    std::cout << "\t" << std::left << std::setw(64) << m.get_format() << ":  " << std::setw(14)
+              << std::setprecision(6) << m.get_elapsed() << std::setw(10) << reference / m.get_elapsed()
+              << std::setw(10) << m.get_effective_bandwidth() << " GB/s" << std::endl;
+  }
+};
+
+template <typename T, typename I>
+std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matrix,
+                                                   bcsr_matrix_class<T, I> &block_matrix, const Options &o)
+{
+  std::map<std::string, double> results;
+  const size_t n = (size_t) matrix.n_rows;
+  std::vector<T> x(n), reference(n);
+  if (o.x == "random") {
+    std::vector<double> xd(n);
+    // uniform[0,1) from the library's counter-based generator, stream 4, seed 0
+    void *d = nullptr;
+    bmfp_malloc(&d, n * sizeof(double));
+    bmfp_fill(d, BMFP_F64, n, 0, 0, 4, 0.0, nullptr);
+    bmfp_memcpy_d2h(xd.data(), d, n * sizeof(double));
+    bmfp_free(d);
+    std::copy(xd.begin(), xd.end(), x.begin());
+  } else {
+    std::fill(x.begin(), x.end(), T(1));
+  }
+  const T *xp = o.x == "random" ? x.data() : nullptr;
+
+  auto measure_multiple_times = [&](const std::function<measurement_class()> &action) {
+    measurement_class acc;
+    for (int i = 0; i < o.reps; i++) acc += action();
+    acc.finalize();
+    results[acc.get_format()] = acc.get_elapsed();
+    return acc;
+  };
+
+  auto cpu = measure_multiple_times(
+      [&] { return cpu_csr_spmv_single_thread_naive(matrix, x.data(), reference.data()); });
+  time_printer printer(cpu.get_elapsed());
+  printer.print_time(cpu);
+  printer.print_time(measure_multiple_times(
+      [&] { return gpu_csr_spmv<T, I>(matrix, xp, reference.data(), nullptr); }));
+  printer.print_time(measure_multiple_times(
+      [&] { return gpu_csr_vector_spmv<T, I>(matrix, xp, reference.data(), nullptr); }));
+
+  std::vector<T> transposed((size_t) block_matrix.size());
+  block_matrix.transpose_blocks(transposed.data());
+  // o.reps calls, averaged by a default-constructed accumulator per format
+  // (the reference seeds the accumulators with an 11th call, main.cpp:296-307).
+  std::vector<measurement_class> acc;
+  for (int i = 0; i < o.reps; i++) {
+    auto r = gpu_bcsr_spmv<T, I>(block_matrix, transposed.data(), xp, reference.data());
+    if (acc.empty()) acc.resize(r.size());
+    for (size_t k = 0; k < r.size(); k++) acc[k] += r[k];
+  }
+  for (auto &m : acc) {
+    m.finalize();
+    results[m.get_format()] = m.get_elapsed();
+    printer.print_time(m);
+  }
+  return results;
+}
+
+std::string json_escape(const std::string &s)
+{
+  std::string o;
+  for (char c : s) {
+    if (c == '"' || c == '\\') o += '\\';
+    o += c;
+  }
+  return o;
+}
+
+std::string json_number(double v)
+{
+  if (!std::isfinite(v)) return "null";
+  std::ostringstream ss;
+  ss << std::setprecision(17) << v;
+  return ss.str();
+}
+
+template <typename T>
+std::map<std::string, double> run_bs(int bs, const Options &o)
+{
+  std::unique_ptr<bcsr_matrix_class<T, int>> block_matrix;
+  if (o.matrix == "stencil") {
+    const int side = (int) std::lround(std::sqrt(o.values / (9.0 * bs * bs)));
+    std::cout << "\nBS: " << bs << " (9-point stencil " << side << "x" << side << ")\n";
+    block_matrix = gen_stencil9_bcsr<T, int>(side, side, bs, 0, 0);
+  } else {
+    std::cout << "\nBS: " << bs << "\n";
+    block_matrix = gen_n_diag_bcsr<T, int>(o.n_rows, o.bpr, bs);
+  }
+  csr_matrix_class<T, int> matrix(*block_matrix);
+  return perform_measurements(matrix, *block_matrix, o);
+}
+
+int run_solve()
+{
+  // 256x256 node grid, bs = 2, diagonally dominant, b = A*1, x0 = 0, eps 1e-8
+  auto A = gen_stencil9_bcsr<double, int>(256, 256, 2, 0, 0);
+  const size_t n = (size_t) A->n_rows * A->bs;
+  csr_matrix_class<double, int> csr(*A);
+  std::vector<double> ones(n, 1.0), b(n);
+  cpu_csr_spmv_single_thread_naive(csr, ones.data(), b.data());
+  gpu_bicgstab<double, int> solver(*A, false);
+  const double *x = solver.solve(*A, b.data(), 1e-8, 10000);
+  double err = 0;
+  for (size_t i = 0; i < n; i++) err = std::max(err, std::fabs(x[i] - 1.0));
+  const bmfp_solve_info &info = solver.last_info();
+  std::cout << "BiCGStab 256x256 bs=2: iterations " << info.iterations << ", ||r||/||b|| " << info.rel_residual
+            << ", true " << info.true_rel_residual << ", max|x-1| " << err << ", " << info.seconds * 1e3 << " ms"
+            << std::endl;
+  return info.status == BMFP_OK ? 0 : 1;
+}
+
+std::vector<int> parse_list(const char *s)
+{
+  std::vector<int> v;
+  std::stringstream ss(s);
+  std::string tok;
+  while (std::getline(ss, tok, ',')) v.push_back(std::atoi(tok.c_str()));
+  return v;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+  Options o;
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto next = [&]() -> const char * {
+      if (i + 1 >= argc) {
+        std::cerr << "missing value for " << a << std::endl;
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (a == "--type") o.type = next();
+    else if (a == "--bs") o.bs = parse_list(next());
+    else if (a == "--matrix") o.matrix = next();
+    else if (a == "--n-rows") o.n_rows = std::atoi(next());
+    else if (a == "--bpr") o.bpr = std::atoi(next());
+    else if (a == "--values") o.values = std::atof(next());
+    else if (a == "--x") o.x = next();
+    else if (a == "--reps") o.reps = std::atoi(next());
+    else if (a == "--out") o.out = next();
+    else if (a == "--solve") o.solve = true;
+    else {
+      std::cerr << "unknown option " << a << std::endl;
+      return 2;
+    }
+  }
+  if (bmfp_init(0) != BMFP_OK) {
+    std::cerr << "bmfp_init: " << bmfp_last_error() << std::endl;
+    return 1;
+  }
+  std::map<int, std::map<std::string, double>> all;
+  for (int bs : o.bs) all[bs] = o.type == "double" ? run_bs<double>(bs, o) : run_bs<float>(bs, o);
+
+  std::ofstream os(o.out);
+  os << "{\n";
+  size_t i = 0;
+  for (const auto &kv : all) {
+    os << "  \"" << kv.first << "\": {\n";
+    size_t j = 0;
+    for (const auto &r : kv.second)
+      os << "    \"" << json_escape(r.first) << "\": " << json_number(r.second)
+         << (++j < kv.second.size() ? ",\n" : "\n");
+    os << "  }" << (++i < all.size() ? ",\n" : "\n");
+  }
+  os << "}\n";
+  os.close();
+  std::cout << "wrote " << o.out << std::endl;
+  return o.solve ? run_solve() : 0;
+}

@@ -80,6 +80,10 @@ int bmfp_device_info(char *name, size_t name_len, size_t *total_mem_bytes,
                      int *compute_units, int *gcn_arch_major_minor);
 int bmfp_malloc(void **ptr, size_t bytes);
 int bmfp_free(void *ptr);
+/* Pinned (page-locked) host memory, as the reference's cudaMallocHost for
+ * the solver result (bicgstab.cu:62). */
+int bmfp_malloc_host(void **ptr, size_t bytes);
+int bmfp_free_host(void *ptr);
 int bmfp_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int bmfp_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int bmfp_memcpy_d2d(void *dst, const void *src, size_t bytes);
