@@ -119,6 +119,9 @@ def _declare_solver(L):
     L.bmfp_bicgstab_solve.argtypes = [vp, vp, vp, c_dbl, c_int, vp, ctypes.POINTER(SolveInfo)]
     L.bmfp_bicgstab_destroy.argtypes = [vp]
     L.bmfp_bicgstab_workspace_bytes.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_size_t)]
+    L.bmfp_refine_create.argtypes = [ctypes.POINTER(vp), vp, c_int, c_int]
+    L.bmfp_refine_solve.argtypes = [vp, vp, vp, c_dbl, c_dbl, c_int, c_int, vp, ctypes.POINTER(SolveInfo)]
+    L.bmfp_refine_destroy.argtypes = [vp]
 
 
 class SolveInfo(ctypes.Structure):
@@ -126,7 +129,8 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int),
                 ("rel_residual", ctypes.c_double), ("true_rel_residual", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("outer_iterations", ctypes.c_int),
-                ("inner_iterations", ctypes.c_int)]
+                ("inner_iterations", ctypes.c_int), ("inner_dtype", ctypes.c_int),
+                ("escalations", ctypes.c_int)]
 
 
 def check(status):
@@ -404,6 +408,41 @@ class BiCGStab:
     def destroy(self):
         if self.handle is not None:
             lib().bmfp_bicgstab_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Refine:
+    """bmfp_refine_*: FP64 iterative refinement around an FP32 / FP16 BiCGStab."""
+
+    def __init__(self, A64, inner_dtype=F32, use_jacobi=False):
+        self.A = A64
+        h = ctypes.c_void_p()
+        check(lib().bmfp_refine_create(ctypes.byref(h), A64.handle, inner_dtype, 1 if use_jacobi else 0))
+        self.handle = h
+
+    def solve(self, b, x, eps=1e-10, inner_tol=1e-5, max_outer=50, max_inner=10000, stream=None):
+        info = SolveInfo()
+        st = lib().bmfp_refine_solve(self.handle, b.ptr, x.ptr, eps, inner_tol, max_outer, max_inner, stream,
+                                     ctypes.byref(info))
+        if st not in SOLVE_STATUSES:
+            check(st)
+        return info
+
+    def solve_host(self, b, eps=1e-10, inner_tol=1e-5, max_outer=50, max_inner=10000):
+        db = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))
+        dx = DeviceArray(self.A.y_len, np.float64)
+        info = self.solve(db, dx, eps, inner_tol, max_outer, max_inner)
+        return dx.download(), info
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().bmfp_refine_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -218,7 +218,7 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
 {
   if (!(eps > 0.0) || max_iterations < 0) return fail(BMFP_ERR_INVALID, "eps must be > 0, max_iterations >= 0");
   if (S.n == 0) {
-    if (info) *info = bmfp_solve_info{0, BMFP_OK, 0.0, 0.0, 0.0, 0, 0};
+    if (info) *info = bmfp_solve_info{};
     return BMFP_OK;
   }
   if (!b || !x) return fail(BMFP_ERR_INVALID, "NULL vector");
@@ -283,6 +283,7 @@ using namespace bmfp;
 
 bmfp_solver_t::~bmfp_solver_t()
 {
+  if (refine) refine_destroy(refine);
   if (graph) (void) hipGraphExecDestroy(graph);
   if (h_state) (void) hipHostFree(h_state);
   if (ev0) (void) hipEventDestroy(ev0);
@@ -311,7 +312,7 @@ extern "C" int bmfp_bicgstab_create(bmfp_solver *out, bmfp_matrix A, int use_jac
 extern "C" int bmfp_bicgstab_solve(bmfp_solver s, const void *b, void *x, double eps, int max_iterations,
                                    bmfp_stream stream, bmfp_solve_info *info)
 {
-  if (!s) return fail(BMFP_ERR_INVALID, "solver is NULL");
+  if (!s || s->refine) return fail(BMFP_ERR_INVALID, "not a BiCGStab handle");
   const int st = solver_run(*s, b, x, eps, max_iterations, static_cast<hipStream_t>(stream), info);
   if (st == BMFP_ERR_NOT_CONVERGED) set_error("BiCGStab: max_iterations (%d) reached", max_iterations);
   else if (st == BMFP_ERR_BREAKDOWN) set_error("BiCGStab breakdown");

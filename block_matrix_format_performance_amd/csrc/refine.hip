@@ -1,0 +1,255 @@
+// refine.hip -- mixed-precision iterative refinement (bmfp_refine_*): FP64
+// outer loop on the FP64 matrix, FP32-vector BiCGStab inner solves on an FP32
+// or FP16+scale copy of the matrix, escalating FP16 -> FP32 on stagnation.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "matrix.hpp"
+#include "solver.hpp"
+#include "solver_kernels.hpp"
+
+namespace bmfp {
+
+struct RefineState {
+  bmfp_matrix_t *A = nullptr;                // FP64 matrix (borrowed)
+  int use_jacobi = 0;
+  int dtype = BMFP_F32;                      // current inner storage type
+  std::unique_ptr<bmfp_matrix_t> low[3];     // indexed by dtype (F32, F16 used)
+  std::unique_ptr<bmfp_solver_t> inner[3];
+  DeviceBuffer x;                            // two FP64 iterates (ping-pong)
+  DeviceBuffer r32, d32;                     // FP32 residual / correction
+  DeviceBuffer part;                         // residual partial sums
+  int np = 1;
+  int64_t n = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_sync = nullptr;
+  int (*resid)(const RefineState &, const double *b, int xin, hipStream_t) = nullptr;
+  ~RefineState()
+  {
+    if (ev0) (void) hipEventDestroy(ev0);
+    if (ev1) (void) hipEventDestroy(ev1);
+    if (ev_sync) (void) hipEventDestroy(ev_sync);
+    if (stream) (void) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+constexpr int resid_vec(int bs, int layout)
+{
+  if (layout != BMFP_ROW_MAJOR || bs < 8) return bs;
+  int v = 1;
+  while (v < 2 && bs % (2 * v) == 0) v *= 2;
+  return v;
+}
+
+template <int BS, int LAYOUT>
+struct Resid {
+  static constexpr int VEC = resid_vec(BS, LAYOUT);
+  static constexpr int G = pow2ceil(BS / VEC);
+  static int groups(const bmfp_matrix_t &A)
+  {
+    const int64_t t = (int64_t) A.n_rows * BS * G;
+    return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
+  }
+  static int run(const RefineState &R, const double *b, int xin, hipStream_t s)
+  {
+    MatArgs<double, double> M;
+    M.n_brows = R.A->n_rows;
+    M.rp = R.A->row_ptr->as<int>();
+    M.cols = R.A->columns->as<int>();
+    M.vals = R.A->values.as<double>();
+    M.scale = nullptr;
+    const double *xo = R.x.as<double>() + (int64_t) xin * R.n;
+    double *xn = R.x.as<double>() + (int64_t) (1 - xin) * R.n;
+    k_refine_residual<BS, G, VEC, LAYOUT><<<R.np, 256, 0, s>>>(M, b, xo, R.d32.as<float>(), xn, R.r32.as<float>(),
+                                                                 R.part.as<double>());
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
+};
+
+template <int LAYOUT>
+bool pick(RefineState &R)
+{
+#define BMFP_RESID_BS(B)                         \
+  case B:                                        \
+    R.resid = Resid<B, LAYOUT>::run;             \
+    R.np = Resid<B, LAYOUT>::groups(*R.A);       \
+    return true;
+  switch (R.A->bs) {
+    BMFP_RESID_BS(1)
+    BMFP_RESID_BS(2)
+    BMFP_RESID_BS(3)
+    BMFP_RESID_BS(4)
+    BMFP_RESID_BS(6)
+    BMFP_RESID_BS(8)
+    BMFP_RESID_BS(12)
+    BMFP_RESID_BS(16)
+    BMFP_RESID_BS(24)
+    BMFP_RESID_BS(32)
+  default:
+    return false;
+  }
+#undef BMFP_RESID_BS
+}
+
+// Creates the inner matrix copy and solver for `dtype` on first use.
+int ensure_level(RefineState &R, int dtype)
+{
+  if (R.inner[dtype]) return BMFP_OK;
+  bmfp_matrix out = nullptr;
+  BMFP_TRY(bmfp_matrix_convert(R.A, dtype, R.A->layout, &out, R.stream));
+  R.low[dtype].reset(out);
+  BMFP_HIP(hipStreamSynchronize(R.stream));
+  std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
+  BMFP_TRY(solver_create(*S, R.low[dtype].get(), R.use_jacobi));
+  R.inner[dtype] = std::move(S);
+  return BMFP_OK;
+}
+
+}  // namespace
+
+int refine_create(bmfp_solver_t &H, bmfp_matrix_t *A, int inner_dtype, int use_jacobi)
+{
+  if (!A->is_bcsr || A->dtype != BMFP_F64) return fail(BMFP_ERR_INVALID, "refinement needs an F64 BCSR matrix");
+  if (A->n_rows != A->n_cols) return fail(BMFP_ERR_INVALID, "refinement needs a square matrix");
+  if (inner_dtype != BMFP_F32 && inner_dtype != BMFP_F16)
+    return fail(BMFP_ERR_INVALID, "inner_dtype must be BMFP_F32 or BMFP_F16");
+  std::unique_ptr<RefineState> R(new RefineState);
+  R->A = A;
+  R->use_jacobi = use_jacobi ? 1 : 0;
+  R->dtype = inner_dtype;
+  R->n = A->y_len();
+  const bool ok = A->layout == BMFP_ROW_MAJOR ? pick<BMFP_ROW_MAJOR>(*R) : pick<BMFP_COL_MAJOR>(*R);
+  if (!ok) return fail(BMFP_ERR_UNSUPPORTED, "no refinement kernels for bs %d", A->bs);
+  const size_t n = (size_t) std::max<int64_t>(R->n, 1);
+  BMFP_TRY(R->x.alloc(2 * n * sizeof(double)));
+  BMFP_TRY(R->r32.alloc(n * sizeof(float)));
+  BMFP_TRY(R->d32.alloc(n * sizeof(float)));
+  BMFP_TRY(R->part.alloc(2 * SOLVER_MAX_GROUPS * sizeof(double)));
+  BMFP_HIP(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+  BMFP_HIP(hipEventCreate(&R->ev0));
+  BMFP_HIP(hipEventCreate(&R->ev1));
+  BMFP_HIP(hipEventCreateWithFlags(&R->ev_sync, hipEventDisableTiming));
+  BMFP_TRY(ensure_level(*R, inner_dtype));
+  H.refine = R.release();
+  return BMFP_OK;
+}
+
+void refine_destroy(RefineState *R) { delete R; }
+
+int refine_run(RefineState &R, const double *b, double *x, double eps, double inner_tol, int max_outer,
+               int max_inner, hipStream_t user, bmfp_solve_info *info)
+{
+  if (!(eps > 0.0) || !(inner_tol > 0.0) || max_outer < 1 || max_inner < 1)
+    return fail(BMFP_ERR_INVALID, "refinement: eps, inner_tol > 0 and max_outer, max_inner >= 1 required");
+  bmfp_solve_info out{};
+  out.inner_dtype = R.dtype;
+  if (R.n == 0) {
+    if (info) *info = out;
+    return BMFP_OK;
+  }
+  if (!b || !x) return fail(BMFP_ERR_INVALID, "NULL vector");
+  hipStream_t s = R.stream;
+  BMFP_HIP(hipEventRecord(R.ev_sync, user));
+  BMFP_HIP(hipStreamWaitEvent(s, R.ev_sync, 0));
+  BMFP_HIP(hipEventRecord(R.ev0, s));
+  BMFP_HIP(hipMemsetAsync(R.x.ptr, 0, (size_t) R.n * sizeof(double), s));
+  BMFP_HIP(hipMemsetAsync(R.d32.ptr, 0, (size_t) R.n * sizeof(float), s));
+  std::vector<double> h(2 * (size_t) R.np);
+  int xin = 0;
+  double norm_b = 0.0, prev_rel = INFINITY;
+  int status = BMFP_ERR_NOT_CONVERGED;
+  for (int k = 0; k <= max_outer; k++) {
+    // x_{k} = x_{k-1} + d, r = b - A x_k (FP64); x_k lands in buffer 1 - xin
+    BMFP_TRY(R.resid(R, b, xin, s));
+    xin = 1 - xin;
+    BMFP_HIP(hipMemcpyAsync(h.data(), R.part.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    BMFP_HIP(hipStreamSynchronize(s));
+    double rr = 0, bb = 0;
+    for (int g = 0; g < R.np; g++) {
+      rr += h[2 * g];
+      bb += h[2 * g + 1];
+    }
+    if (k == 0) norm_b = std::sqrt(bb);
+    const double rel = norm_b > 0 ? std::sqrt(rr) / norm_b : 0.0;
+    out.rel_residual = out.true_rel_residual = rel;
+    out.outer_iterations = k;
+    if (!std::isfinite(rel)) {
+      status = BMFP_ERR_NONFINITE;
+      break;
+    }
+    if (rel < eps) {
+      status = BMFP_OK;
+      break;
+    }
+    if (k == max_outer) break;
+    // FP16 inner solves that stopped paying off escalate to FP32
+    if (R.dtype == BMFP_F16 && k >= 2 && rel > 0.5 * prev_rel) {
+      BMFP_TRY(ensure_level(R, BMFP_F32));
+      R.dtype = BMFP_F32;
+      out.escalations++;
+    }
+    prev_rel = rel;
+    bmfp_solve_info in{};
+    int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in);
+    out.inner_iterations += in.iterations;
+    if ((st == BMFP_ERR_BREAKDOWN || st == BMFP_ERR_NONFINITE) && R.dtype == BMFP_F16) {
+      BMFP_TRY(ensure_level(R, BMFP_F32));
+      R.dtype = BMFP_F32;
+      out.escalations++;
+      st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in);
+      out.inner_iterations += in.iterations;
+    }
+    if (st == BMFP_ERR_NONFINITE) {
+      status = st;
+      break;
+    }
+    if (st != BMFP_OK && st != BMFP_ERR_NOT_CONVERGED && st != BMFP_ERR_BREAKDOWN) return st;
+    // a breakdown still leaves the last valid inner iterate in d
+  }
+  BMFP_HIP(hipMemcpyAsync(x, R.x.as<double>() + (int64_t) xin * R.n, (size_t) R.n * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+  BMFP_HIP(hipEventRecord(R.ev1, s));
+  BMFP_HIP(hipEventSynchronize(R.ev1));
+  float ms = 0.f;
+  BMFP_HIP(hipEventElapsedTime(&ms, R.ev0, R.ev1));
+  out.seconds = ms * 1e-3;
+  out.iterations = out.inner_iterations;
+  out.status = status;
+  out.inner_dtype = R.dtype;
+  if (info) *info = out;
+  return status;
+}
+
+}  // namespace bmfp
+
+using namespace bmfp;
+
+extern "C" int bmfp_refine_create(bmfp_solver *out, bmfp_matrix A64, int inner_dtype, int use_jacobi)
+{
+  if (!out || !A64) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  std::unique_ptr<bmfp_solver_t> H(new bmfp_solver_t);
+  BMFP_TRY(refine_create(*H, A64, inner_dtype, use_jacobi));
+  *out = H.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps, double inner_tol,
+                                 int max_outer, int max_inner, bmfp_stream stream, bmfp_solve_info *info)
+{
+  if (!s || !s->refine) return fail(BMFP_ERR_INVALID, "not a refinement handle");
+  const int st = refine_run(*s->refine, b, x, eps, inner_tol, max_outer, max_inner,
+                            static_cast<hipStream_t>(stream), info);
+  if (st == BMFP_ERR_NOT_CONVERGED) set_error("refinement: max_outer (%d) reached", max_outer);
+  return st;
+}
+
+extern "C" int bmfp_refine_destroy(bmfp_solver s)
+{
+  delete s;
+  return BMFP_OK;
+}

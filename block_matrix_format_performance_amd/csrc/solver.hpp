@@ -6,6 +6,8 @@
 
 namespace bmfp {
 struct SolverState;
+struct RefineState;
+void refine_destroy(RefineState *R);
 }
 
 struct bmfp_solver_t {
@@ -23,6 +25,7 @@ struct bmfp_solver_t {
   int (*ops_init)(const bmfp_solver_t &, const void *, double, int, hipStream_t) = nullptr;
   int (*ops_iterate)(const bmfp_solver_t &, hipStream_t) = nullptr;
   int (*ops_jacobi)(const bmfp_solver_t &, hipStream_t) = nullptr;
+  bmfp::RefineState *refine = nullptr;  // set for bmfp_refine_* handles only
   ~bmfp_solver_t();
 };
 

@@ -445,3 +445,40 @@ __global__ __launch_bounds__(256) void k_jacobi(MatArgs<VT, AT> M, int bs, AT *_
 }
 
 }  // namespace bmfp
+
+namespace bmfp {
+
+// Refinement residual: x_new = x_old + d (owned rows), r = b - A x_new in
+// FP64 (A x_new formed with x_old + d gathered on the fly), r32 = (float) r;
+// partial sums of (r, r) and (b, b).
+template <int BS, int G, int VEC, int LAYOUT>
+__global__ __launch_bounds__(256) void k_refine_residual(MatArgs<double, double> M, const double *__restrict__ b,
+                                                         const double *__restrict__ x_old,
+                                                         const float *__restrict__ d, double *__restrict__ x_new,
+                                                         float *__restrict__ r32, double *__restrict__ part)
+{
+  __shared__ double lds[2][4];
+  const OperandXD op{x_old, d};
+  double acc[2] = {0.0, 0.0};
+  const int64_t threads = (int64_t) M.n_brows * BS * G;
+  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
+    const int64_t tt = base + threadIdx.x;
+    if (tt < threads) {
+      const int g = (int) (tt % G);
+      const int64_t row = tt / G;
+      const int br = (int) (row / BS), r = (int) (row % BS);
+      const double y = fused_row<double, double, BS, G, VEC, LAYOUT, false>(M.rp, M.cols, M.vals, nullptr, op, br, r, g);
+      if (g == 0) {
+        x_new[row] = op.raw(row);
+        const double bi = b[row];
+        const double ri = bi - y;
+        r32[row] = (float) ri;
+        acc[0] += ri * ri;
+        acc[1] += bi * bi;
+      }
+    }
+  }
+  write_partials<2>(part, acc, lds);
+}
+
+}  // namespace bmfp

@@ -192,6 +192,8 @@ typedef struct {
   double seconds;           /* device time of the solve (hipEvents on its stream)      */
   int outer_iterations;     /* refinement steps (0 for plain BiCGStab)                 */
   int inner_iterations;     /* inner BiCGStab iterations summed over refinement steps  */
+  int inner_dtype;          /* refinement: storage type of the last inner solve        */
+  int escalations;          /* refinement: precision escalations (FP16 -> FP32)        */
 } bmfp_solve_info;
 
 /* use_jacobi: point-Jacobi preconditioning (bicgstab.cu:201-238). */
@@ -203,6 +205,23 @@ int bmfp_bicgstab_solve(bmfp_solver s, const void *b, void *x, double eps,
                         int max_iterations, bmfp_stream stream,
                         bmfp_solve_info *info);
 int bmfp_bicgstab_destroy(bmfp_solver s);
+/* ---- mixed-precision iterative refinement (no reference counterpart) ----
+ * Solves A x = b for an F64 BCSR matrix A:
+ *   x_0 = 0;  r_k = b - A x_k   (FP64 matrix and vectors, fused SpMV kernel)
+ *   stop when ||r_k||/||b|| < eps  (true residual, FP64)
+ *   solve A_low d_k = r_k by BiCGStab to inner_tol (relative), FP32 vectors,
+ *   A_low an FP32 copy of A (inner_dtype BMFP_F32) or FP16 values with one
+ *   FP32 scale per block row (BMFP_F16);  x_{k+1} = x_k + d_k  (FP64).
+ * If an FP16 inner solve breaks down, or the outer residual shrinks by less
+ * than 2x in a step, the inner matrix escalates to FP32 (info->escalations).
+ * b, x are device FP64 vectors.  Synchronous; returns info->status. */
+int bmfp_refine_create(bmfp_solver *out, bmfp_matrix A64, int inner_dtype,
+                       int use_jacobi);
+int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps,
+                      double inner_tol, int max_outer, int max_inner,
+                      bmfp_stream stream, bmfp_solve_info *info);
+int bmfp_refine_destroy(bmfp_solver s);
+
 /* Device bytes the solver for A holds beyond A itself (the reference's
  * ctor memory estimate, bicgstab.cu:30-45). */
 int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes);
