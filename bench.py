@@ -31,6 +31,14 @@ sys.path.insert(0, REPO)
 import block_matrix_format_performance_amd as bm  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled 2.0 / 1.0 by a 1.024 GB copy-kernel calibration) on the HBM-resident
+# sweep matrices; profiles/spmv_hbm_traffic_pmc.md gives the runs and commands.
+PMC_TRAFFIC_BYTES = {
+    ("hbm", 1, "thread_per_row"): 7.263e9,
+    ("hbm", 4, "group_per_row"): 4.134e9,
+    ("hbm", 32, "group_per_row"): 3.639e9,
+}
 BS_SWEEP = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
 FAMILY_NAMES = {
     bm.BCSR_THREAD_PER_ROW: "thread_per_row",
@@ -126,6 +134,59 @@ def run_only(args, total_values, stream):
     print(json.dumps(res), flush=True)
 
 
+def solver_comparison(stream, max_iter):
+    """FP64 BiCGStab vs FP64-refinement around FP32 / FP16+scale inner BiCGStab
+    on the 2048x1024 bs=4 convective block system (off-diagonal blocks
+    -(K+0.3C), diagonal 1.02*rowsum+1), b = A x*, x* uniform[-1,1] (seed 1),
+    x0 = 0, to 1e-10 relative residual."""
+    nx, ny, bs = 2048, 1024, 4
+    t0 = time.perf_counter()
+    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, 1, seed=0)
+    n = nx * ny
+    A = bm.Matrix.bcsr(rp, cols, vals, n, n, bs)
+    del rp, cols, vals
+    xs = bm.DeviceArray(n * bs)
+    xs.fill(1, seed=1, stream_id=5)
+    b = bm.DeviceArray(n * bs)
+    A.spmv(xs, b)
+    bm.synchronize()
+    h_xs = xs.download()
+    log(f"[bench] solver system {nx}x{ny} bs={bs} n={n * bs} built in {time.perf_counter() - t0:.1f} s")
+    res = {"system": f"{nx}x{ny} nodes, bs={bs}, convective 9-point stencil, n={n * bs}, "
+                     f"nnzb={A.nnzb}, eps=1e-10, x0=0, b=A x*"}
+
+    def err(x, ref):
+        return float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+
+    x64 = bm.DeviceArray(n * bs)
+    S = bm.BiCGStab(A)
+    S.solve(b, x64, 1e-10, 2)                       # builds the iteration graph
+    info = S.solve(b, x64, 1e-10, max_iter)
+    h64 = x64.download()
+    res["fp64"] = {"seconds": round(info.seconds, 5), "iterations": info.iterations,
+                   "status": info.status, "rel_residual": info.true_rel_residual,
+                   "err_vs_xstar": err(h64, h_xs)}
+    log(f"[bench] FP64 BiCGStab: {info.iterations} it, {info.seconds*1e3:.1f} ms, "
+        f"res {info.true_rel_residual:.2e}")
+    S.destroy()
+    for name, inner, tol in (("mixed_fp32", bm.F32, 1e-5), ("mixed_fp16", bm.F16, 1e-3)):
+        R = bm.Refine(A, inner)
+        x = bm.DeviceArray(n * bs)
+        R.solve(b, x, 1e-10, tol, 1, 2)             # builds the inner graphs
+        ri = R.solve(b, x, 1e-10, tol, 50, max_iter)
+        hx = x.download()
+        res[name] = {"seconds": round(ri.seconds, 5), "outer_iterations": ri.outer_iterations,
+                     "inner_iterations": ri.inner_iterations, "status": ri.status,
+                     "escalations": ri.escalations, "rel_residual": ri.true_rel_residual,
+                     "err_vs_fp64_solution": err(hx, h64), "err_vs_xstar": err(hx, h_xs),
+                     "time_vs_fp64": round(ri.seconds / info.seconds, 4)}
+        log(f"[bench] {name}: {ri.outer_iterations} outer / {ri.inner_iterations} inner, "
+            f"{ri.seconds*1e3:.1f} ms ({ri.seconds / info.seconds:.3f}x FP64), res {ri.true_rel_residual:.2e}")
+        R.destroy()
+    A.destroy()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +203,9 @@ def main():
                          "--steps times (for counter collection) and print its timing")
     ap.add_argument("--copy-mb", type=float, default=2048.0,
                     help="size of the streaming-copy yardstick (0 = skip)")
+    ap.add_argument("--no-solver", action="store_true",
+                    help="skip the FP64 vs mixed-precision BiCGStab comparison")
+    ap.add_argument("--solver-max-iter", type=int, default=20000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -253,7 +317,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": PMC_TRAFFIC_BYTES.get((args.sizing, dom["bs"], dom["key"].split("/")[1])),
             "kernel": f"bs={dom['bs']} {dom['key']}",
             "launch_us": round(dom_t * 1e6, 2),
             "algorithmic_bytes": dom["bytes"],
@@ -266,6 +330,11 @@ def main():
     }
     if args.copy_mb > 0:
         out["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 50, stream)
+    for c in cases:
+        c["A"].destroy()
+    cases.clear()
+    if not args.no_solver:
+        out["solver"] = solver_comparison(stream, args.solver_max_iter)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         cb = cpu_baseline(args.cpu_seconds)

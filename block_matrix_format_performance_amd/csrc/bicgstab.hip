@@ -52,26 +52,13 @@ SolverVecs<AT> vec_args(const bmfp_solver_t &S, const void *b)
   return V;
 }
 
-// Lane-group shape for the fused SpMVs: thread per row for small blocks and
-// for column-major blocks; 16-byte column slices per lane for row-major
-// blocks from bs = 8 up.
-template <class VT> constexpr int solver_vec(int bs, int layout)
-{
-  if (layout != BMFP_ROW_MAJOR || bs < 8) return bs;
-  constexpr int W = 16 / (int) sizeof(VT);
-  int v = 1;
-  while (v < W && bs % (2 * v) == 0) v *= 2;
-  return v;
-}
-
 template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
 struct Ops {
-  static constexpr int VEC = solver_vec<VT>(BS, LAYOUT);
-  static constexpr int G = pow2ceil(BS / VEC);
+  using Engine = SolverEngine<VT, AT, BS, LAYOUT, SCALED>;
 
   static int groups(const bmfp_matrix_t &A)
   {
-    const int64_t t = (int64_t) A.n_rows * BS * G;
+    const int64_t t = Engine::threads(A.n_rows);
     return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
   }
 
@@ -86,8 +73,8 @@ struct Ops {
   {
     const MatArgs<VT, AT> M = mat_args<VT, AT, SCALED>(*S.A);
     const SolverVecs<AT> V = vec_args<AT>(S, nullptr);
-    k_bicg_a<VT, AT, BS, G, VEC, LAYOUT, SCALED><<<S.np, 256, 0, s>>>(M, V);
-    k_bicg_b<VT, AT, BS, G, VEC, LAYOUT, SCALED><<<S.np, 256, 0, s>>>(M, V);
+    k_bicg_a<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
+    k_bicg_b<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
     k_bicg_c<AT><<<S.np, 256, 0, s>>>(V);
     BMFP_LAUNCHED();
     return BMFP_OK;
@@ -214,7 +201,7 @@ static int build_graph(bmfp_solver_t &S)
 }
 
 int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
-               bmfp_solve_info *info)
+               bmfp_solve_info *info, bool true_residual)
 {
   if (!(eps > 0.0) || max_iterations < 0) return fail(BMFP_ERR_INVALID, "eps must be > 0, max_iterations >= 0");
   if (S.n == 0) {
@@ -253,7 +240,7 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   out.rel_residual = st.norm_b2 > 0 ? std::sqrt(st.rr) / std::sqrt(st.norm_b2) : 0.0;
   out.seconds = ms * 1e-3;
   // true residual ||b - A x|| / ||b|| (after the timed region)
-  {
+  if (true_residual) {
     void *y = S.vecs.as<char>() + (size_t) 8 * S.n * vb;  // t buffer
     BMFP_TRY(launch_bcsr(*S.A, auto_variant(*S.A), x, y, s));
     const int ng = std::min<int64_t>((S.n + 255) / 256, SOLVER_MAX_GROUPS);

@@ -36,21 +36,12 @@ struct RefineState {
 
 namespace {
 
-constexpr int resid_vec(int bs, int layout)
-{
-  if (layout != BMFP_ROW_MAJOR || bs < 8) return bs;
-  int v = 1;
-  while (v < 2 && bs % (2 * v) == 0) v *= 2;
-  return v;
-}
-
 template <int BS, int LAYOUT>
 struct Resid {
-  static constexpr int VEC = resid_vec(BS, LAYOUT);
-  static constexpr int G = pow2ceil(BS / VEC);
+  using Engine = SolverEngine<double, double, BS, LAYOUT, false>;
   static int groups(const bmfp_matrix_t &A)
   {
-    const int64_t t = (int64_t) A.n_rows * BS * G;
+    const int64_t t = Engine::threads(A.n_rows);
     return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
   }
   static int run(const RefineState &R, const double *b, int xin, hipStream_t s)
@@ -63,7 +54,7 @@ struct Resid {
     M.scale = nullptr;
     const double *xo = R.x.as<double>() + (int64_t) xin * R.n;
     double *xn = R.x.as<double>() + (int64_t) (1 - xin) * R.n;
-    k_refine_residual<BS, G, VEC, LAYOUT><<<R.np, 256, 0, s>>>(M, b, xo, R.d32.as<float>(), xn, R.r32.as<float>(),
+    k_refine_residual<Engine><<<R.np, 256, 0, s>>>(M, b, xo, R.d32.as<float>(), xn, R.r32.as<float>(),
                                                                  R.part.as<double>());
     BMFP_LAUNCHED();
     return BMFP_OK;
@@ -194,13 +185,13 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
     }
     prev_rel = rel;
     bmfp_solve_info in{};
-    int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in);
+    int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false);
     out.inner_iterations += in.iterations;
     if ((st == BMFP_ERR_BREAKDOWN || st == BMFP_ERR_NONFINITE) && R.dtype == BMFP_F16) {
       BMFP_TRY(ensure_level(R, BMFP_F32));
       R.dtype = BMFP_F32;
       out.escalations++;
-      st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in);
+      st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false);
       out.inner_iterations += in.iterations;
     }
     if (st == BMFP_ERR_NONFINITE) {

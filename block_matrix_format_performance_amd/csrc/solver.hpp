@@ -33,5 +33,5 @@ namespace bmfp {
 size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi);
 int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi);
 int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
-               bmfp_solve_info *info);
+               bmfp_solve_info *info, bool true_residual = true);
 }  // namespace bmfp

@@ -19,6 +19,8 @@
 // only earlier kernels write.
 #pragma once
 
+#include <type_traits>
+
 #include "spmv_kernels.hpp"
 
 namespace bmfp {
@@ -185,7 +187,7 @@ struct OperandS {
   }
 };
 
-// Residual refinement operand: xn_i = x_i + d_i (FP64 outer update).
+// Refinement residual operand: xn_i = x_i + d_i (FP64 outer update).
 struct OperandXD {
   const double *__restrict__ x;
   const float *__restrict__ d;
@@ -257,6 +259,109 @@ struct MatArgs {
   const float *__restrict__ scale;
 };
 
+// Row engines: walk every scalar row of the matrix in a persistent grid and
+// call epi(row, y) once per row with y = (A op)_row.
+//
+// GroupEngine: G lanes per row, lane g owning a column slice of every block
+// (row-major blocks with bs >= 12: whole 128-byte lines per block row).
+template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED>
+struct GroupEngine {
+  static int64_t threads(int64_t n_brows) { return n_brows * BS * G; }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    const int64_t threads = (int64_t) M.n_brows * BS * G;
+    for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
+      const int64_t tt = base + threadIdx.x;
+      if (tt < threads) {
+        const int g = (int) (tt % G);
+        const int64_t row = tt / G;
+        const int br = (int) (row / BS), r = (int) (row % BS);
+        const AT y = fused_row<VT, AT, BS, G, VEC, LAYOUT, SCALED>(M.rp, M.cols, M.vals, M.scale, op, br, r, g);
+        if (g == 0) epi(row, y);
+      }
+    }
+  }
+};
+
+// ShflEngine: one lane per scalar row, 64/BS block rows per wavefront.  For
+// each block, lane r forms only operand element r of the block's x segment
+// (the operands here cost 2-4 loads each) and reads the other BS-1 from its
+// block-row neighbours by shuffles; loads of U blocks are issued together.
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+struct ShflEngine {
+  static constexpr int U = BS <= 2 ? 9 : BS <= 8 ? 3 : 1;
+  static int64_t threads(int64_t n_brows) { return (n_brows + 64 / BS - 1) / (64 / BS) * 64; }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    constexpr int RPW = 64 / BS;
+    const int lane = threadIdx.x & 63;
+    const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
+    const int64_t n_tiles = ((int64_t) M.n_brows + RPW - 1) / RPW;
+    const int64_t waves = (int64_t) gridDim.x * 4;
+    for (int64_t wt = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6; wt < n_tiles; wt += waves) {
+      const int64_t br64 = wt * RPW + lbr;
+      const bool active = lane < RPW * BS && br64 < M.n_brows;
+      const int br = (int) br64;
+      int b = 0, b1 = 0;
+      if (active) {
+        b = M.rp[br];
+        b1 = M.rp[br + 1];
+      }
+      // all lanes of a block row share b..b1, so every shuffle source below
+      // is active in the same iteration
+      AT acc = 0;
+      for (; b + U <= b1; b += U) {
+        int col[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) col[k] = M.cols[b + k];
+        VT v[U][BS];
+#pragma unroll
+        for (int k = 0; k < U; ++k) load_block_row<VT, BS, LAYOUT>(M.vals + (int64_t) (b + k) * (BS * BS), r, v[k]);
+        AT xe[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) xe[k] = op((int64_t) col[k] * BS + r);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+#pragma unroll
+          for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[k][c]), __shfl(xe[k], base_lane + c), acc);
+      }
+      for (; b < b1; ++b) {
+        VT v[BS];
+        load_block_row<VT, BS, LAYOUT>(M.vals + (int64_t) b * (BS * BS), r, v);
+        const AT xe = op((int64_t) M.cols[b] * BS + r);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[c]), __shfl(xe, base_lane + c), acc);
+      }
+      if (active) {
+        if constexpr (SCALED) acc *= (AT) M.scale[br];
+        epi((int64_t) br * BS + r, acc);
+      }
+    }
+  }
+};
+
+// Columns per lane for GroupEngine: one 16-byte load, two for FP64 bs >= 12.
+template <class VT> constexpr int engine_vec(int bs)
+{
+  constexpr int W = 16 / (int) sizeof(VT);
+  int v = 1;
+  while (v < W && bs % (2 * v) == 0) v *= 2;
+  if (sizeof(VT) == 8 && bs >= 12 && bs % 4 == 0) v = 4;
+  return v;
+}
+
+// Engine of the fused SpMVs: shuffle-shared operands (one operand element per
+// lane per block) except for row-major blocks of 12 and more columns, where a
+// lane row of bs values is too long and column slices stream better.
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+using SolverEngine =
+    typename std::conditional<(LAYOUT == BMFP_ROW_MAJOR && BS >= 12),
+                              GroupEngine<VT, AT, BS, pow2ceil(BS / engine_vec<VT>(BS)), engine_vec<VT>(BS), LAYOUT,
+                                          SCALED>,
+                              ShflEngine<VT, AT, BS, LAYOUT, SCALED>>::type;
+
 // Init: x = 0, r = r^ = b, p = v = 0 (both parities); partC = ((b,b), (b,b)).
 template <class AT>
 __global__ __launch_bounds__(256) void k_bicg_init(SolverVecs<AT> V, double eps, int max_iter)
@@ -289,7 +394,7 @@ __global__ __launch_bounds__(256) void k_bicg_init(SolverVecs<AT> V, double eps,
   }
 }
 
-template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED>
+template <class VT, class AT, class ENGINE>
 __global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT> V)
 {
   __shared__ double lds[2][4];
@@ -326,25 +431,17 @@ __global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT
   const double beta = rho_c / rho_p * alpha_p / omega_p;
   OperandP<AT> op{V.r, V.p[pv], V.v[pv], V.pinv, (AT) beta, (AT) omega_p};
   double part[1] = {0.0};
-  const int64_t threads = (int64_t) M.n_brows * BS * G;
-  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
-    const int64_t tt = base + threadIdx.x;
-    if (tt < threads) {
-      const int g = (int) (tt % G);
-      const int64_t row = tt / G;
-      const int br = (int) (row / BS), r = (int) (row % BS);
-      const AT y = fused_row<VT, AT, BS, G, VEC, LAYOUT, SCALED>(M.rp, M.cols, M.vals, M.scale, op, br, r, g);
-      if (g == 0) {
-        V.p[cu][row] = op.raw(row);
-        V.v[cu][row] = y;
-        part[0] += (double) V.rh[row] * (double) y;
-      }
-    }
-  }
+  AT *__restrict__ pc = V.p[cu];
+  AT *__restrict__ vc = V.v[cu];
+  ENGINE::run(M, op, [&](int64_t row, AT y) {
+    pc[row] = op.raw(row);
+    vc[row] = y;
+    part[0] += (double) V.rh[row] * (double) y;
+  });
   write_partials<1>(V.partA, part, lds);
 }
 
-template <class VT, class AT, int BS, int G, int VEC, int LAYOUT, bool SCALED>
+template <class VT, class AT, class ENGINE>
 __global__ __launch_bounds__(256) void k_bicg_b(MatArgs<VT, AT> M, SolverVecs<AT> V)
 {
   __shared__ double lds[2][4];
@@ -369,23 +466,13 @@ __global__ __launch_bounds__(256) void k_bicg_b(MatArgs<VT, AT> M, SolverVecs<AT
   if (stop != 1) return;
   OperandS<AT> op{V.r, V.v[it & 1], V.pinv, (AT) alpha};
   double part[2] = {0.0, 0.0};
-  const int64_t threads = (int64_t) M.n_brows * BS * G;
-  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
-    const int64_t tt = base + threadIdx.x;
-    if (tt < threads) {
-      const int g = (int) (tt % G);
-      const int64_t row = tt / G;
-      const int br = (int) (row / BS), r = (int) (row % BS);
-      const AT y = fused_row<VT, AT, BS, G, VEC, LAYOUT, SCALED>(M.rp, M.cols, M.vals, M.scale, op, br, r, g);
-      if (g == 0) {
-        const AT s = op.raw(row);
-        V.s[row] = s;
-        V.t[row] = y;
-        part[0] += (double) y * (double) s;
-        part[1] += (double) y * (double) y;
-      }
-    }
-  }
+  ENGINE::run(M, op, [&](int64_t row, AT y) {
+    const AT s = op.raw(row);
+    V.s[row] = s;
+    V.t[row] = y;
+    part[0] += (double) y * (double) s;
+    part[1] += (double) y * (double) y;
+  });
   write_partials<2>(V.partB, part, lds);
 }
 
@@ -451,7 +538,7 @@ namespace bmfp {
 // Refinement residual: x_new = x_old + d (owned rows), r = b - A x_new in
 // FP64 (A x_new formed with x_old + d gathered on the fly), r32 = (float) r;
 // partial sums of (r, r) and (b, b).
-template <int BS, int G, int VEC, int LAYOUT>
+template <class ENGINE>
 __global__ __launch_bounds__(256) void k_refine_residual(MatArgs<double, double> M, const double *__restrict__ b,
                                                          const double *__restrict__ x_old,
                                                          const float *__restrict__ d, double *__restrict__ x_new,
@@ -460,24 +547,14 @@ __global__ __launch_bounds__(256) void k_refine_residual(MatArgs<double, double>
   __shared__ double lds[2][4];
   const OperandXD op{x_old, d};
   double acc[2] = {0.0, 0.0};
-  const int64_t threads = (int64_t) M.n_brows * BS * G;
-  for (int64_t base = (int64_t) blockIdx.x * 256; base < threads; base += (int64_t) gridDim.x * 256) {
-    const int64_t tt = base + threadIdx.x;
-    if (tt < threads) {
-      const int g = (int) (tt % G);
-      const int64_t row = tt / G;
-      const int br = (int) (row / BS), r = (int) (row % BS);
-      const double y = fused_row<double, double, BS, G, VEC, LAYOUT, false>(M.rp, M.cols, M.vals, nullptr, op, br, r, g);
-      if (g == 0) {
-        x_new[row] = op.raw(row);
-        const double bi = b[row];
-        const double ri = bi - y;
-        r32[row] = (float) ri;
-        acc[0] += ri * ri;
-        acc[1] += bi * bi;
-      }
-    }
-  }
+  ENGINE::run(M, op, [&](int64_t row, double y) {
+    x_new[row] = op.raw(row);
+    const double bi = b[row];
+    const double ri = bi - y;
+    r32[row] = (float) ri;
+    acc[0] += ri * ri;
+    acc[1] += bi * bi;
+  });
   write_partials<2>(part, acc, lds);
 }
 
