@@ -450,3 +450,149 @@ class Refine:
             self.destroy()
         except Exception:
             pass
+
+
+# ---- serving ------------------------------------------------------------------------
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget_bytes", ctypes.c_size_t), ("max_systems", ctypes.c_int),
+                ("iterations_per_round", ctypes.c_int), ("slot_groups", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int), ("bs", ctypes.c_int), ("nnzb", ctypes.c_int),
+                ("row_ptr", ctypes.c_void_p), ("columns", ctypes.c_void_p), ("values", ctypes.c_void_p),
+                ("stencil_nx", ctypes.c_int), ("stencil_ny", ctypes.c_int), ("stencil_kind", ctypes.c_int),
+                ("stencil_seed", ctypes.c_uint64), ("nrhs", ctypes.c_int), ("b", ctypes.c_void_p),
+                ("rhs_seed", ctypes.c_uint64), ("tol", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("nrhs", ctypes.c_int), ("n", ctypes.c_int64),
+                ("outer_iterations", ctypes.c_int), ("inner_iterations", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("submit_time", ctypes.c_double),
+                ("admit_time", ctypes.c_double), ("done_time", ctypes.c_double)]
+
+
+def _declare_server(L):
+    vp = ctypes.c_void_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.bmfp_server_default_config.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.bmfp_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)]
+    L.bmfp_server_destroy.argtypes = [vp]
+    L.bmfp_server_request_bytes.argtypes = [ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_size_t)]
+    L.bmfp_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.bmfp_server_step.argtypes = [vp, ip, ip]
+    L.bmfp_server_poll.argtypes = [vp, ctypes.POINTER(RequestResult), vp, ctypes.c_size_t, ip]
+    L.bmfp_server_peek.argtypes = [vp, ctypes.POINTER(RequestResult), ip]
+    L.bmfp_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), ip, ip,
+                                    ctypes.POINTER(ctypes.c_double)]
+
+
+class Server:
+    """bmfp_server_*: continuous-batching BiCGStab request loop."""
+
+    def __init__(self, **overrides):
+        L = lib()
+        if not getattr(L, "_server_declared", False):
+            _declare_server(L)
+            L._server_declared = True
+        cfg = ServerConfig()
+        check(L.bmfp_server_default_config(ctypes.byref(cfg)))
+        for k, v in overrides.items():
+            setattr(cfg, k, v)
+        h = ctypes.c_void_p()
+        check(L.bmfp_server_create(ctypes.byref(h), ctypes.byref(cfg)))
+        self.handle = h
+        self._keep = {}
+
+    @staticmethod
+    def stencil_request(nx, ny, bs, kind=0, seed=0, nrhs=1, rhs_seed=0, tol=1e-10, b=None):
+        q = Request()
+        q.bs, q.stencil_nx, q.stencil_ny, q.stencil_kind, q.stencil_seed = bs, nx, ny, kind, seed
+        q.nrhs, q.rhs_seed, q.tol = nrhs, rhs_seed, tol
+        keep = []
+        if b is not None:
+            bb = np.ascontiguousarray(b, np.float64)
+            q.b = bb.ctypes.data
+            keep.append(bb)
+        return q, keep
+
+    @staticmethod
+    def bcsr_request(row_ptr, columns, values, n_rows, bs, nrhs=1, b=None, rhs_seed=0, tol=1e-10):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        cols = np.ascontiguousarray(columns, np.int32)
+        vals = np.ascontiguousarray(values, np.float64)
+        q = Request()
+        q.n_rows, q.bs, q.nnzb = n_rows, bs, int(rp[-1])
+        q.row_ptr, q.columns, q.values = rp.ctypes.data, cols.ctypes.data, vals.ctypes.data
+        q.nrhs, q.rhs_seed, q.tol = nrhs, rhs_seed, tol
+        keep = [rp, cols, vals]
+        if b is not None:
+            bb = np.ascontiguousarray(b, np.float64)
+            q.b = bb.ctypes.data
+            keep.append(bb)
+        return q, keep
+
+    def submit(self, req_keep):
+        q, keep = req_keep
+        rid = ctypes.c_uint64(0)
+        check(lib().bmfp_server_submit(self.handle, ctypes.byref(q), ctypes.byref(rid)))
+        return rid.value
+
+    @staticmethod
+    def request_bytes(req_keep):
+        nb = ctypes.c_size_t(0)
+        check(lib().bmfp_server_request_bytes(ctypes.byref(req_keep[0]), ctypes.byref(nb)))
+        return nb.value
+
+    def step(self):
+        a, q = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().bmfp_server_step(self.handle, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def poll(self, want_x=True):
+        """Returns (RequestResult, x[nrhs, n] or None) for one finished request, or None."""
+        res = RequestResult()
+        got = ctypes.c_int(0)
+        check(lib().bmfp_server_peek(self.handle, ctypes.byref(res), ctypes.byref(got)))
+        if not got.value:
+            return None
+        x = np.empty(res.nrhs * res.n if want_x else 1, np.float64)
+        check(lib().bmfp_server_poll(self.handle, ctypes.byref(res), x.ctypes.data if want_x else None,
+                                     x.size, ctypes.byref(got)))
+        return res, (x.reshape(res.nrhs, res.n) if want_x else None)
+
+    def stats(self):
+        used, cap = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        act, q = ctypes.c_int(0), ctypes.c_int(0)
+        busy = ctypes.c_double(0)
+        check(lib().bmfp_server_stats(self.handle, ctypes.byref(used), ctypes.byref(cap), ctypes.byref(act),
+                                      ctypes.byref(q), ctypes.byref(busy)))
+        return {"used": used.value, "capacity": cap.value, "active": act.value, "queued": q.value,
+                "busy_seconds": busy.value}
+
+    def drain(self, max_steps=1000000):
+        """Steps until nothing is active or queued; returns {id: (result, x)}."""
+        out = {}
+        for _ in range(max_steps):
+            a, q = self.step()
+            while True:
+                r = self.poll()
+                if r is None:
+                    break
+                out[r[0].id] = r
+            if a == 0 and q == 0:
+                break
+        return out
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().bmfp_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

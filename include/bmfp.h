@@ -222,6 +222,76 @@ int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps,
                       bmfp_stream stream, bmfp_solve_info *info);
 int bmfp_refine_destroy(bmfp_solver s);
 
+/* ---- serving: continuous-batching request loop (no reference counterpart) --
+ * Requests -- an FP64 BCSR system with 1..nrhs right-hand sides -- queue on
+ * the host and are admitted first-come first-served while their device
+ * footprint (FP64 matrix + FP32 copy + 13 vectors per right-hand side) fits
+ * the memory budget.  Admitted requests are set up on their own stream while
+ * earlier ones compute, then join the batch: every (request, right-hand side)
+ * is solved by FP64 iterative refinement around FP32 BiCGStab, all systems
+ * advancing together through batched persistent kernels replayed from a HIP
+ * graph (iterations_per_round batched iterations per bmfp_server_step).
+ * Each system's arithmetic is independent of the batch it shares: results
+ * are bitwise identical alone and in any mix.  Block sizes 1,2,3,4,6,8. */
+typedef struct bmfp_server_t *bmfp_server;
+
+typedef struct {
+  size_t memory_budget_bytes; /* device bytes for admitted requests (capped at 90% free) */
+  int max_systems;            /* concurrent (request, right-hand side) systems         */
+  int iterations_per_round;   /* batched iterations per graph replay                   */
+  int slot_groups;            /* persistent workgroups of the batched SpMV kernels     */
+  double inner_tol;           /* FP32 BiCGStab relative tolerance per refinement step  */
+  int max_outer;              /* refinement steps per system                           */
+  int max_inner;              /* inner iterations per refinement step                  */
+} bmfp_server_config;
+
+typedef struct {
+  /* explicit system (used when stencil_nx == 0): row-major blocks, FP64 */
+  int n_rows, bs, nnzb;
+  const int *row_ptr;
+  const int *columns;
+  const double *values;
+  /* synthetic 9-point block stencil generated on the device, bit-identical
+   * to bmfp_gen_stencil9(stencil_nx, stencil_ny, bs, kind, seed) */
+  int stencil_nx, stencil_ny, stencil_kind;
+  uint64_t stencil_seed;
+  int nrhs;
+  const double *b;   /* host, nrhs * n_rows*bs doubles; NULL: uniform[-1,1]   */
+  uint64_t rhs_seed; /* (key rhs_seed, element k*n + i) for right-hand side k  */
+  double tol;        /* relative residual ||b - A x|| / ||b|| (FP64)           */
+} bmfp_request;
+
+typedef struct {
+  uint64_t id;
+  int status;            /* worst status over the right-hand sides               */
+  int nrhs;
+  int64_t n;             /* rows per right-hand side                             */
+  int outer_iterations;  /* max over right-hand sides                            */
+  int inner_iterations;  /* max over right-hand sides (summed over outer steps)  */
+  double rel_residual;   /* max FP64 true relative residual                      */
+  double submit_time, admit_time, done_time;  /* host steady-clock seconds       */
+} bmfp_request_result;
+
+int bmfp_server_default_config(bmfp_server_config *cfg);
+int bmfp_server_create(bmfp_server *out, const bmfp_server_config *cfg);
+int bmfp_server_destroy(bmfp_server s);
+/* Device bytes a request needs (the admission footprint). */
+int bmfp_server_request_bytes(const bmfp_request *req, size_t *bytes);
+/* Queues a request (copies explicit host arrays).  BMFP_ERR_BUDGET when it
+ * could never fit the budget. */
+int bmfp_server_submit(bmfp_server s, const bmfp_request *req, uint64_t *id);
+/* Admits what fits, runs one round of batched iterations, retires finished
+ * requests.  Synchronous; reports active systems and waiting requests. */
+int bmfp_server_step(bmfp_server s, int *n_active, int *n_queued);
+/* Pops one finished request (FIFO by completion): result and, if x is not
+ * NULL, its solutions (nrhs * n doubles, right-hand side major). */
+int bmfp_server_poll(bmfp_server s, bmfp_request_result *res, double *x,
+                     size_t x_capacity, int *got);
+/* The result bmfp_server_poll would return next, without removing it. */
+int bmfp_server_peek(bmfp_server s, bmfp_request_result *res, int *got);
+int bmfp_server_stats(bmfp_server s, size_t *used_bytes, size_t *budget_bytes,
+                      int *active_systems, int *queued, double *busy_seconds);
+
 /* Device bytes the solver for A holds beyond A itself (the reference's
  * ctor memory estimate, bicgstab.cu:30-45). */
 int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes);
