@@ -58,16 +58,9 @@ def grid_side(bs, total_values):
 
 
 def build_case(bs, side, layouts):
-    rp, cols, vals = bm.gen_stencil9(side, side, bs, 0, seed=0)
-    n = side * side
-    mats = {}
-    for layout in layouts:
-        if layout == bm.ROW_MAJOR:
-            v = vals
-        else:
-            v = np.ascontiguousarray(vals.reshape(-1, bs, bs).transpose(0, 2, 1)).ravel()
-        mats[layout] = bm.Matrix.bcsr(rp, cols, v, n, n, bs, dtype=bm.F64, layout=layout)
-    return mats, rp, cols, vals
+    """The sweep matrix for one bs, generated in device memory in each layout."""
+    return {layout: bm.Matrix.stencil(side, side, bs, 0, seed=0, dtype=bm.F64, layout=layout)
+            for layout in layouts}
 
 
 def measure_copy(nbytes, reps, stream):
@@ -120,7 +113,7 @@ def run_only(args, total_values, stream):
     layout = bm.ROW_MAJOR if lay_s == "row" else bm.COL_MAJOR
     fam = {v: k for k, v in FAMILY_NAMES.items()}[fam_s]
     side = grid_side(bs, total_values)
-    mats, _, _, _ = build_case(bs, side, (layout,))
+    mats = build_case(bs, side, (layout,))
     A = mats[layout]
     x = bm.DeviceArray(A.x_len)
     x.fill(0, seed=0, stream_id=4)
@@ -141,10 +134,8 @@ def solver_comparison(stream, max_iter):
     x0 = 0, to 1e-10 relative residual."""
     nx, ny, bs = 2048, 1024, 4
     t0 = time.perf_counter()
-    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, 1, seed=0)
     n = nx * ny
-    A = bm.Matrix.bcsr(rp, cols, vals, n, n, bs)
-    del rp, cols, vals
+    A = bm.Matrix.stencil(nx, ny, bs, 1, seed=0)
     xs = bm.DeviceArray(n * bs)
     xs.fill(1, seed=1, stream_id=5)
     b = bm.DeviceArray(n * bs)
@@ -187,6 +178,187 @@ def solver_comparison(stream, max_iter):
     return res
 
 
+SERVE_STREAM = 20000
+SERVE_BS = (2, 3, 4, 6, 8)
+
+
+def serving_stream(n_sample, seed=2):
+    """The request stream: 20,000 requests from `seed`; the first n_sample are used.
+    Per request: square grid side U{16..512}, bs from {2,3,4,6,8}, 1-4
+    right-hand sides uniform[-1,1], tolerance 1e-10; unit-rate exponential
+    inter-arrival gaps (divided by the arrival rate)."""
+    rng = np.random.default_rng(seed)
+    side = rng.integers(16, 513, size=SERVE_STREAM)
+    bs = rng.choice(SERVE_BS, size=SERVE_STREAM)
+    nrhs = rng.integers(1, 5, size=SERVE_STREAM)
+    arrivals = np.cumsum(rng.exponential(1.0, size=SERVE_STREAM))
+    reqs = [{"side": int(side[i]), "bs": int(bs[i]), "nrhs": int(nrhs[i]),
+             "seed": (seed << 32) + i, "rhs_seed": (seed << 32) + i} for i in range(n_sample)]
+    return reqs, arrivals[:n_sample] - arrivals[0]
+
+
+def plain_one_at_a_time(reqs, max_iter):
+    """Each request's right-hand sides solved one after another by the plain
+    FP64 BiCGStab (x0 = 0, 1e-10).  Timed: the solve calls only (matrix
+    generation, solver set-up and its first graph build are excluded)."""
+    total, per_req, xs, iters = 0.0, [], [], []
+    for r in reqs:
+        A = bm.Matrix.stencil(r["side"], r["side"], r["bs"], 0, seed=r["seed"])
+        n = A.y_len
+        B = bm.DeviceArray(r["nrhs"] * n)
+        B.fill(1, seed=r["rhs_seed"], stream_id=6)
+        x = bm.DeviceArray(n)
+        S = bm.BiCGStab(A)
+        S.solve(B.view(0, n), x, 1e-10, 2)            # graph build, untimed
+        t_req, xr = 0.0, []
+        for k in range(r["nrhs"]):
+            bm.synchronize()
+            t0 = time.perf_counter()
+            info = S.solve(B.view(k * n, n), x, 1e-10, max_iter)
+            t_req += time.perf_counter() - t0
+            if info.status != bm.OK:
+                raise RuntimeError(f"plain FP64 BiCGStab failed on {r}: status {info.status}")
+            iters.append(info.iterations)
+            xr.append(x.download())
+        total += t_req
+        per_req.append(t_req)
+        xs.append(np.stack(xr))
+        S.destroy()
+        A.destroy()
+    return total, np.array(per_req), xs, iters
+
+
+def request_bytes_moved(r, outer, inner):
+    """Compulsory HBM bytes of one served request: every refinement step reads
+    the FP64 matrix once (x+d, b read, x', r32 written per right-hand side);
+    every inner iteration reads the FP32 matrix twice plus ~16 FP32 vector
+    accesses per right-hand side and row (the right-hand sides share the
+    matrix reads)."""
+    side, bs, nr = r["side"], r["bs"], r["nrhs"]
+    nodes = side * side
+    nnzb = (3 * side - 2) ** 2
+    n = nodes * bs
+    idx = 4.0 * nnzb + 4.0 * (nodes + 1)
+    resid = 8.0 * nnzb * bs * bs + idx + nr * n * (8 + 4 + 8 + 8 + 4)
+    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 4.0 * 16
+    return outer * resid + inner * inner_it
+
+
+def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
+    """Replays the stream at `lam` requests/s in real time through bmfp_server.
+    Latency = completion seen by the host - scheduled arrival.  Like the plain
+    comparator's timing, solutions stay in device memory; with ref_x (an
+    untimed accuracy pass) they are downloaded and compared."""
+    S = bm.Server(**cfg)
+    schedule = S.schedule()
+    n = len(reqs)
+    arr = arrivals / lam
+    lat = np.zeros(n)
+    ids = {}
+    xs = {}
+    worst_res, outer, inner, moved = 0.0, [], [], 0.0
+    i = done = 0
+    t0 = time.perf_counter()
+    t_last = 0.0
+    while done < n:
+        now = time.perf_counter() - t0
+        while i < n and arr[i] <= now:
+            r = reqs[i]
+            ids[S.submit(S.stencil_request(r["side"], r["side"], r["bs"], 0, r["seed"], r["nrhs"],
+                                           r["rhs_seed"], 1e-10))] = i
+            i += 1
+        a, q = S.step()
+        while True:
+            p = S.poll(want_x=ref_x is not None)
+            if p is None:
+                break
+            res, x = p
+            j = ids[res.id]
+            t_last = time.perf_counter() - t0
+            lat[j] = t_last - arr[j]
+            if res.status != bm.OK:
+                raise RuntimeError(f"served request {reqs[j]} failed: status {res.status}")
+            worst_res = max(worst_res, res.rel_residual)
+            outer.append(res.outer_iterations)
+            inner.append(res.inner_iterations)
+            moved += request_bytes_moved(reqs[j], res.outer_iterations, res.inner_iterations)
+            if ref_x is not None:
+                xs[j] = x
+            done += 1
+        if a == 0 and q == 0 and i < n:
+            time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+    busy = S.stats()["busy_seconds"]
+    S.destroy()
+    span = t_last - arr[0]
+    out = {"lambda_per_s": lam, "requests": n, "span_s": round(span, 4),
+           "solves_per_s": round(n / span, 2),
+           "rhs_solves_per_s": round(sum(r["nrhs"] for r in reqs) / span, 2),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "max_ms": round(float(lat.max()) * 1e3, 3),
+           "gpu_busy_frac": round(busy / span, 3),
+           "busy_GBs": round(moved / busy / 1e9, 1),
+           "max_rel_residual": worst_res,
+           "mean_outer": round(float(np.mean(outer)), 2), "mean_inner": round(float(np.mean(inner)), 1),
+           "schedule": {"slot_groups": schedule[0], "chunks_per_tile": schedule[1]}}
+    if ref_x is not None:
+        out["max_err_vs_fp64"] = max(float((np.linalg.norm(xs[j] - ref_x[j], axis=1) /
+                                            np.linalg.norm(ref_x[j], axis=1)).max()) for j in xs)
+    return out
+
+
+def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=True):
+    reqs, arrivals = serving_stream(n_sample)
+    log(f"[bench] serving sample: {n_sample} requests, "
+        f"{sum(r['nrhs'] for r in reqs)} right-hand sides")
+    t0 = time.perf_counter()
+    if plain:
+        plain_s, plain_per, ref_x, plain_it = plain_one_at_a_time(reqs, max_iter)
+        log(f"[bench] plain FP64 one at a time: {plain_s:.3f} s of solves "
+            f"({n_sample / plain_s:.1f} solves/s), wall {time.perf_counter() - t0:.1f} s")
+    else:
+        plain_s, plain_per, ref_x, plain_it = float("nan"), np.array([np.nan]), None, [np.nan]
+    cfg = {"memory_budget_bytes": int(200e9)}
+    cfg.update(cfg_overrides or {})
+    runs = []
+    accuracy = None
+    if ref_x is not None:
+        # every request once more, all at t=0, solutions downloaded: results are
+        # bitwise those of the timed runs (independent of mix and schedule)
+        acc = serve_replay(reqs, arrivals, float("inf"), cfg, ref_x)
+        accuracy = {"max_err_vs_fp64": acc["max_err_vs_fp64"], "max_rel_residual": acc["max_rel_residual"],
+                    "reference": "plain FP64 GPU BiCGStab solutions (1e-10)"}
+        log(f"[bench] served solutions vs plain FP64: max rel err {acc['max_err_vs_fp64']:.2e}, "
+            f"max rel residual {acc['max_rel_residual']:.2e}")
+    for k, lam in enumerate(lambdas):
+        r = serve_replay(reqs, arrivals, lam, cfg)
+        r["speedup_vs_plain"] = round(r["solves_per_s"] / (n_sample / plain_s), 3) if plain else None
+        log(f"[bench] serve lambda={lam}: {r['solves_per_s']} solves/s ({r['speedup_vs_plain']}x plain), "
+            f"p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, busy {r['gpu_busy_frac']}")
+        runs.append(r)
+    return {
+        "workload": f"first {n_sample} of a 20,000-request stream (seed 2): side U{{16..512}}, "
+                    f"bs in {{2,3,4,6,8}}, 1-4 RHS uniform[-1,1], tol 1e-10, diagonally dominant "
+                    f"9-point stencils; served by FP64 refinement around FP32 BiCGStab",
+        "config": {k: v for k, v in cfg.items()},
+        "plain_fp64": {"solves_per_s": round(n_sample / plain_s, 2), "solve_seconds": round(plain_s, 4),
+                       "p50_ms": round(float(np.percentile(plain_per, 50)) * 1e3, 3),
+                       "p99_ms": round(float(np.percentile(plain_per, 99)) * 1e3, 3),
+                       "mean_iterations": round(float(np.mean(plain_it)), 1),
+                       "timing": "solve calls only, one request after another"} if plain else None,
+        "served": runs,
+        "accuracy": accuracy,
+    }
+
+
+def parse_overrides(text):
+    out = {}
+    for kv in filter(None, text.split(",")):
+        k, v = kv.split("=")
+        out[k] = float(v) if "." in v or "e" in v else int(v)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +378,14 @@ def main():
     ap.add_argument("--no-solver", action="store_true",
                     help="skip the FP64 vs mixed-precision BiCGStab comparison")
     ap.add_argument("--solver-max-iter", type=int, default=20000)
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the serving stream replayed (0 = skip the serving leg)")
+    ap.add_argument("--serve-lambdas", default="200,1000,5000")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--serve-no-plain", action="store_true",
+                    help="skip the one-at-a-time FP64 comparator of the serving leg")
+    ap.add_argument("--serve-config", default="",
+                    help="server overrides, e.g. iterations_per_round=8,slot_groups=2048")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -223,13 +403,19 @@ def main():
     if args.only:
         run_only(args, total_values, stream)
         return
+    if args.serve_only:
+        print(json.dumps(serving_comparison(args.serve_requests,
+                                            [float(v) for v in args.serve_lambdas.split(",")],
+                                            args.solver_max_iter, parse_overrides(args.serve_config),
+                                            plain=not args.serve_no_plain)), flush=True)
+        return
 
     cases = []
     sweep = {}
     for bs in bs_list:
         side = grid_side(bs, total_values)
         t0 = time.perf_counter()
-        mats, _, _, _ = build_case(bs, side, (bm.ROW_MAJOR, bm.COL_MAJOR))
+        mats = build_case(bs, side, (bm.ROW_MAJOR, bm.COL_MAJOR))
         A0 = mats[bm.ROW_MAJOR]
         x = bm.DeviceArray(A0.x_len)
         x.fill(0, seed=0, stream_id=4)
@@ -335,6 +521,10 @@ def main():
     cases.clear()
     if not args.no_solver:
         out["solver"] = solver_comparison(stream, args.solver_max_iter)
+    if args.serve_requests > 0:
+        out["serving"] = serving_comparison(args.serve_requests,
+                                            [float(v) for v in args.serve_lambdas.split(",")],
+                                            args.solver_max_iter)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         cb = cpu_baseline(args.cpu_seconds)

@@ -101,6 +101,7 @@ def _declare(L):
     L.bmfp_gen_stencil9.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_uint64, vp, vp, vp]
     L.bmfp_bcsr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, c_int, c_int, vp, vp, vp]
     L.bmfp_csr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, vp, vp, vp]
+    L.bmfp_stencil_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, ctypes.c_uint64, c_int, c_int]
     L.bmfp_matrix_convert.argtypes = [vp, c_int, c_int, ctypes.POINTER(vp), vp]
     L.bmfp_matrix_destroy.argtypes = [vp]
     L.bmfp_matrix_shape.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
@@ -203,6 +204,12 @@ class DeviceArray:
     def zero(self):
         check(lib().bmfp_memset(self.ptr, 0, self.nbytes))
 
+    def view(self, start, n):
+        """Non-owning view of elements [start, start+n) (keeps this buffer alive)."""
+        if start < 0 or n < 0 or start + n > self.n:
+            raise IndexError(f"view [{start}, {start + n}) outside [0, {self.n})")
+        return DeviceView(self, start, n)
+
     def free(self):
         if self.ptr is not None and self.ptr.value:
             lib().bmfp_free(self.ptr)
@@ -213,6 +220,22 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """A contiguous slice of a DeviceArray; usable wherever a DeviceArray is read or written."""
+
+    def __init__(self, owner, start, n):
+        self.owner = owner
+        self.n = int(n)
+        self.dtype = owner.dtype
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p(owner.ptr.value + int(start) * self.dtype.itemsize)
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().bmfp_memcpy_d2h(_ptr(out), self.ptr, self.nbytes))
+        return out
 
 
 class Stream:
@@ -301,6 +324,13 @@ class Matrix:
         h = ctypes.c_void_p()
         check(lib().bmfp_bcsr_create(ctypes.byref(h), dtype, layout, n_rows, n_cols, bs,
                                      int(rp[-1]), _ptr(rp), _ptr(cols), _ptr(vals)))
+        return cls(h)
+
+    @classmethod
+    def stencil(cls, nx, ny, bs, kind=0, seed=0, dtype=F64, layout=ROW_MAJOR):
+        """9-point block stencil generated in device memory (same values as gen_stencil9)."""
+        h = ctypes.c_void_p()
+        check(lib().bmfp_stencil_create(ctypes.byref(h), nx, ny, bs, kind, seed, dtype, layout))
         return cls(h)
 
     @classmethod
@@ -456,7 +486,8 @@ class Refine:
 class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget_bytes", ctypes.c_size_t), ("max_systems", ctypes.c_int),
                 ("iterations_per_round", ctypes.c_int), ("slot_groups", ctypes.c_int),
-                ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int)]
+                ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int),
+                ("chunks_per_tile", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -485,6 +516,7 @@ def _declare_server(L):
     L.bmfp_server_step.argtypes = [vp, ip, ip]
     L.bmfp_server_poll.argtypes = [vp, ctypes.POINTER(RequestResult), vp, ctypes.c_size_t, ip]
     L.bmfp_server_peek.argtypes = [vp, ctypes.POINTER(RequestResult), ip]
+    L.bmfp_server_schedule.argtypes = [vp, ip, ip]
     L.bmfp_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), ip, ip,
                                     ctypes.POINTER(ctypes.c_double)]
 
@@ -562,6 +594,13 @@ class Server:
         check(lib().bmfp_server_poll(self.handle, ctypes.byref(res), x.ctypes.data if want_x else None,
                                      x.size, ctypes.byref(got)))
         return res, (x.reshape(res.nrhs, res.n) if want_x else None)
+
+    def schedule(self):
+        """(slot_groups, {bs: chunks per tile}) chosen by the configuration or the autotuner."""
+        g = ctypes.c_int(0)
+        per = (ctypes.c_int * 9)()
+        check(lib().bmfp_server_schedule(self.handle, ctypes.byref(g), per))
+        return g.value, {bs: per[bs] for bs in range(9) if per[bs]}
 
     def stats(self):
         used, cap = ctypes.c_size_t(0), ctypes.c_size_t(0)

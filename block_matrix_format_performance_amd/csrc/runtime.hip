@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "matrix.hpp"
+#include "stencil_kernels.hpp"
 
 namespace bmfp {
 
@@ -434,6 +435,56 @@ extern "C" int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_co
     BMFP_TRY(m->values.alloc(sizeof(float) * (size_t) nnz));
     if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, f.data(), sizeof(float) * (size_t) nnz, hipMemcpyHostToDevice));
   }
+  *out = m.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int kind, uint64_t seed, int dtype,
+                                   int layout)
+{
+  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || bs < 1 || (kind != 0 && kind != 1))
+    return fail(BMFP_ERR_INVALID, "bad stencil (nx %d, ny %d, bs %d, kind %d)", nx, ny, bs, kind);
+  if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
+  if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
+  const int64_t nodes = (int64_t) nx * ny, nnzb = stencil_block_count(nx, ny);
+  if (nnzb * bs * bs > INT_MAX || nodes * bs > INT_MAX) return fail(BMFP_ERR_INVALID, "stencil too large for int32 indices");
+  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+  m->is_bcsr = 1;
+  m->dtype = dtype;
+  m->layout = layout;
+  m->n_rows = m->n_cols = (int) nodes;
+  m->bs = bs;
+  m->nnzb = (int) nnzb;
+  m->max_row_blocks = std::min(nx, 3) * std::min(ny, 3);
+  m->mean_row_blocks = (double) nnzb / (double) nodes;
+  BMFP_HIP(hipGetDevice(&m->device));
+  std::vector<int> rp;
+  stencil_row_ptr(nx, ny, rp);
+  m->row_ptr = std::make_shared<DeviceBuffer>();
+  m->columns = std::make_shared<DeviceBuffer>();
+  BMFP_TRY(m->row_ptr->alloc(sizeof(int) * rp.size()));
+  BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnzb));
+  BMFP_HIP(hipMemcpy(m->row_ptr->ptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice));
+  const size_t nv = (size_t) m->n_vals();
+  const bool direct = dtype == BMFP_F64 && layout == BMFP_ROW_MAJOR;
+  DeviceBuffer staging;
+  double *gen = nullptr;
+  if (direct) {
+    BMFP_TRY(m->values.alloc(nv * sizeof(double)));
+    gen = m->values.as<double>();
+  } else {
+    BMFP_TRY(staging.alloc(nv * sizeof(double)));
+    gen = staging.as<double>();
+  }
+  const int64_t rows = nodes * bs;
+  k_gen_stencil_rows<<<(unsigned) ((rows + 255) / 256), 256, 0, nullptr>>>(nx, ny, bs, kind, seed,
+                                                                          m->row_ptr->as<int>(),
+                                                                          m->columns->as<int>(), gen);
+  BMFP_LAUNCHED();
+  if (!direct) BMFP_TRY(convert_values(*m, gen, BMFP_ROW_MAJOR, nullptr));
+  BMFP_HIP(hipDeviceSynchronize());
   *out = m.release();
   return BMFP_OK;
 }

@@ -1,25 +1,33 @@
 // serve.hip -- continuous-batching request loop (bmfp_server_*).
 //
-// Requests (a BCSR FP64 system with 1..4 right-hand sides) queue on the host,
-// are admitted while their footprint fits the device memory budget, and are
-// solved by FP64 iterative refinement around FP32 BiCGStab.  Every admitted
-// (request, right-hand side) pair is one "system"; all active systems advance
-// together through a per-system state machine on the device:
+// A request is an FP64 BCSR system with 1..4 right-hand sides, solved by FP64
+// iterative refinement around FP32 BiCGStab.  Admitted requests advance
+// together through a per-request state machine on the device; the right-hand
+// sides of one request move in lockstep so that every block row of its
+// matrix is read once per SpMV for all of them:
 //
 //   RESID   (slot 1)  x' = x + d, r = b - A64 x', r32 = r      fin1: DONE | INIT
-//   INIT    (slot 2)  d = 0, ri = rh = r32, p = v = 0           fin2: INNER_A
-//   INNER_A (slot 1)  p = ri + beta (p - omega v), v = A32 p    fin1: alpha
-//   INNER_B (slot 2)  s = ri - alpha v, t = A32 s               fin2: omega
+//   INIT    (slot 2)  d = 0, ri = rh = r32, p = v = 0           fin2: INNER_A | RESID
+//   INNER_A (slot 1)  p = ri + beta (p - omega v), v = A32 p    fin1: INNER_B | RESID
+//   INNER_B (slot 2)  s = ri - alpha v, t = A32 s               fin2: INNER_C
 //   INNER_C (slot 3)  ri = s - omega t, d += alpha p + omega s  fin3: INNER_A | RESID
 //
-// One batched iteration is six kernels (slot1 fin1 slot2 fin2 slot3 fin3),
-// captured several times over in one HIP graph that is replayed every round.
-// Slot kernels walk a tile table (a tile = 4 wavefronts x 64/bs block rows of
-// one system) with a persistent grid; a tile works only if its system's phase
-// matches the slot.  Each tile writes its own partial sums and the fin
-// kernels reduce each system's tiles in tile order, so a system's arithmetic
-// is the same whatever else is in the batch (bitwise identical results).
-// New requests are set up on a separate stream while rounds run.
+// Per right-hand side k two bit masks select the work: `live` (not yet
+// converged or failed) for RESID and `act` (still inside the current inner
+// solve) for the rest.  A right-hand side whose inner solve stops is frozen
+// until the others stop too, so its arithmetic is exactly that of solving it
+// alone.
+//
+// One batched iteration is seven kernels (RESID and INNER_A slot kernels,
+// fin1, slot2, fin2, slot3, fin3), captured iterations_per_round times in one
+// HIP graph.  Slot kernels give
+// every wavefront its own tiles from a tile table (no workgroup barriers); a
+// tile is `passes` consecutive chunks of one request, a chunk is 4 wavefront
+// passes of 64/bs block rows.  Partial sums are kept per chunk and the fin
+// kernels (one wavefront per request) reduce a request's chunks in chunk
+// order, so the arithmetic of a request depends neither on the rest of the
+// batch nor on the tile schedule: results are bitwise identical alone, in
+// any mix, and under any schedule the autotuner picks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,22 +39,29 @@
 
 #include "matrix.hpp"
 #include "solver_kernels.hpp"
+#include "stencil_kernels.hpp"
 
 namespace bmfp {
 
+constexpr int SERVE_MAX_RHS = 4;
+constexpr int SERVE_PART = 2 * SERVE_MAX_RHS;  // partial sums per chunk
+constexpr int SERVE_CHUNK_PASSES = 4;          // wavefront passes per chunk
+
 enum : int {
-  SYS_EMPTY = 0,
-  SYS_RESID = 1,
-  SYS_INIT = 2,
-  SYS_INNER_A = 3,
-  SYS_INNER_B = 4,
-  SYS_INNER_C = 5,
-  SYS_DONE = 6
+  REQ_EMPTY = 0,
+  REQ_RESID = 1,
+  REQ_INIT = 2,
+  REQ_INNER_A = 3,
+  REQ_INNER_B = 4,
+  REQ_INNER_C = 5,
+  REQ_DONE = 6
 };
 
-struct SysDesc {
-  int bs, n_brows, tile0, n_tiles;
-  int64_t n;
+// Device view of one request.  Vectors interleave the nrhs right-hand sides:
+// element i of right-hand side k is at i*nrhs + k.
+struct ReqDesc {
+  int bs, n_brows, nrhs, chunk0, n_chunks, pad;
+  int64_t n;  // scalar rows of one right-hand side
   const int *rp;
   const int *cols;
   const double *v64;
@@ -58,327 +73,602 @@ struct SysDesc {
   int max_outer, max_inner;
 };
 
-struct SysState {
-  int phase, status, outer, inner_it, inner_total, pad;
+struct RhsState {
+  int status, outer, inner_it, inner_total;
   double norm_b2, rel, rho, alpha, omega, beta, tt, inner_rr0;
 };
 
-struct TileDesc {
-  int sys, br0;
+// `outer` counts the refinement steps of the live right-hand sides and
+// `inner_it` the iterations of the current inner solve of the active ones
+// (each set moves in lockstep, so one counter serves all its members).
+struct ReqState {
+  int phase;
+  unsigned live, act;
+  int outer, inner_it, pad[3];
+  RhsState k[SERVE_MAX_RHS];
 };
 
-__host__ __device__ constexpr int serve_tile_brows(int bs) { return 4 * (64 / bs); }
+struct TileDesc {
+  int req, chunk, passes;
+};
+
+__host__ __device__ constexpr int serve_chunk_brows(int bs) { return SERVE_CHUNK_PASSES * (64 / bs); }
 
 namespace {
 
-// y = (A op)_row for the rows of one tile: lane (w, lbr, r) owns scalar row
-// (br0 + w*RPW + lbr)*BS + r; operand elements are shared by shuffles.
-template <class VT, int BS, class OP, class EPI>
-__device__ __forceinline__ void tile_spmv(const SysDesc &D, const VT *__restrict__ vals, int br0, const OP &op,
-                                          EPI &&epi)
+// Slot kernels: 0 = RESID, 1 = INNER_A, 2 = INIT | INNER_B, 3 = INNER_C.
+// Finalize kernels: 1 = RESID | INNER_A, 2 = INIT | INNER_B, 3 = INNER_C.
+template <int SLOT> __device__ __forceinline__ bool slot_owns(int phase)
 {
-  constexpr int RPW = 64 / BS;
-  using AT = decltype(op((int64_t) 0));
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
-  const int br = br0 + w * RPW + lbr;
-  const bool active = lane < RPW * BS && br < D.n_brows;
-  int b = 0, b1 = 0;
-  if (active) {
-    b = D.rp[br];
-    b1 = D.rp[br + 1];
-  }
-  AT acc = 0;
-  for (; b + 3 <= b1; b += 3) {
-    int col[3];
-    VT v[3][BS];
-    AT xe[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) col[k] = D.cols[b + k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) load_block_row<VT, BS, BMFP_ROW_MAJOR>(vals + (int64_t) (b + k) * (BS * BS), r, v[k]);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) xe[k] = op((int64_t) col[k] * BS + r);
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int c = 0; c < BS; ++c) acc = fmadd((AT) v[k][c], __shfl(xe[k], base_lane + c), acc);
-  }
-  for (; b < b1; ++b) {
-    VT v[BS];
-    load_block_row<VT, BS, BMFP_ROW_MAJOR>(vals + (int64_t) b * (BS * BS), r, v);
-    const AT xe = op((int64_t) D.cols[b] * BS + r);
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc = fmadd((AT) v[c], __shfl(xe, base_lane + c), acc);
-  }
-  if (active) epi((int64_t) br * BS + r, acc);
+  if constexpr (SLOT == 0) return phase == REQ_RESID;
+  else if constexpr (SLOT == 1) return phase == REQ_INNER_A;
+  else if constexpr (SLOT == 2) return phase == REQ_INIT || phase == REQ_INNER_B;
+  else return phase == REQ_INNER_C;
 }
 
-#define BMFP_SERVE_BS_SWITCH(bs, CALL) \
-  switch (bs) {                        \
+template <int SLOT> __device__ __forceinline__ bool fin_owns(int phase)
+{
+  if constexpr (SLOT == 1) return phase == REQ_RESID || phase == REQ_INNER_A;
+  else return slot_owns<SLOT>(phase);
+}
+
+using Acc = double[SERVE_MAX_RHS][2];
+
+// Pointers read from the request tables are generic; accesses through these
+// global-address-space views compile to global (not flat) loads and stores.
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>) p; }
+
+// Element `off` of an array: the byte offset fits 32 bits (guaranteed at
+// submission), so the access is a uniform base plus a 32-bit lane offset.
+template <class T> __device__ __forceinline__ T ld(gptr<const T> base, uint32_t off)
+{
+  return *(gptr<const T>) ((gptr<const char>) base + off * (uint32_t) sizeof(T));
+}
+
+// The NR interleaved right-hand-side values at element offset `off`
+// (off = i*NR, so the group is NR*sizeof(T)-aligned), in the widest loads and
+// stores that alignment allows.
+template <class T, int NR> __device__ __forceinline__ void ldv(gptr<const T> base, uint32_t off, T (&o)[NR])
+{
+  gptr<const char> p = (gptr<const char>) base + off * (uint32_t) sizeof(T);
+  constexpr int BYTES = (int) sizeof(T) * NR;
+  if constexpr (BYTES == 8 || BYTES == 16 || BYTES == 32) {
+    constexpr int PER = (BYTES >= 16 ? 16 : BYTES) / (int) sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(PER)));
+#pragma unroll
+    for (int c = 0; c < NR; c += PER) {
+      const vec_t t = *(gptr<const vec_t>) (p + c * (int) sizeof(T));
+#pragma unroll
+      for (int e = 0; e < PER; ++e) o[c + e] = t[e];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) o[c] = *(gptr<const T>) (p + c * (int) sizeof(T));
+  }
+}
+template <class T, int NR> __device__ __forceinline__ void stv(gptr<T> base, uint32_t off, const T (&v)[NR])
+{
+  gptr<char> p = (gptr<char>) base + off * (uint32_t) sizeof(T);
+  constexpr int BYTES = (int) sizeof(T) * NR;
+  if constexpr (BYTES == 8 || BYTES == 16 || BYTES == 32) {
+    constexpr int PER = (BYTES >= 16 ? 16 : BYTES) / (int) sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(PER)));
+#pragma unroll
+    for (int c = 0; c < NR; c += PER) {
+      vec_t t;
+#pragma unroll
+      for (int e = 0; e < PER; ++e) t[e] = v[c + e];
+      *(gptr<vec_t>) (p + c * (int) sizeof(T)) = t;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) *(gptr<T>) (p + c * (int) sizeof(T)) = v[c];
+  }
+}
+
+// N consecutive values at p (row r of a row-major block), in the widest loads
+// the alignment of a block row allows (up to 16 bytes).
+template <class T, int N> __device__ __forceinline__ void load_row(gptr<const T> p, T (&v)[N])
+{
+  constexpr int ALIGN = lowbit((int) sizeof(T) * N) < 16 ? lowbit((int) sizeof(T) * N) : 16;
+  constexpr int PER = ALIGN / (int) sizeof(T);
+  if constexpr (PER <= 1) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) v[c] = p[c];
+  } else {
+    typedef T vec_t __attribute__((ext_vector_type(PER)));
+#pragma unroll
+    for (int c = 0; c < N; c += PER) {
+      const vec_t t = *(gptr<const vec_t>) (p + c);
+#pragma unroll
+      for (int e = 0; e < PER; ++e) v[c + e] = t[e];
+    }
+  }
+}
+
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
+{
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order of additions: DPP
+// butterflies inside each 16-lane row, then the four row sums in row order.
+// The result is uniform.
+__device__ __forceinline__ double wave_total(double v)
+{
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+
+// Multi-right-hand-side SpMV over one chunk.  Lane (lbr, r) owns scalar row
+// i = (br0 + pass*RPW + lbr)*BS + r; each block row is loaded once and applied
+// to the gathered operands of all NR right-hand sides (operand elements shared
+// across the block row's lanes by shuffles).  All lanes of a block row run
+// the same trip counts, so every shuffle reads an active lane.  Vectors
+// interleave the right-hand sides: element i of k sits at i*NR + k.
+// gather(off, x[NR]) forms the operand at element offset off; own(off, m)
+// loads what the epilogue needs of the owned row before the block loop;
+// epi(off, y[NR], m) finishes the row.  Every right-hand side is computed;
+// the epilogues keep the values the state machine must not change.
+// Blocks whose loads are issued together: as many as keep the batch's
+// registers (block row values + G gathered vectors of NR operands) near 40.
+template <class VT, class AT, int BS, int NR, int G> constexpr int spmv_batch()
+{
+  constexpr int regs = BS * (int) sizeof(VT) / 4 + G * NR * (int) sizeof(AT) / 4;
+  return regs <= 13 ? 3 : (regs <= 20 ? 2 : 1);
+}
+
+template <class VT, class AT, int BS, int NR, int G, class OWNT, class GATHER, class OWN, class EPI>
+__device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals, int br0, const GATHER &gather,
+                                           const OWN &own, const EPI &epi)
+{
+  constexpr int RPW = 64 / BS;
+  constexpr int U = spmv_batch<VT, AT, BS, NR, G>();
+  const int lane = threadIdx.x & 63;
+  const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
+  const gptr<const int> rp = glb(D.rp), cols = glb(D.cols);
+#pragma unroll 1
+  for (int ps = 0; ps < SERVE_CHUNK_PASSES; ++ps) {
+    const int br = br0 + ps * RPW + lbr;
+    const bool on = lane < RPW * BS && br < D.n_brows;
+    const uint32_t off = (uint32_t) (br * BS + r) * NR;
+    int b = 0, b1 = 0;
+    OWNT mine;
+    if (on) {
+      b = ld(rp, (uint32_t) br);
+      b1 = ld(rp, (uint32_t) br + 1u);
+      own(off, mine);
+    }
+    AT acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0;
+    for (; b + U <= b1; b += U) {
+      uint32_t xo[U];
+      VT v[U][BS];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xo[u] = (uint32_t) (ld(cols, (uint32_t) (b + u)) * BS + r) * NR;
+#pragma unroll
+      for (int u = 0; u < U; ++u) load_row<VT, BS>(vals + ((int64_t) (b + u) * BS + r) * BS, v[u]);
+      AT xe[U][NR];
+#pragma unroll
+      for (int u = 0; u < U; ++u) gather(xo[u], xe[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+          for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[u][c], __shfl(xe[u][k], base_lane + c), acc[k]);
+    }
+    for (; U > 1 && b < b1; ++b) {
+      VT v[BS];
+      load_row<VT, BS>(vals + ((int64_t) b * BS + r) * BS, v);
+      AT xe[NR];
+      gather((uint32_t) (ld(cols, (uint32_t) b) * BS + r) * NR, xe);
+#pragma unroll
+      for (int k = 0; k < NR; ++k)
+#pragma unroll
+        for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[c], __shfl(xe[k], base_lane + c), acc[k]);
+    }
+    if (on) epi(off, acc, mine);
+  }
+}
+
+// Elementwise pass over the scalar rows of one chunk: f(off) per row.
+template <int BS, int NR, class F> __device__ __forceinline__ void chunk_rows(const ReqDesc &D, int br0, const F &f)
+{
+  constexpr int RPW = 64 / BS;
+  const int lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (int ps = 0; ps < SERVE_CHUNK_PASSES; ++ps) {
+    const int64_t row = (int64_t) (br0 + ps * RPW) * BS + lane;
+    if (lane < RPW * BS && row < D.n) f((uint32_t) row * NR);
+  }
+}
+
+template <int NR> struct ResidOwn {
+  double xr[NR], b[NR], old[NR];
+};
+
+template <int BS, int NR>
+__device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const int par = S.outer & 1;
+  const unsigned live = S.live;
+  const gptr<const double> xs = glb((const double *) D.x[par]), b = glb(D.b);
+  const gptr<double> xn = glb(D.x[1 - par]);
+  const gptr<const float> d = glb((const float *) D.d);
+  const gptr<float> r32 = glb(D.r32);
+  auto gather = [&](uint32_t o, double (&x)[NR]) {
+    float dd[NR];
+    ldv<double, NR>(xs, o, x);
+    ldv<float, NR>(d, o, dd);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) x[k] += (double) dd[k];
+  };
+  auto own = [&](uint32_t o, ResidOwn<NR> &m) {
+    gather(o, m.xr);
+    ldv<double, NR>(b, o, m.b);
+    ldv<double, NR>((gptr<const double>) xn, o, m.old);
+  };
+  chunk_spmv<double, double, BS, NR, 2, ResidOwn<NR>>(D, glb(D.v64), br0, gather, own,
+                                                   [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
+                                                     double xw[NR];
+                                                     float rw[NR];
+#pragma unroll
+                                                     for (int k = 0; k < NR; ++k) {
+                                                       // a finished right-hand side keeps its iterate
+                                                       xw[k] = ((live >> k) & 1u) ? m.xr[k] : m.old[k];
+                                                       const double ri = m.b[k] - y[k];
+                                                       rw[k] = (float) ri;
+                                                       acc[k][0] += ri * ri;
+                                                       acc[k][1] += m.b[k] * m.b[k];
+                                                     }
+                                                     stv<double, NR>(xn, o, xw);
+                                                     stv<float, NR>(r32, o, rw);
+                                                   });
+}
+
+template <int NR> struct AOwn {
+  float p[NR], rh[NR];
+};
+
+template <int BS, int NR>
+__device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const int it = S.inner_it;
+  float beta[NR], omega[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    beta[k] = (float) S.k[k].beta;
+    omega[k] = (float) (it == 0 ? 1.0 : S.k[k].omega);
+  }
+  const gptr<const float> pp = glb((const float *) D.p[(it - 1) & 1]), vp = glb((const float *) D.v[(it - 1) & 1]);
+  const gptr<float> pc = glb(D.p[it & 1]), vc = glb(D.v[it & 1]);
+  const gptr<const float> ri = glb((const float *) D.ri), rh = glb((const float *) D.rh);
+  auto gather = [&](uint32_t o, float (&x)[NR]) {
+    float rr[NR], p0[NR], v0[NR];
+    ldv<float, NR>(ri, o, rr);
+    ldv<float, NR>(pp, o, p0);
+    ldv<float, NR>(vp, o, v0);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) x[k] = p_update(rr[k], p0[k], v0[k], beta[k], omega[k]);
+  };
+  auto own = [&](uint32_t o, AOwn<NR> &m) {
+    gather(o, m.p);
+    ldv<float, NR>(rh, o, m.rh);
+  };
+  chunk_spmv<float, float, BS, NR, 3, AOwn<NR>>(D, glb(D.v32), br0, gather, own,
+                                             [&](uint32_t o, const float (&y)[NR], const AOwn<NR> &m) {
+                                               stv<float, NR>(pc, o, m.p);
+                                               stv<float, NR>(vc, o, y);
+#pragma unroll
+                                               for (int k = 0; k < NR; ++k) acc[k][0] += (double) m.rh[k] * (double) y[k];
+                                             });
+}
+
+template <int NR> struct BOwn {
+  float s[NR];
+};
+
+template <int BS, int NR>
+__device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  float alpha[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) alpha[k] = (float) S.k[k].alpha;
+  const gptr<const float> vcur = glb((const float *) D.v[S.inner_it & 1]), ri = glb((const float *) D.ri);
+  const gptr<float> s_out = glb(D.s), t_out = glb(D.t);
+  auto gather = [&](uint32_t o, float (&x)[NR]) {
+    float rr[NR], vv[NR];
+    ldv<float, NR>(ri, o, rr);
+    ldv<float, NR>(vcur, o, vv);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) x[k] = s_update(rr[k], vv[k], alpha[k]);
+  };
+  chunk_spmv<float, float, BS, NR, 2, BOwn<NR>>(D, glb(D.v32), br0, gather,
+                                             [&](uint32_t o, BOwn<NR> &m) { gather(o, m.s); },
+                                             [&](uint32_t o, const float (&y)[NR], const BOwn<NR> &m) {
+                                               stv<float, NR>(s_out, o, m.s);
+                                               stv<float, NR>(t_out, o, y);
+#pragma unroll
+                                               for (int k = 0; k < NR; ++k) {
+                                                 acc[k][0] += (double) y[k] * (double) m.s[k];
+                                                 acc[k][1] += (double) y[k] * (double) y[k];
+                                               }
+                                             });
+}
+
+template <int BS, int NR>
+__device__ __forceinline__ void init_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const gptr<const float> r32 = glb((const float *) D.r32);
+  const gptr<float> d = glb(D.d), ri = glb(D.ri), rh = glb(D.rh);
+  const gptr<float> p0 = glb(D.p[0]), p1 = glb(D.p[1]), v0 = glb(D.v[0]), v1 = glb(D.v[1]);
+  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
+    float r0[NR], z[NR];
+    ldv<float, NR>(r32, o, r0);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      z[k] = 0.f;
+      const double q = (double) r0[k] * (double) r0[k];
+      acc[k][0] += q;
+      acc[k][1] += q;
+    }
+    stv<float, NR>(d, o, z);
+    stv<float, NR>(ri, o, r0);
+    stv<float, NR>(rh, o, r0);
+    stv<float, NR>(p0, o, z);
+    stv<float, NR>(p1, o, z);
+    stv<float, NR>(v0, o, z);
+    stv<float, NR>(v1, o, z);
+  });
+}
+
+template <int BS, int NR>
+__device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const unsigned act = S.act;
+  float alpha[NR], omega[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    alpha[k] = (float) S.k[k].alpha;
+    omega[k] = (float) S.k[k].omega;
+  }
+  const gptr<const float> pcur = glb((const float *) D.p[S.inner_it & 1]), s_in = glb((const float *) D.s),
+                          t_in = glb((const float *) D.t), rh = glb((const float *) D.rh);
+  const gptr<float> d = glb(D.d), ri = glb(D.ri);
+  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
+    float s[NR], p[NR], dd[NR], t[NR], h[NR], rn[NR];
+    ldv<float, NR>(s_in, o, s);
+    ldv<float, NR>(pcur, o, p);
+    ldv<float, NR>((gptr<const float>) d, o, dd);
+    ldv<float, NR>(t_in, o, t);
+    ldv<float, NR>(rh, o, h);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      rn[k] = fmadd(-omega[k], t[k], s[k]);
+      // a frozen right-hand side keeps its correction for the next refinement step
+      if ((act >> k) & 1u) dd[k] = fmadd(omega[k], s[k], fmadd(alpha[k], p[k], dd[k]));
+      acc[k][0] += (double) h[k] * (double) rn[k];
+      acc[k][1] += (double) rn[k] * (double) rn[k];
+    }
+    stv<float, NR>(d, o, dd);
+    stv<float, NR>(ri, o, rn);
+  });
+}
+
+#define BMFP_SERVE_BS_SWITCH(bs, CALL)           \
+  switch (bs) {                                  \
   case 1: { constexpr int BS = 1; CALL; } break; \
   case 2: { constexpr int BS = 2; CALL; } break; \
   case 3: { constexpr int BS = 3; CALL; } break; \
   case 4: { constexpr int BS = 4; CALL; } break; \
   case 6: { constexpr int BS = 6; CALL; } break; \
   case 8: { constexpr int BS = 8; CALL; } break; \
-  default: break;                      \
+  default: break;                                \
+  }
+#define BMFP_SERVE_NR_SWITCH(nr, CALL)           \
+  switch (nr) {                                  \
+  case 1: { constexpr int NR = 1; CALL; } break; \
+  case 2: { constexpr int NR = 2; CALL; } break; \
+  case 3: { constexpr int NR = 3; CALL; } break; \
+  case 4: { constexpr int NR = 4; CALL; } break; \
+  default: break;                                \
   }
 
-__device__ __forceinline__ int tile_phase(const SysState *ss, int sys)
+template <int SLOT, int BS, int NR>
+__device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, int phase, const TileDesc &T,
+                                         gptr<double> part)
 {
-  __shared__ int s_phase;
-  __syncthreads();
-  if (threadIdx.x == 0) s_phase = ss[sys].phase;
-  __syncthreads();
-  return s_phase;
-}
-
-__device__ __forceinline__ void tile_partials(double *__restrict__ part, int tile, double (&v)[2], double (*lds)[4])
-{
-  block_sum<2>(v, lds);
-  if (threadIdx.x == 0) {
-    part[(int64_t) tile * 2] = v[0];
-    part[(int64_t) tile * 2 + 1] = v[1];
-  }
-}
-
-// ---- slot 1: RESID (FP64 residual) and INNER_A (v = A32 p) -------------------
-__global__ __launch_bounds__(256) void k_slot1(const SysDesc *__restrict__ sd, const SysState *__restrict__ ss,
-                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                               double *__restrict__ part)
-{
-  __shared__ double lds[2][4];
-  const int nt = *ntp;
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const TileDesc T = tiles[tile];
-    const int phase = tile_phase(ss, T.sys);
-    if (phase != SYS_RESID && phase != SYS_INNER_A) continue;
-    const SysDesc &D = sd[T.sys];
-    const SysState &S = ss[T.sys];
-    double acc[2] = {0.0, 0.0};
-    if (phase == SYS_RESID) {
-      const int xp = S.outer & 1;
-      const OperandXD op{D.x[xp], D.d};
-      double *xn = D.x[1 - xp];
-      BMFP_SERVE_BS_SWITCH(D.bs, (tile_spmv<double, BS>(D, D.v64, T.br0, op, [&](int64_t row, double y) {
-                             xn[row] = op.raw(row);
-                             const double bi = D.b[row];
-                             const double ri = bi - y;
-                             D.r32[row] = (float) ri;
-                             acc[0] += ri * ri;
-                             acc[1] += bi * bi;
-                           })));
+  const int lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (int c = 0; c < T.passes; ++c) {
+    const int chunk = T.chunk + c;
+    const int br0 = chunk * serve_chunk_brows(BS);
+    Acc acc = {};
+    if constexpr (SLOT == 0) {
+      resid_chunk<BS, NR>(D, S, br0, acc);
+    } else if constexpr (SLOT == 1) {
+      inner_a_chunk<BS, NR>(D, S, br0, acc);
+    } else if constexpr (SLOT == 2) {
+      if (phase == REQ_INIT) init_chunk<BS, NR>(D, S, br0, acc);
+      else inner_b_chunk<BS, NR>(D, S, br0, acc);
     } else {
-      const int it = S.inner_it, pv = (it - 1) & 1, cu = it & 1;
-      const OperandP<float> op{D.ri, D.p[pv], D.v[pv], nullptr, (float) S.beta, (float) (it == 0 ? 1.0 : S.omega)};
-      float *pc = D.p[cu];
-      float *vc = D.v[cu];
-      BMFP_SERVE_BS_SWITCH(D.bs, (tile_spmv<float, BS>(D, D.v32, T.br0, op, [&](int64_t row, float y) {
-                             pc[row] = op.raw(row);
-                             vc[row] = y;
-                             acc[0] += (double) D.rh[row] * (double) y;
-                           })));
+      inner_c_chunk<BS, NR>(D, S, br0, acc);
     }
-    tile_partials(part, tile, acc, lds);
+    // per-chunk partial sums (0 for absent right-hand sides), lane j < 8 writes value j
+    double mine = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const double s0 = wave_total(acc[k][0]), s1 = wave_total(acc[k][1]);
+      mine = lane == 2 * k ? s0 : (lane == 2 * k + 1 ? s1 : mine);
+    }
+    if (lane < SERVE_PART) *(part + (int64_t) (D.chunk0 + chunk) * SERVE_PART + lane) = mine;
   }
 }
 
-// ---- slot 2: INIT (inner start) and INNER_B (t = A32 s) ----------------------
-__global__ __launch_bounds__(256) void k_slot2(const SysDesc *__restrict__ sd, const SysState *__restrict__ ss,
-                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                               double *__restrict__ part)
-{
-  __shared__ double lds[2][4];
-  const int nt = *ntp;
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const TileDesc T = tiles[tile];
-    const int phase = tile_phase(ss, T.sys);
-    if (phase != SYS_INIT && phase != SYS_INNER_B) continue;
-    const SysDesc &D = sd[T.sys];
-    const SysState &S = ss[T.sys];
-    double acc[2] = {0.0, 0.0};
-    if (phase == SYS_INIT) {
-      const int rows = serve_tile_brows(D.bs) * D.bs;
-      const int64_t row = (int64_t) T.br0 * D.bs + threadIdx.x;
-      if (threadIdx.x < rows && row < D.n) {
-        const float r0 = D.r32[row];
-        D.d[row] = 0.f;
-        D.ri[row] = r0;
-        D.rh[row] = r0;
-        D.p[0][row] = D.p[1][row] = 0.f;
-        D.v[0][row] = D.v[1][row] = 0.f;
-        acc[0] = (double) r0 * (double) r0;
-        acc[1] = acc[0];
-      }
-    } else {
-      const int cu = S.inner_it & 1;
-      const OperandS<float> op{D.ri, D.v[cu], nullptr, (float) S.alpha};
-      BMFP_SERVE_BS_SWITCH(D.bs, (tile_spmv<float, BS>(D, D.v32, T.br0, op, [&](int64_t row, float y) {
-                             const float s = op.raw(row);
-                             D.s[row] = s;
-                             D.t[row] = y;
-                             acc[0] += (double) y * (double) s;
-                             acc[1] += (double) y * (double) y;
-                           })));
-    }
-    tile_partials(part, tile, acc, lds);
-  }
-}
-
-// ---- slot 3: INNER_C (ri = s - omega t, d += alpha p + omega s) --------------
-__global__ __launch_bounds__(256) void k_slot3(const SysDesc *__restrict__ sd, const SysState *__restrict__ ss,
-                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                               double *__restrict__ part)
-{
-  __shared__ double lds[2][4];
-  const int nt = *ntp;
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const TileDesc T = tiles[tile];
-    const int phase = tile_phase(ss, T.sys);
-    if (phase != SYS_INNER_C) continue;
-    const SysDesc &D = sd[T.sys];
-    const SysState &S = ss[T.sys];
-    double acc[2] = {0.0, 0.0};
-    const int rows = serve_tile_brows(D.bs) * D.bs;
-    const int64_t row = (int64_t) T.br0 * D.bs + threadIdx.x;
-    if (threadIdx.x < rows && row < D.n) {
-      const float a = (float) S.alpha, w = (float) S.omega;
-      const float s = D.s[row];
-      const float h = fmadd(a, D.p[S.inner_it & 1][row], D.d[row]);
-      const float rn = fmadd(-w, D.t[row], s);
-      D.d[row] = fmadd(w, s, h);
-      D.ri[row] = rn;
-      acc[0] = (double) D.rh[row] * (double) rn;
-      acc[1] = (double) rn * (double) rn;
-    }
-    tile_partials(part, tile, acc, lds);
-  }
-}
-
-// ---- finalize: one wavefront per system, fixed-order reduction ----------------
+// ---- slot kernels: every wavefront walks its own tiles -------------------------
 template <int SLOT>
-__global__ __launch_bounds__(256) void k_fin(const SysDesc *__restrict__ sd, SysState *__restrict__ ss,
+__global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
+                                              const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
+                                              double *__restrict__ part)
+{
+  const int nt = *ntp;
+  const int wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const int nw = (int) gridDim.x * 4;
+  for (int ti = wave; ti < nt; ti += nw) {
+    const TileDesc T = tiles[ti];
+    const ReqState &S = rs[T.req];
+    const int phase = S.phase;
+    if (!slot_owns<SLOT>(phase)) continue;
+    const ReqDesc &D = rd[T.req];
+    BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+  }
+}
+
+// ---- finalize: one wavefront per request, lane k owns right-hand side k --------
+template <int SLOT>
+__global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, ReqState *__restrict__ rs,
                                              const int *__restrict__ list, const int *__restrict__ nlp,
                                              const double *__restrict__ part)
 {
-  const int wave = (int) ((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  if (wave >= *nlp) return;
-  const int sys = list[wave];
-  SysState &S = ss[sys];
+  const int w = (int) ((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (w >= *nlp) return;
+  const int q = list[w];
+  ReqState &S = rs[q];
   const int phase = S.phase;
-  const bool mine = SLOT == 1 ? (phase == SYS_RESID || phase == SYS_INNER_A)
-                  : SLOT == 2 ? (phase == SYS_INIT || phase == SYS_INNER_B)
-                              : phase == SYS_INNER_C;
-  if (!mine) return;
-  const SysDesc &D = sd[sys];
-  double a0 = 0.0, a1 = 0.0;
-  for (int k = lane; k < D.n_tiles; k += 64) {
-    a0 += part[(int64_t) (D.tile0 + k) * 2];
-    a1 += part[(int64_t) (D.tile0 + k) * 2 + 1];
+  if (!fin_owns<SLOT>(phase)) return;
+  const ReqDesc &D = rd[q];
+  double a[SERVE_PART];
+#pragma unroll
+  for (int j = 0; j < SERVE_PART; ++j) a[j] = 0.0;
+  for (int c = lane; c < D.n_chunks; c += 64) {
+    const double *pc = part + (int64_t) (D.chunk0 + c) * SERVE_PART;
+#pragma unroll
+    for (int j = 0; j < SERVE_PART; ++j) a[j] += pc[j];
   }
-  a0 = wave_sum(a0);
-  a1 = wave_sum(a1);
-  if (lane != 0) return;
-  if (phase == SYS_RESID) {
-    // a0 = ||r||^2, a1 = ||b||^2 for x' = x + d (now in x[1 - outer%2])
-    if (S.outer == 0) S.norm_b2 = a1;
-    S.outer += 1;
-    // b = 0 gives x = 0 exactly; a non-finite ||b|| or ||r|| gives a NaN rel
-    const double rel = S.norm_b2 == 0.0 ? 0.0 : sqrt(a0) / sqrt(S.norm_b2);
-    S.rel = rel;
-    if (!isfinite(rel)) { S.status = BMFP_ERR_NONFINITE; S.phase = SYS_DONE; }
-    else if (rel < D.tol) { S.status = BMFP_OK; S.phase = SYS_DONE; }
-    else if (S.outer > D.max_outer) { S.status = BMFP_ERR_NOT_CONVERGED; S.phase = SYS_DONE; }
-    else S.phase = SYS_INIT;
-  } else if (phase == SYS_INIT) {
-    // a0 = (rh, ri) = rho_0, a1 = ||ri||^2
-    S.inner_rr0 = a1;
-    S.inner_it = 0;
-    S.rho = a0;
-    S.alpha = 1.0;
-    S.omega = 1.0;
-    S.beta = a0;  // rho_0 / 1 * 1 / 1
-    S.phase = (a0 == 0.0 || !isfinite(a0)) ? SYS_RESID : SYS_INNER_A;
-  } else if (phase == SYS_INNER_A) {
-    // a0 = (rh, v)
-    if (a0 == 0.0 || !isfinite(a0)) {
-      S.inner_total += S.inner_it;
-      S.phase = SYS_RESID;  // inner breakdown: d keeps the last valid iterate
-    } else {
-      S.alpha = S.rho / a0;
-      S.phase = SYS_INNER_B;
+#pragma unroll
+  for (int j = 0; j < SERVE_PART; ++j) a[j] = wave_sum(a[j]);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < SERVE_MAX_RHS; ++k)
+    if (lane == k) {
+      a0 = a[2 * k];
+      a1 = a[2 * k + 1];
     }
-  } else if (phase == SYS_INNER_B) {
-    // a0 = (t, s), a1 = (t, t)
-    S.tt = a1;
-    S.omega = a1 == 0.0 ? 0.0 : a0 / a1;
-    S.phase = SYS_INNER_C;
-  } else {
-    // a0 = (rh, ri), a1 = ||ri||^2 after the update
-    S.inner_it += 1;
-    const double rel_in = S.inner_rr0 > 0.0 ? sqrt(a1) / sqrt(S.inner_rr0) : 0.0;
-    const bool stop = !(rel_in >= D.inner_tol) || S.inner_it >= D.max_inner || S.tt == 0.0 || S.omega == 0.0 ||
-                      a0 == 0.0 || !isfinite(a0);
-    if (stop) {
-      S.inner_total += S.inner_it;
-      S.phase = SYS_RESID;
+  const unsigned live = S.live, act = S.act;
+  const int outer = S.outer, it = S.inner_it;
+  const int k = lane;
+  bool keep = false;
+  if (k < D.nrhs) {
+    RhsState &R = S.k[k];
+    if (phase == REQ_RESID) {
+      if ((live >> k) & 1u) {
+        // a0 = ||r||^2, a1 = ||b||^2 for x' = x + d (now in x[1 - outer%2])
+        if (outer == 0) R.norm_b2 = a1;
+        R.outer = outer + 1;
+        // b = 0 gives x = 0 exactly; a non-finite ||b|| or ||r|| gives a NaN rel
+        const double rel = R.norm_b2 == 0.0 ? 0.0 : sqrt(a0) / sqrt(R.norm_b2);
+        R.rel = rel;
+        if (!isfinite(rel)) R.status = BMFP_ERR_NONFINITE;
+        else if (rel < D.tol) R.status = BMFP_OK;
+        else if (R.outer > D.max_outer) R.status = BMFP_ERR_NOT_CONVERGED;
+        else keep = true;
+      }
+    } else if ((act >> k) & 1u) {
+      if (phase == REQ_INIT) {
+        // a0 = (rh, ri) = rho_0, a1 = ||ri||^2
+        R.inner_rr0 = a1;
+        R.inner_it = 0;
+        R.rho = a0;
+        R.alpha = 1.0;
+        R.omega = 1.0;
+        R.beta = a0;
+        keep = !(a0 == 0.0 || !isfinite(a0));
+      } else if (phase == REQ_INNER_A) {
+        // a0 = (rh, v); a breakdown ends the inner solve, d keeps the last iterate
+        if (a0 == 0.0 || !isfinite(a0)) {
+          R.inner_total += it;
+        } else {
+          R.alpha = R.rho / a0;
+          keep = true;
+        }
+      } else if (phase == REQ_INNER_B) {
+        // a0 = (t, s), a1 = (t, t)
+        R.tt = a1;
+        R.omega = a1 == 0.0 ? 0.0 : a0 / a1;
+        keep = true;
+      } else {
+        // a0 = (rh, ri), a1 = ||ri||^2 after the update
+        R.inner_it = it + 1;
+        const double rel_in = R.inner_rr0 > 0.0 ? sqrt(a1) / sqrt(R.inner_rr0) : 0.0;
+        const bool stop = !(rel_in >= D.inner_tol) || it + 1 >= D.max_inner || R.tt == 0.0 || R.omega == 0.0 ||
+                          a0 == 0.0 || !isfinite(a0);
+        if (stop) {
+          R.inner_total += it + 1;
+        } else {
+          R.beta = a0 / R.rho * R.alpha / R.omega;
+          R.rho = a0;
+          keep = true;
+        }
+      }
+    }
+  }
+  const unsigned kept = (unsigned) (__ballot(keep) & 0xFull);
+  if (lane == 0) {
+    if (phase == REQ_RESID) {
+      S.outer = outer + 1;
+      S.live = kept;
+      S.act = kept;
+      S.phase = kept ? REQ_INIT : REQ_DONE;
+    } else if (phase == REQ_INIT) {
+      S.inner_it = 0;
+      S.act = kept;
+      S.phase = kept ? REQ_INNER_A : REQ_RESID;
+    } else if (phase == REQ_INNER_B) {
+      S.phase = REQ_INNER_C;
     } else {
-      S.beta = a0 / S.rho * S.alpha / S.omega;
-      S.rho = a0;
-      S.phase = SYS_INNER_A;
+      if (phase == REQ_INNER_C) S.inner_it = it + 1;
+      S.act = kept;
+      S.phase = kept ? (phase == REQ_INNER_A ? REQ_INNER_B : REQ_INNER_A) : REQ_RESID;
     }
   }
 }
 
-// ---- admission-time device work --------------------------------------------------
-// 9-point stencil values, one thread per scalar row; identical operations (and
-// order of the row sums) to the host generator, with explicit rounding so no
-// multiply-add is fused.
-__global__ __launch_bounds__(256) void k_gen_stencil_rows(int nx, int ny, int bs, int kind, uint64_t seed,
-                                                          const int *__restrict__ rp, int *__restrict__ cols,
-                                                          double *__restrict__ vals)
+// ---- admission-time device work (the stencil generator is in stencil_kernels.hpp)
+// Stencil block row pointers in closed form: the nodes before (iy, ix) hold
+// F(iy, ny) * Sx + f(iy, ny) * F(ix, nx) blocks, with f(i, m) = 3 - [i = 0] -
+// [i = m-1] neighbours per axis, F(i, m) = sum of f over the first i indices
+// and Sx = F(nx, nx).
+__device__ __forceinline__ int64_t stencil_prefix(int i, int m) { return 3 * (int64_t) i - (i >= 1) - (i >= m); }
+
+__global__ __launch_bounds__(256) void k_stencil_rp(int nx, int ny, int *__restrict__ rp)
 {
-  const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
-  const int64_t n = (int64_t) nx * ny * bs;
-  if (t >= n) return;
-  const int64_t i = t / bs;
-  const int r = (int) (t - i * bs);
-  const int iy = (int) (i / nx), ix = (int) (i - (int64_t) iy * nx);
-  const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
-  const int64_t bs2 = (int64_t) bs * bs;
-  int64_t nb = rp[i];
-  int64_t diag = -1;
-  double sum = 0.0;
-  for (int s = 0; s < 9; s++) {
-    const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
-    if (jy < 0 || jy >= ny || jx < 0 || jx >= nx) continue;
-    if (r == 0) cols[nb] = jy * nx + jx;
-    const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) bs2;
-    double *row = vals + nb * bs2 + (int64_t) r * bs;
-    if (s == 4) diag = nb;
-    for (int c = 0; c < bs; c++) {
-      double v;
-      if (kind == 0) {
-        v = rng_sym(ka, base + (uint64_t) r * bs + c);
-      } else if (s == 4) {
-        v = 0.0;
-      } else {
-        const int lo = r < c ? r : c, hi = r < c ? c : r;
-        const double k = rng_unit(ka, base + (uint64_t) lo * bs + hi);
-        double cc = 0.0;
-        if (r < c) cc = rng_sym(kc, base + (uint64_t) r * bs + c);
-        else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * bs + r);
-        v = -__dadd_rn(k, __dmul_rn(0.3, cc));
-      }
-      row[c] = v;
-      if (!(s == 4 && c == r)) sum = __dadd_rn(sum, fabs(v));
+  const int64_t nodes = (int64_t) nx * ny;
+  const int64_t sx = stencil_prefix(nx, nx);
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i <= nodes; i += (int64_t) gridDim.x * 256) {
+    int64_t v;
+    if (i == nodes) {
+      v = stencil_prefix(ny, ny) * sx;
+    } else {
+      const int iy = (int) (i / nx), ix = (int) (i - (int64_t) iy * nx);
+      const int64_t fy = 3 - (iy == 0) - (iy == ny - 1);
+      v = stencil_prefix(iy, ny) * sx + fy * stencil_prefix(ix, nx);
     }
-    nb++;
+    rp[i] = (int) v;
   }
-  double *dg = vals + diag * bs2 + (int64_t) r * bs + r;
-  if (kind == 0) *dg = __dadd_rn(*dg, __dadd_rn(sum, 1.0));
-  else *dg = __dadd_rn(__dmul_rn(1.02, sum), 1.0);
 }
 
 __global__ __launch_bounds__(256) void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n)
@@ -386,12 +676,46 @@ __global__ __launch_bounds__(256) void k_to_f32(const double *__restrict__ a, fl
   for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) b[i] = (float) a[i];
 }
 
-// b[k] for right-hand side k of a request: uniform[-1,1] keyed by (seed, k*n + i).
-__global__ __launch_bounds__(256) void k_rhs(double *__restrict__ b, int64_t n, int k, uint64_t seed)
+// b of a request with nr right-hand sides, interleaved (element i of k at
+// i*nr + k): uniform[-1,1] keyed by (seed, k*n + i).
+__global__ __launch_bounds__(256) void k_rhs(double *__restrict__ b, int64_t n, int nr, uint64_t seed)
 {
   const uint64_t key = rng_key(seed, STREAM_RHS);
-  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256)
-    b[i] = rng_sym(key, (uint64_t) k * (uint64_t) n + (uint64_t) i);
+  for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < n * nr; t += (int64_t) gridDim.x * 256) {
+    const int64_t i = t / nr, k = t - i * nr;
+    b[t] = rng_sym(key, (uint64_t) (k * n + i));
+  }
+}
+
+// Solutions in right-hand-side-major order: out[k*n + i] = x_par(k)[i*nr + k],
+// where bit k of par_mask names the iterate buffer holding right-hand side k.
+__global__ __launch_bounds__(256) void k_gather_solution(const double *__restrict__ x0, const double *__restrict__ x1,
+                                                         unsigned par_mask, double *__restrict__ out, int64_t n,
+                                                         int nr)
+{
+  for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < n * nr; t += (int64_t) gridDim.x * 256) {
+    const int64_t k = t / n, i = t - k * n;
+    out[t] = (((par_mask >> k) & 1u) ? x1 : x0)[i * nr + k];
+  }
+}
+
+// Initial state of a request: every right-hand side live, refinement step 0.
+// `phase`, `act` and the scalars allow forcing a phase (schedule tuning).
+__global__ void k_req_init(ReqState *__restrict__ st, int nrhs, int phase)
+{
+  if (threadIdx.x != 0) return;
+  ReqState S{};
+  S.phase = phase;
+  S.live = S.act = (1u << nrhs) - 1u;
+  S.inner_it = phase == REQ_RESID ? 0 : 1;
+  for (int k = 0; k < SERVE_MAX_RHS; ++k) {
+    S.k[k] = RhsState{};
+    S.k[k].status = BMFP_OK;
+    S.k[k].alpha = S.k[k].omega = S.k[k].beta = 0.5;
+    S.k[k].rho = 1.0;
+    S.k[k].inner_it = phase == REQ_RESID ? 0 : 1;
+  }
+  *st = S;
 }
 
 }  // namespace

@@ -140,6 +140,12 @@ int bmfp_bcsr_create(bmfp_matrix *out, int dtype, int layout, int n_rows,
 int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_cols,
                     int nnz, const int *row_ptr, const int *columns,
                     const double *values);
+/* The 9-point block stencil of bmfp_gen_stencil9 generated directly in
+ * device memory (bit-identical values), stored as dtype / layout.  No
+ * reference counterpart: it builds the multi-GB synthetic workloads without
+ * a host round trip. */
+int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int kind,
+                        uint64_t seed, int dtype, int layout);
 /* Device-side copy of a BCSR matrix into another storage type / layout
  * (indices shared by reference count).  Used for mixed precision. */
 int bmfp_matrix_convert(bmfp_matrix src, int dtype, int layout,
@@ -223,26 +229,29 @@ int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps,
 int bmfp_refine_destroy(bmfp_solver s);
 
 /* ---- serving: continuous-batching request loop (no reference counterpart) --
- * Requests -- an FP64 BCSR system with 1..nrhs right-hand sides -- queue on
- * the host and are admitted first-come first-served while their device
- * footprint (FP64 matrix + FP32 copy + 13 vectors per right-hand side) fits
- * the memory budget.  Admitted requests are set up on their own stream while
- * earlier ones compute, then join the batch: every (request, right-hand side)
- * is solved by FP64 iterative refinement around FP32 BiCGStab, all systems
- * advancing together through batched persistent kernels replayed from a HIP
- * graph (iterations_per_round batched iterations per bmfp_server_step).
- * Each system's arithmetic is independent of the batch it shares: results
- * are bitwise identical alone and in any mix.  Block sizes 1,2,3,4,6,8. */
+ * Requests -- an FP64 BCSR system with 1..4 right-hand sides -- queue on the
+ * host and are admitted first-come first-served while their device footprint
+ * (FP64 matrix + FP32 copy + 13 vectors per right-hand side) fits the memory
+ * budget.  Admitted requests are set up on their own stream while earlier
+ * ones compute, then join the batch: each request is solved by FP64
+ * iterative refinement around FP32 BiCGStab, its right-hand sides in lockstep
+ * (one read of the matrix per SpMV for all of them), all requests advancing
+ * together through batched persistent kernels replayed from a HIP graph
+ * (iterations_per_round batched iterations per bmfp_server_step).  A
+ * request's arithmetic is independent of the batch it shares and of the tile
+ * schedule: results are bitwise identical alone and in any mix.  Block sizes
+ * 1,2,3,4,6,8. */
 typedef struct bmfp_server_t *bmfp_server;
 
 typedef struct {
   size_t memory_budget_bytes; /* device bytes for admitted requests (capped at 90% free) */
-  int max_systems;            /* concurrent (request, right-hand side) systems         */
+  int max_systems;            /* resident requests (active + finished, not yet polled) */
   int iterations_per_round;   /* batched iterations per graph replay                   */
-  int slot_groups;            /* persistent workgroups of the batched SpMV kernels     */
+  int slot_groups;            /* persistent workgroups of the slot kernels; 0: tuned   */
   double inner_tol;           /* FP32 BiCGStab relative tolerance per refinement step  */
-  int max_outer;              /* refinement steps per system                           */
+  int max_outer;              /* refinement steps per right-hand side                  */
   int max_inner;              /* inner iterations per refinement step                  */
+  int chunks_per_tile;        /* tile height in 256-row chunks; 0: tuned per bs        */
 } bmfp_server_config;
 
 typedef struct {
@@ -284,11 +293,16 @@ int bmfp_server_submit(bmfp_server s, const bmfp_request *req, uint64_t *id);
  * requests.  Synchronous; reports active systems and waiting requests. */
 int bmfp_server_step(bmfp_server s, int *n_active, int *n_queued);
 /* Pops one finished request (FIFO by completion): result and, if x is not
- * NULL, its solutions (nrhs * n doubles, right-hand side major). */
+ * NULL, its solutions (nrhs * n doubles, right-hand side major).  A finished
+ * request keeps its device memory until it is polled. */
 int bmfp_server_poll(bmfp_server s, bmfp_request_result *res, double *x,
                      size_t x_capacity, int *got);
 /* The result bmfp_server_poll would return next, without removing it. */
 int bmfp_server_peek(bmfp_server s, bmfp_request_result *res, int *got);
+/* The tile schedule in use (from the configuration or the autotuner run by
+ * bmfp_server_create): slot-kernel workgroups and chunks per tile, the
+ * latter indexed by block size (9 entries, 0 for unsupported sizes). */
+int bmfp_server_schedule(bmfp_server s, int *slot_groups, int *chunks_per_tile_by_bs);
 int bmfp_server_stats(bmfp_server s, size_t *used_bytes, size_t *budget_bytes,
                       int *active_systems, int *queued, double *busy_seconds);
 

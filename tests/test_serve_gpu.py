@@ -164,3 +164,52 @@ def test_invalid_requests_rejected():
         S.submit(S.bcsr_request([0, 1, 1], [7], np.ones(4), 2, 2))  # column out of range
     with pytest.raises(bm.BmfpError):
         S.submit(S.stencil_request(10, 10, 2, nrhs=0))
+
+
+def test_each_rhs_matches_solving_it_alone():
+    # right-hand sides of one request move in lockstep (finished ones frozen);
+    # each must come out bit-identical to a one-right-hand-side request
+    nx, ny, bs, nrhs = 21, 17, 4, 4
+    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, 1, seed=12)
+    n = nx * ny * bs
+    rng = np.random.default_rng(3)
+    B = rng.uniform(-1, 1, (nrhs, n))
+    B[2] *= 1e-3                       # very different scales: different iteration counts
+    B[3] = 0.0                         # the zero right-hand side gives x = 0 at once
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=16)
+    multi = S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=nrhs, b=B))
+    singles = [S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=1, b=B[k])) for k in range(nrhs)]
+    out = S.drain()
+    res, X = out[multi]
+    assert res.status == bm.OK
+    for k in range(nrhs):
+        rk, xk = out[singles[k]]
+        assert rk.status == bm.OK
+        assert np.array_equal(X[k], xk[0]), k
+    assert not X[3].any()
+    c = oracle.bcsr_to_csr(oracle.BCSR(nx * ny, nx * ny, bs, rp, cols, vals))
+    for k in range(3):
+        r = B[k] - oracle.csr_spmv(c, X[k])
+        assert np.linalg.norm(r) / np.linalg.norm(B[k]) < 1e-10
+
+
+@pytest.mark.parametrize("groups,chunks", [(256, 1), (4096, 8), (1024, 3)])
+def test_results_independent_of_tile_schedule(groups, chunks):
+    spec = [dict(nx=30, ny=11, bs=2, kind=0, seed=5, nrhs=3, rhs_seed=1),
+            dict(nx=13, ny=19, bs=6, kind=1, seed=6, nrhs=2, rhs_seed=2)]
+    ref = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    ids = [ref.submit(ref.stencil_request(**s)) for s in spec]
+    base = ref.drain()
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8, slot_groups=groups, chunks_per_tile=chunks)
+    assert S.schedule() == (groups, {b: chunks for b in (1, 2, 3, 4, 6, 8)})
+    ids2 = [S.submit(S.stencil_request(**s)) for s in spec]
+    out = S.drain()
+    for a, b in zip(ids, ids2):
+        assert np.array_equal(base[a][1], out[b][1])
+
+
+def test_autotuned_schedule_is_reported():
+    S = bm.Server(memory_budget_bytes=8 << 30, max_systems=8)
+    groups, per_bs = S.schedule()
+    assert groups in (512, 1024, 2048, 4096, 8192)
+    assert sorted(per_bs) == [1, 2, 3, 4, 6, 8] and set(per_bs.values()) <= {1, 2, 4, 8}

@@ -226,3 +226,21 @@ def test_fill_matches_host_rng():
     assert np.array_equal(d.download(), oracle.rng_sym_vec(5, 6, n))
     d.fill(2, value=2.5)
     assert np.all(d.download() == 2.5)
+
+
+@pytest.mark.parametrize("bs,kind,layout,dtype", [(1, 0, bm.ROW_MAJOR, bm.F64), (3, 1, bm.ROW_MAJOR, bm.F64),
+                                                  (4, 0, bm.COL_MAJOR, bm.F64), (6, 1, bm.ROW_MAJOR, bm.F32),
+                                                  (8, 0, bm.ROW_MAJOR, bm.F16)])
+def test_device_stencil_matches_host_generator(bs, kind, layout, dtype):
+    # bmfp_stencil_create generates on the device; the matrix must equal the
+    # one uploaded from the oracle's generator, bit for bit
+    nx, ny = 13, 7
+    D = bm.Matrix.stencil(nx, ny, bs, kind, seed=9, dtype=dtype, layout=layout)
+    m = oracle.gen_stencil9(nx, ny, bs, kind, 9)
+    H = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, bs, dtype=dtype, layout=bm.ROW_MAJOR)
+    if layout != bm.ROW_MAJOR:
+        H = H.convert(dtype, layout)
+    assert (D.n_rows, D.nnzb, D.bs, D.layout) == (H.n_rows, H.nnzb, H.bs, H.layout)
+    assert np.array_equal(D.download_values(), H.download_values())
+    x = oracle.rng_unit_vec(2, 4, m.n_cols * bs)
+    assert np.array_equal(bm.spmv_host(D, x), bm.spmv_host(H, x))
