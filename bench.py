@@ -300,7 +300,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
            "busy_GBs": round(moved / busy / 1e9, 1),
            "max_rel_residual": worst_res,
            "mean_outer": round(float(np.mean(outer)), 2), "mean_inner": round(float(np.mean(inner)), 1),
-           "schedule": {"slot_groups": schedule[0], "chunks_per_tile": schedule[1]}}
+           "schedule": {"slot_groups": schedule[0], "segments": schedule[1], "chunks_per_tile": schedule[2]}}
     if ref_x is not None:
         out["max_err_vs_fp64"] = max(float((np.linalg.norm(xs[j] - ref_x[j], axis=1) /
                                             np.linalg.norm(ref_x[j], axis=1)).max()) for j in xs)
@@ -378,7 +378,7 @@ def main():
     ap.add_argument("--no-solver", action="store_true",
                     help="skip the FP64 vs mixed-precision BiCGStab comparison")
     ap.add_argument("--solver-max-iter", type=int, default=20000)
-    ap.add_argument("--serve-requests", type=int, default=200,
+    ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving stream replayed (0 = skip the serving leg)")
     ap.add_argument("--serve-lambdas", default="200,1000,5000")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")

@@ -516,7 +516,7 @@ def _declare_server(L):
     L.bmfp_server_step.argtypes = [vp, ip, ip]
     L.bmfp_server_poll.argtypes = [vp, ctypes.POINTER(RequestResult), vp, ctypes.c_size_t, ip]
     L.bmfp_server_peek.argtypes = [vp, ctypes.POINTER(RequestResult), ip]
-    L.bmfp_server_schedule.argtypes = [vp, ip, ip]
+    L.bmfp_server_schedule.argtypes = [vp, ip, ip, ip]
     L.bmfp_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), ip, ip,
                                     ctypes.POINTER(ctypes.c_double)]
 
@@ -596,11 +596,11 @@ class Server:
         return res, (x.reshape(res.nrhs, res.n) if want_x else None)
 
     def schedule(self):
-        """(slot_groups, {bs: chunks per tile}) chosen by the configuration or the autotuner."""
-        g = ctypes.c_int(0)
+        """(slot_groups, segments, {bs: chunks per tile}) from the configuration or the autotuner."""
+        g, seg = ctypes.c_int(0), ctypes.c_int(0)
         per = (ctypes.c_int * 9)()
-        check(lib().bmfp_server_schedule(self.handle, ctypes.byref(g), per))
-        return g.value, {bs: per[bs] for bs in range(9) if per[bs]}
+        check(lib().bmfp_server_schedule(self.handle, ctypes.byref(g), ctypes.byref(seg), per))
+        return g.value, seg.value, {bs: per[bs] for bs in range(9) if per[bs]}
 
     def stats(self):
         used, cap = ctypes.c_size_t(0), ctypes.c_size_t(0)

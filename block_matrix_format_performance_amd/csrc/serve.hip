@@ -6,27 +6,31 @@
 // sides of one request move in lockstep so that every block row of its
 // matrix is read once per SpMV for all of them:
 //
-//   RESID   (slot 1)  x' = x + d, r = b - A64 x', r32 = r      fin1: DONE | INIT
-//   INIT    (slot 2)  d = 0, ri = rh = r32, p = v = 0           fin2: INNER_A | RESID
-//   INNER_A (slot 1)  p = ri + beta (p - omega v), v = A32 p    fin1: INNER_B | RESID
-//   INNER_B (slot 2)  s = ri - alpha v, t = A32 s               fin2: INNER_C
-//   INNER_C (slot 3)  ri = s - omega t, d += alpha p + omega s  fin3: INNER_A | RESID
+//   RESID   (slot 0)  x' = x + d, r = b - A64 x', r32 = r              fin0: DONE | INNER_A | RESID
+//   INNER_A (slot 1)  p = ri + beta (p - omega v), v = A32 p            fin1: INNER_B | RESID
+//   INNER_B (slot 2)  s = ri - alpha v, t = A32 s                       fin2: INNER_C
+//   INNER_C (slot 3)  ri = s - omega t, d += alpha p + omega s          fin3: INNER_A | RESID
 //
-// Per right-hand side k two bit masks select the work: `live` (not yet
-// converged or failed) for RESID and `act` (still inside the current inner
-// solve) for the rest.  A right-hand side whose inner solve stops is frozen
-// until the others stop too, so its arithmetic is exactly that of solving it
-// alone.
+// The inner solve starts from ri = r32 with the shadow residual r32 itself
+// (it stays constant until the next RESID), p = v = d = 0: iteration 0 reads
+// r32 in place of ri and skips the zero terms, so no initialisation pass is
+// needed.  With x = 0, the first RESID is r = b and reads no matrix.
 //
-// One batched iteration is seven kernels (RESID and INNER_A slot kernels,
-// fin1, slot2, fin2, slot3, fin3), captured iterations_per_round times in one
-// HIP graph.  Slot kernels give
-// every wavefront its own tiles from a tile table (no workgroup barriers); a
-// tile is `passes` consecutive chunks of one request, a chunk is 4 wavefront
-// passes of 64/bs block rows.  Partial sums are kept per chunk and the fin
-// kernels (one wavefront per request) reduce a request's chunks in chunk
-// order, so the arithmetic of a request depends neither on the rest of the
-// batch nor on the tile schedule: results are bitwise identical alone, in
+// Per right-hand side k three bit masks select the work: `live` (not yet
+// converged or failed) for RESID, `act` (still inside the current inner
+// solve) for the rest, and `dval` (d holds this refinement step's correction;
+// otherwise it counts as 0).  A right-hand side whose inner solve stops is
+// frozen until the others stop too, so its arithmetic is exactly that of
+// solving it alone.
+//
+// One batched iteration is eight kernels (slot0 fin0 slot1 fin1 slot2 fin2
+// slot3 fin3), captured iterations_per_round times in one HIP graph.  Slot
+// kernels give every wavefront its own tiles from a tile table (no workgroup
+// barriers); a tile is `passes` consecutive chunks of one request, a chunk is
+// 4 wavefront passes of 64/bs block rows.  Partial sums are kept per chunk and
+// the fin kernels (one wavefront per request) reduce a request's chunks in
+// chunk order, so the arithmetic of a request depends neither on the rest of
+// the batch nor on the tile schedule: results are bitwise identical alone, in
 // any mix, and under any schedule the autotuner picks.
 #include <hip/hip_runtime.h>
 
@@ -47,15 +51,7 @@ constexpr int SERVE_MAX_RHS = 4;
 constexpr int SERVE_PART = 2 * SERVE_MAX_RHS;  // partial sums per chunk
 constexpr int SERVE_CHUNK_PASSES = 4;          // wavefront passes per chunk
 
-enum : int {
-  REQ_EMPTY = 0,
-  REQ_RESID = 1,
-  REQ_INIT = 2,
-  REQ_INNER_A = 3,
-  REQ_INNER_B = 4,
-  REQ_INNER_C = 5,
-  REQ_DONE = 6
-};
+enum : int { REQ_EMPTY = 0, REQ_RESID = 1, REQ_INNER_A = 2, REQ_INNER_B = 3, REQ_INNER_C = 4, REQ_DONE = 5 };
 
 // Device view of one request.  Vectors interleave the nrhs right-hand sides:
 // element i of right-hand side k is at i*nrhs + k.
@@ -68,7 +64,7 @@ struct ReqDesc {
   const float *v32;
   const double *b;
   double *x[2];
-  float *r32, *d, *ri, *rh, *p[2], *v[2], *s, *t;
+  float *r32, *d, *ri, *p[2], *v[2], *s, *t;
   double tol, inner_tol;
   int max_outer, max_inner;
 };
@@ -83,8 +79,8 @@ struct RhsState {
 // (each set moves in lockstep, so one counter serves all its members).
 struct ReqState {
   int phase;
-  unsigned live, act;
-  int outer, inner_it, pad[3];
+  unsigned live, act, dval;
+  int outer, inner_it, pad[2];
   RhsState k[SERVE_MAX_RHS];
 };
 
@@ -96,21 +92,8 @@ __host__ __device__ constexpr int serve_chunk_brows(int bs) { return SERVE_CHUNK
 
 namespace {
 
-// Slot kernels: 0 = RESID, 1 = INNER_A, 2 = INIT | INNER_B, 3 = INNER_C.
-// Finalize kernels: 1 = RESID | INNER_A, 2 = INIT | INNER_B, 3 = INNER_C.
-template <int SLOT> __device__ __forceinline__ bool slot_owns(int phase)
-{
-  if constexpr (SLOT == 0) return phase == REQ_RESID;
-  else if constexpr (SLOT == 1) return phase == REQ_INNER_A;
-  else if constexpr (SLOT == 2) return phase == REQ_INIT || phase == REQ_INNER_B;
-  else return phase == REQ_INNER_C;
-}
-
-template <int SLOT> __device__ __forceinline__ bool fin_owns(int phase)
-{
-  if constexpr (SLOT == 1) return phase == REQ_RESID || phase == REQ_INNER_A;
-  else return slot_owns<SLOT>(phase);
-}
+// Slot and finalize kernel SLOT serve phase REQ_RESID + SLOT.
+template <int SLOT> __device__ __forceinline__ bool slot_owns(int phase) { return phase == REQ_RESID + SLOT; }
 
 using Acc = double[SERVE_MAX_RHS][2];
 
@@ -212,27 +195,71 @@ __device__ __forceinline__ double wave_total(double v)
   return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
 }
 
+// Lane c of each group of BS consecutive lanes, broadcast to the group: no
+// exchange for BS = 1, DPP quad permutations (VALU) for BS = 2 and 4, a
+// bit-mode swizzle for BS = 8, a general permute otherwise.
+template <int BS> __device__ __forceinline__ int group_bcast_i32(int v, int c, int base_lane)
+{
+  if constexpr (BS == 1) {
+    return v;
+  } else if constexpr (BS == 2) {
+    return c == 0 ? __builtin_amdgcn_mov_dpp(v, 0xA0, 0xF, 0xF, false) : __builtin_amdgcn_mov_dpp(v, 0xF5, 0xF, 0xF, false);
+  } else if constexpr (BS == 4) {
+    switch (c) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false);
+    }
+  } else if constexpr (BS == 8) {
+    switch (c) {  // lane (i & 0x18) | c within each 32-lane half
+    case 0: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (0 << 5));
+    case 1: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (1 << 5));
+    case 2: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (2 << 5));
+    case 3: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (3 << 5));
+    case 4: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (4 << 5));
+    case 5: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (5 << 5));
+    case 6: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (6 << 5));
+    default: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (7 << 5));
+    }
+  } else {
+    return __shfl(v, base_lane + c);
+  }
+}
+
+template <int BS> __device__ __forceinline__ float group_bcast(float v, int c, int base_lane)
+{
+  return __int_as_float(group_bcast_i32<BS>(__float_as_int(v), c, base_lane));
+}
+template <int BS> __device__ __forceinline__ double group_bcast(double v, int c, int base_lane)
+{
+  return __hiloint2double(group_bcast_i32<BS>(__double2hiint(v), c, base_lane),
+                          group_bcast_i32<BS>(__double2loint(v), c, base_lane));
+}
+
 // Multi-right-hand-side SpMV over one chunk.  Lane (lbr, r) owns scalar row
 // i = (br0 + pass*RPW + lbr)*BS + r; each block row is loaded once and applied
 // to the gathered operands of all NR right-hand sides (operand elements shared
 // across the block row's lanes by shuffles).  All lanes of a block row run
-// the same trip counts, so every shuffle reads an active lane.  Vectors
+// the same trip counts, so every exchange reads an active lane.  Vectors
 // interleave the right-hand sides: element i of k sits at i*NR + k.
+// Without use_matrix the product is 0 (no block is read).
 // gather(off, x[NR]) forms the operand at element offset off; own(off, m)
 // loads what the epilogue needs of the owned row before the block loop;
 // epi(off, y[NR], m) finishes the row.  Every right-hand side is computed;
 // the epilogues keep the values the state machine must not change.
 // Blocks whose loads are issued together: as many as keep the batch's
-// registers (block row values + G gathered vectors of NR operands) near 40.
+// registers (block row values + G gathered vectors of NR operands) within
+// about 40, up to a whole 9-block stencil row.
 template <class VT, class AT, int BS, int NR, int G> constexpr int spmv_batch()
 {
   constexpr int regs = BS * (int) sizeof(VT) / 4 + G * NR * (int) sizeof(AT) / 4;
-  return regs <= 13 ? 3 : (regs <= 20 ? 2 : 1);
+  return 40 / regs > 9 ? 9 : (40 / regs < 1 ? 1 : 40 / regs);
 }
 
 template <class VT, class AT, int BS, int NR, int G, class OWNT, class GATHER, class OWN, class EPI>
-__device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals, int br0, const GATHER &gather,
-                                           const OWN &own, const EPI &epi)
+__device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals, int br0, bool use_matrix,
+                                           const GATHER &gather, const OWN &own, const EPI &epi)
 {
   constexpr int RPW = 64 / BS;
   constexpr int U = spmv_batch<VT, AT, BS, NR, G>();
@@ -247,8 +274,10 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
     int b = 0, b1 = 0;
     OWNT mine;
     if (on) {
-      b = ld(rp, (uint32_t) br);
-      b1 = ld(rp, (uint32_t) br + 1u);
+      if (use_matrix) {
+        b = ld(rp, (uint32_t) br);
+        b1 = ld(rp, (uint32_t) br + 1u);
+      }
       own(off, mine);
     }
     AT acc[NR];
@@ -269,7 +298,7 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
 #pragma unroll
         for (int k = 0; k < NR; ++k)
 #pragma unroll
-          for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[u][c], __shfl(xe[u][k], base_lane + c), acc[k]);
+          for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[u][c], group_bcast<BS>(xe[u][k], c, base_lane), acc[k]);
     }
     for (; U > 1 && b < b1; ++b) {
       VT v[BS];
@@ -279,7 +308,7 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
 #pragma unroll
       for (int k = 0; k < NR; ++k)
 #pragma unroll
-        for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[c], __shfl(xe[k], base_lane + c), acc[k]);
+        for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[c], group_bcast<BS>(xe[k], c, base_lane), acc[k]);
     }
     if (on) epi(off, acc, mine);
   }
@@ -301,49 +330,63 @@ template <int NR> struct ResidOwn {
   double xr[NR], b[NR], old[NR];
 };
 
+// x' = x + d (d counted as 0 where dval is clear), r = b - A64 x', r32 = r;
+// partials ||r||^2 (FP64) and ||r32||^2.  With x = 0 (step 0) the matrix is
+// not read: r = b.
 template <int BS, int NR>
 __device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
   const int par = S.outer & 1;
-  const unsigned live = S.live;
+  const unsigned live = S.live, dval = S.dval;
+  const bool first = S.outer == 0;
   const gptr<const double> xs = glb((const double *) D.x[par]), b = glb(D.b);
   const gptr<double> xn = glb(D.x[1 - par]);
   const gptr<const float> d = glb((const float *) D.d);
   const gptr<float> r32 = glb(D.r32);
   auto gather = [&](uint32_t o, double (&x)[NR]) {
-    float dd[NR];
-    ldv<double, NR>(xs, o, x);
-    ldv<float, NR>(d, o, dd);
+    if (first) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) x[k] += (double) dd[k];
+      for (int k = 0; k < NR; ++k) x[k] = 0.0;
+      return;
+    }
+    ldv<double, NR>(xs, o, x);
+    if (dval) {
+      float dd[NR];
+      ldv<float, NR>(d, o, dd);
+#pragma unroll
+      for (int k = 0; k < NR; ++k)
+        if ((dval >> k) & 1u) x[k] += (double) dd[k];
+    }
   };
   auto own = [&](uint32_t o, ResidOwn<NR> &m) {
     gather(o, m.xr);
     ldv<double, NR>(b, o, m.b);
-    ldv<double, NR>((gptr<const double>) xn, o, m.old);
+    if (live != (1u << NR) - 1u) ldv<double, NR>((gptr<const double>) xn, o, m.old);
   };
-  chunk_spmv<double, double, BS, NR, 2, ResidOwn<NR>>(D, glb(D.v64), br0, gather, own,
-                                                   [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
-                                                     double xw[NR];
-                                                     float rw[NR];
+  chunk_spmv<double, double, BS, NR, 2, ResidOwn<NR>>(
+      D, glb(D.v64), br0, !first, gather, own, [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
+        double xw[NR];
+        float rw[NR];
 #pragma unroll
-                                                     for (int k = 0; k < NR; ++k) {
-                                                       // a finished right-hand side keeps its iterate
-                                                       xw[k] = ((live >> k) & 1u) ? m.xr[k] : m.old[k];
-                                                       const double ri = m.b[k] - y[k];
-                                                       rw[k] = (float) ri;
-                                                       acc[k][0] += ri * ri;
-                                                       acc[k][1] += m.b[k] * m.b[k];
-                                                     }
-                                                     stv<double, NR>(xn, o, xw);
-                                                     stv<float, NR>(r32, o, rw);
-                                                   });
+        for (int k = 0; k < NR; ++k) {
+          // a finished right-hand side keeps its iterate
+          xw[k] = ((live >> k) & 1u) ? m.xr[k] : m.old[k];
+          const double ri = m.b[k] - y[k];
+          rw[k] = (float) ri;
+          acc[k][0] += ri * ri;
+          acc[k][1] += (double) rw[k] * (double) rw[k];
+        }
+        stv<double, NR>(xn, o, xw);
+        stv<float, NR>(r32, o, rw);
+      });
 }
 
 template <int NR> struct AOwn {
   float p[NR], rh[NR];
 };
 
+// p = ri + beta (p_prev - omega v_prev) (p = r32 at iteration 0), v = A32 p;
+// partial (r32, v).
 template <int BS, int NR>
 __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
@@ -352,12 +395,16 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     beta[k] = (float) S.k[k].beta;
-    omega[k] = (float) (it == 0 ? 1.0 : S.k[k].omega);
+    omega[k] = (float) S.k[k].omega;
   }
   const gptr<const float> pp = glb((const float *) D.p[(it - 1) & 1]), vp = glb((const float *) D.v[(it - 1) & 1]);
   const gptr<float> pc = glb(D.p[it & 1]), vc = glb(D.v[it & 1]);
-  const gptr<const float> ri = glb((const float *) D.ri), rh = glb((const float *) D.rh);
+  const gptr<const float> ri = glb((const float *) D.ri), rh = glb((const float *) D.r32);
   auto gather = [&](uint32_t o, float (&x)[NR]) {
+    if (it == 0) {
+      ldv<float, NR>(rh, o, x);
+      return;
+    }
     float rr[NR], p0[NR], v0[NR];
     ldv<float, NR>(ri, o, rr);
     ldv<float, NR>(pp, o, p0);
@@ -367,28 +414,37 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
   };
   auto own = [&](uint32_t o, AOwn<NR> &m) {
     gather(o, m.p);
-    ldv<float, NR>(rh, o, m.rh);
-  };
-  chunk_spmv<float, float, BS, NR, 3, AOwn<NR>>(D, glb(D.v32), br0, gather, own,
-                                             [&](uint32_t o, const float (&y)[NR], const AOwn<NR> &m) {
-                                               stv<float, NR>(pc, o, m.p);
-                                               stv<float, NR>(vc, o, y);
+    if (it == 0) {
 #pragma unroll
-                                               for (int k = 0; k < NR; ++k) acc[k][0] += (double) m.rh[k] * (double) y[k];
-                                             });
+      for (int k = 0; k < NR; ++k) m.rh[k] = m.p[k];
+    } else {
+      ldv<float, NR>(rh, o, m.rh);
+    }
+  };
+  chunk_spmv<float, float, BS, NR, 3, AOwn<NR>>(D, glb(D.v32), br0, true, gather, own,
+                                                [&](uint32_t o, const float (&y)[NR], const AOwn<NR> &m) {
+                                                  stv<float, NR>(pc, o, m.p);
+                                                  stv<float, NR>(vc, o, y);
+#pragma unroll
+                                                  for (int k = 0; k < NR; ++k)
+                                                    acc[k][0] += (double) m.rh[k] * (double) y[k];
+                                                });
 }
 
 template <int NR> struct BOwn {
   float s[NR];
 };
 
+// s = ri - alpha v (ri = r32 at iteration 0), t = A32 s; partials (t, s), (t, t).
 template <int BS, int NR>
 __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
   float alpha[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) alpha[k] = (float) S.k[k].alpha;
-  const gptr<const float> vcur = glb((const float *) D.v[S.inner_it & 1]), ri = glb((const float *) D.ri);
+  const int it = S.inner_it;
+  const gptr<const float> vcur = glb((const float *) D.v[it & 1]);
+  const gptr<const float> ri = glb((const float *) (it == 0 ? D.r32 : D.ri));
   const gptr<float> s_out = glb(D.s), t_out = glb(D.t);
   auto gather = [&](uint32_t o, float (&x)[NR]) {
     float rr[NR], vv[NR];
@@ -397,69 +453,51 @@ __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &
 #pragma unroll
     for (int k = 0; k < NR; ++k) x[k] = s_update(rr[k], vv[k], alpha[k]);
   };
-  chunk_spmv<float, float, BS, NR, 2, BOwn<NR>>(D, glb(D.v32), br0, gather,
-                                             [&](uint32_t o, BOwn<NR> &m) { gather(o, m.s); },
-                                             [&](uint32_t o, const float (&y)[NR], const BOwn<NR> &m) {
-                                               stv<float, NR>(s_out, o, m.s);
-                                               stv<float, NR>(t_out, o, y);
+  chunk_spmv<float, float, BS, NR, 2, BOwn<NR>>(D, glb(D.v32), br0, true, gather,
+                                                [&](uint32_t o, BOwn<NR> &m) { gather(o, m.s); },
+                                                [&](uint32_t o, const float (&y)[NR], const BOwn<NR> &m) {
+                                                  stv<float, NR>(s_out, o, m.s);
+                                                  stv<float, NR>(t_out, o, y);
 #pragma unroll
-                                               for (int k = 0; k < NR; ++k) {
-                                                 acc[k][0] += (double) y[k] * (double) m.s[k];
-                                                 acc[k][1] += (double) y[k] * (double) y[k];
-                                               }
-                                             });
+                                                  for (int k = 0; k < NR; ++k) {
+                                                    acc[k][0] += (double) y[k] * (double) m.s[k];
+                                                    acc[k][1] += (double) y[k] * (double) y[k];
+                                                  }
+                                                });
 }
 
-template <int BS, int NR>
-__device__ __forceinline__ void init_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
-{
-  const gptr<const float> r32 = glb((const float *) D.r32);
-  const gptr<float> d = glb(D.d), ri = glb(D.ri), rh = glb(D.rh);
-  const gptr<float> p0 = glb(D.p[0]), p1 = glb(D.p[1]), v0 = glb(D.v[0]), v1 = glb(D.v[1]);
-  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
-    float r0[NR], z[NR];
-    ldv<float, NR>(r32, o, r0);
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      z[k] = 0.f;
-      const double q = (double) r0[k] * (double) r0[k];
-      acc[k][0] += q;
-      acc[k][1] += q;
-    }
-    stv<float, NR>(d, o, z);
-    stv<float, NR>(ri, o, r0);
-    stv<float, NR>(rh, o, r0);
-    stv<float, NR>(p0, o, z);
-    stv<float, NR>(p1, o, z);
-    stv<float, NR>(v0, o, z);
-    stv<float, NR>(v1, o, z);
-  });
-}
-
+// ri = s - omega t, d += alpha p + omega s (d = 0 before iteration 0; a live
+// right-hand side outside the inner solve gets d = 0 at iteration 0 and keeps
+// d afterwards); partials (r32, ri), (ri, ri).
 template <int BS, int NR>
 __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
   const unsigned act = S.act;
+  const int it = S.inner_it;
   float alpha[NR], omega[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     alpha[k] = (float) S.k[k].alpha;
     omega[k] = (float) S.k[k].omega;
   }
-  const gptr<const float> pcur = glb((const float *) D.p[S.inner_it & 1]), s_in = glb((const float *) D.s),
-                          t_in = glb((const float *) D.t), rh = glb((const float *) D.rh);
+  const gptr<const float> pcur = glb((const float *) D.p[it & 1]), s_in = glb((const float *) D.s),
+                          t_in = glb((const float *) D.t), rh = glb((const float *) D.r32);
   const gptr<float> d = glb(D.d), ri = glb(D.ri);
   chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
     float s[NR], p[NR], dd[NR], t[NR], h[NR], rn[NR];
     ldv<float, NR>(s_in, o, s);
     ldv<float, NR>(pcur, o, p);
-    ldv<float, NR>((gptr<const float>) d, o, dd);
+    if (it == 0) {
+#pragma unroll
+      for (int k = 0; k < NR; ++k) dd[k] = 0.f;
+    } else {
+      ldv<float, NR>((gptr<const float>) d, o, dd);
+    }
     ldv<float, NR>(t_in, o, t);
     ldv<float, NR>(rh, o, h);
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       rn[k] = fmadd(-omega[k], t[k], s[k]);
-      // a frozen right-hand side keeps its correction for the next refinement step
       if ((act >> k) & 1u) dd[k] = fmadd(omega[k], s[k], fmadd(alpha[k], p[k], dd[k]));
       acc[k][0] += (double) h[k] * (double) rn[k];
       acc[k][1] += (double) rn[k] * (double) rn[k];
@@ -503,8 +541,7 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
     } else if constexpr (SLOT == 1) {
       inner_a_chunk<BS, NR>(D, S, br0, acc);
     } else if constexpr (SLOT == 2) {
-      if (phase == REQ_INIT) init_chunk<BS, NR>(D, S, br0, acc);
-      else inner_b_chunk<BS, NR>(D, S, br0, acc);
+      inner_b_chunk<BS, NR>(D, S, br0, acc);
     } else {
       inner_c_chunk<BS, NR>(D, S, br0, acc);
     }
@@ -520,15 +557,21 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
 }
 
 // ---- slot kernels: every wavefront walks its own tiles -------------------------
+// With `segments` = 8 the tile table is cut into 8 contiguous ranges and the
+// workgroups of each XCD (workgroups are dispatched to the XCDs round-robin)
+// walk one range, so neighbouring chunks of a request -- which gather the
+// same operand rows -- share one L2.  With 1 all waves walk the whole table.
 template <int SLOT>
 __global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                              double *__restrict__ part)
+                                              double *__restrict__ part, int segments)
 {
   const int nt = *ntp;
-  const int wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
-  const int nw = (int) gridDim.x * 4;
-  for (int ti = wave; ti < nt; ti += nw) {
+  const int seg = (int) blockIdx.x % segments;
+  const int wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x / segments) * 4 + (int) (threadIdx.x >> 6));
+  const int nw = (int) (gridDim.x / segments) * 4;
+  const int t1 = (int) ((int64_t) nt * (seg + 1) / segments);
+  for (int ti = (int) ((int64_t) nt * seg / segments) + wave; ti < t1; ti += nw) {
     const TileDesc T = tiles[ti];
     const ReqState &S = rs[T.req];
     const int phase = S.phase;
@@ -549,7 +592,7 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
   const int q = list[w];
   ReqState &S = rs[q];
   const int phase = S.phase;
-  if (!fin_owns<SLOT>(phase)) return;
+  if (!slot_owns<SLOT>(phase)) return;
   const ReqDesc &D = rd[q];
   double a[SERVE_PART];
 #pragma unroll
@@ -571,13 +614,13 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
   const unsigned live = S.live, act = S.act;
   const int outer = S.outer, it = S.inner_it;
   const int k = lane;
-  bool keep = false;
+  bool keep = false, inner = false;
   if (k < D.nrhs) {
     RhsState &R = S.k[k];
-    if (phase == REQ_RESID) {
+    if (SLOT == 0) {
       if ((live >> k) & 1u) {
-        // a0 = ||r||^2, a1 = ||b||^2 for x' = x + d (now in x[1 - outer%2])
-        if (outer == 0) R.norm_b2 = a1;
+        // a0 = ||r||^2 (FP64), a1 = ||r32||^2 for x' = x + d (now in x[1 - outer%2])
+        if (outer == 0) R.norm_b2 = a0;  // x = 0: r = b
         R.outer = outer + 1;
         // b = 0 gives x = 0 exactly; a non-finite ||b|| or ||r|| gives a NaN rel
         const double rel = R.norm_b2 == 0.0 ? 0.0 : sqrt(a0) / sqrt(R.norm_b2);
@@ -586,18 +629,19 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
         else if (rel < D.tol) R.status = BMFP_OK;
         else if (R.outer > D.max_outer) R.status = BMFP_ERR_NOT_CONVERGED;
         else keep = true;
+        if (keep) {
+          // the inner solve starts: ri = rh = r32, rho_0 = (rh, ri)
+          R.inner_rr0 = a1;
+          R.inner_it = 0;
+          R.rho = a1;
+          R.alpha = 1.0;
+          R.omega = 1.0;
+          R.beta = a1;
+          inner = !(a1 == 0.0 || !isfinite(a1));
+        }
       }
     } else if ((act >> k) & 1u) {
-      if (phase == REQ_INIT) {
-        // a0 = (rh, ri) = rho_0, a1 = ||ri||^2
-        R.inner_rr0 = a1;
-        R.inner_it = 0;
-        R.rho = a0;
-        R.alpha = 1.0;
-        R.omega = 1.0;
-        R.beta = a0;
-        keep = !(a0 == 0.0 || !isfinite(a0));
-      } else if (phase == REQ_INNER_A) {
+      if (SLOT == 1) {
         // a0 = (rh, v); a breakdown ends the inner solve, d keeps the last iterate
         if (a0 == 0.0 || !isfinite(a0)) {
           R.inner_total += it;
@@ -605,7 +649,7 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
           R.alpha = R.rho / a0;
           keep = true;
         }
-      } else if (phase == REQ_INNER_B) {
+      } else if (SLOT == 2) {
         // a0 = (t, s), a1 = (t, t)
         R.tt = a1;
         R.omega = a1 == 0.0 ? 0.0 : a0 / a1;
@@ -627,22 +671,26 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
     }
   }
   const unsigned kept = (unsigned) (__ballot(keep) & 0xFull);
+  const unsigned inner_set = (unsigned) (__ballot(inner) & 0xFull);
   if (lane == 0) {
-    if (phase == REQ_RESID) {
+    if (SLOT == 0) {
       S.outer = outer + 1;
-      S.live = kept;
-      S.act = kept;
-      S.phase = kept ? REQ_INIT : REQ_DONE;
-    } else if (phase == REQ_INIT) {
       S.inner_it = 0;
+      S.live = kept;
+      S.act = inner_set;
+      S.dval = 0u;
+      S.phase = kept == 0u ? REQ_DONE : (inner_set ? REQ_INNER_A : REQ_RESID);
+    } else if (SLOT == 1) {
       S.act = kept;
-      S.phase = kept ? REQ_INNER_A : REQ_RESID;
-    } else if (phase == REQ_INNER_B) {
+      S.phase = kept ? REQ_INNER_B : REQ_RESID;
+    } else if (SLOT == 2) {
       S.phase = REQ_INNER_C;
     } else {
-      if (phase == REQ_INNER_C) S.inner_it = it + 1;
+      // d now holds this step's correction for every live right-hand side
+      S.inner_it = it + 1;
+      S.dval = live;
       S.act = kept;
-      S.phase = kept ? (phase == REQ_INNER_A ? REQ_INNER_B : REQ_INNER_A) : REQ_RESID;
+      S.phase = kept ? REQ_INNER_A : REQ_RESID;
     }
   }
 }
@@ -707,6 +755,7 @@ __global__ void k_req_init(ReqState *__restrict__ st, int nrhs, int phase)
   ReqState S{};
   S.phase = phase;
   S.live = S.act = (1u << nrhs) - 1u;
+  S.dval = 0u;
   S.inner_it = phase == REQ_RESID ? 0 : 1;
   for (int k = 0; k < SERVE_MAX_RHS; ++k) {
     S.k[k] = RhsState{};
