@@ -40,6 +40,7 @@ BCSR_GROUP_PER_ROW = 4
 BCSR_SHUFFLE_X = 5
 BCSR_BLOCK_ROW_GROUP = 6
 BCSR_LIBRARY = 7
+BCSR_JIT = 8
 
 BCSR_VARIANTS = (BCSR_THREAD_PER_ROW, BCSR_GROUP_PER_ROW, BCSR_SHUFFLE_X, BCSR_BLOCK_ROW_GROUP)
 PRECOMPILED_BS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
@@ -110,6 +111,7 @@ def _declare(L):
     L.bmfp_spmv.argtypes = [vp, c_int, vp, vp, vp]
     L.bmfp_spmv_timed.argtypes = [vp, c_int, vp, vp, c_int, c_int, vp, dp]
     L.bmfp_spmv_auto_variant.argtypes = [vp, ip]
+    L.bmfp_jit_build.argtypes = [c_int, c_int, c_int, ctypes.POINTER(sz), dp]
     _declare_solver(L)
 
 
@@ -395,6 +397,13 @@ class Matrix:
             self.destroy()
         except Exception:
             pass
+
+
+def jit_build(bs, dtype=F64, layout=ROW_MAJOR):
+    """Compiles (hipRTC, gfx950) the BCSR_JIT kernel for (bs, dtype, layout): (code bytes, seconds)."""
+    nb, sec = ctypes.c_size_t(0), ctypes.c_double(0)
+    check(lib().bmfp_jit_build(bs, dtype, layout, ctypes.byref(nb), ctypes.byref(sec)))
+    return nb.value, sec.value
 
 
 def spmv_host(A, x, variant=SPMV_AUTO):

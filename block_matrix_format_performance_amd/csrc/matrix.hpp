@@ -33,6 +33,8 @@ struct bmfp_matrix_t {
 
 namespace bmfp {
 int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
+// The hipRTC-specialised kernel for A's block size (F64 / F32).
+int launch_bcsr_jit(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
 int launch_csr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
 int launch_library(bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
 void destroy_library(bmfp_matrix_t &A);

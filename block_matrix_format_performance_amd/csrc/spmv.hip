@@ -161,7 +161,7 @@ bool precompiled_bs(int bs)
 bool bcsr_supported(int variant, int dtype, int layout, int bs)
 {
   if (bs < 1) return false;
-  if (variant == BMFP_BCSR_LIBRARY) return dtype != BMFP_F16;
+  if (variant == BMFP_BCSR_LIBRARY || variant == BMFP_BCSR_JIT) return dtype != BMFP_F16;
   if (!precompiled_bs(bs)) return variant == BMFP_BCSR_THREAD_PER_ROW;
   if (variant == BMFP_BCSR_BLOCK_ROW_GROUP) return layout == BMFP_COL_MAJOR;
   return variant == BMFP_BCSR_THREAD_PER_ROW || variant == BMFP_BCSR_GROUP_PER_ROW ||
@@ -175,7 +175,9 @@ int auto_variant(const bmfp_matrix_t &A)
   // matrices (FP64, ~4.4e8 stored values): batched thread-per-row for small
   // blocks; 16-byte lane slices (row-major) or shuffle-shared x segments
   // (column-major) from bs = 4 up.
-  if (!precompiled_bs(A.bs)) return BMFP_BCSR_THREAD_PER_ROW;
+  // block sizes without a precompiled instantiation: a kernel specialised by
+  // hipRTC (F64 / F32), the runtime-bs thread-per-row kernel for F16
+  if (!precompiled_bs(A.bs)) return A.dtype == BMFP_F16 ? BMFP_BCSR_THREAD_PER_ROW : BMFP_BCSR_JIT;
   if (A.bs <= 3) return BMFP_BCSR_THREAD_PER_ROW;
   return A.layout == BMFP_COL_MAJOR ? BMFP_BCSR_SHUFFLE_X : BMFP_BCSR_GROUP_PER_ROW;
 }
@@ -186,6 +188,7 @@ int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hip
   if (!bcsr_supported(variant, A.dtype, A.layout, A.bs))
     return fail(BMFP_ERR_UNSUPPORTED, "BCSR variant %d unsupported for dtype %d layout %d bs %d",
                 variant, A.dtype, A.layout, A.bs);
+  if (variant == BMFP_BCSR_JIT) return launch_bcsr_jit(A, x, y, s);
   switch (A.dtype) {
   case BMFP_F64: return launch_typed<double, double, false>(A, variant, x, y, s);
   case BMFP_F32: return launch_typed<float, float, false>(A, variant, x, y, s);

@@ -193,6 +193,29 @@ std::vector<measurement_class> gpu_bcsr_spmv(bcsr_matrix_class<T, I> &m, const T
   return gpu_bcsr_spmv<T, I>(m, transposed, nullptr, reference_y);
 }
 
+template <typename T, typename I>
+measurement_class gpu_bcsr_jit_spmv(bcsr_matrix_class<T, I> &m, const T *transposed, const T *x, const T *reference_y)
+{
+  Handle col;
+  bcsr_handle(col, m, transposed, BMFP_COL_MAJOR);
+  const size_t xn = (size_t) m.n_cols * m.bs, yn = (size_t) m.n_rows * m.bs;
+  DevBuf dx(xn * sizeof(T)), dy(yn * sizeof(T));
+  if (x)
+    check(bmfp_memcpy_h2d(dx.p, x, xn * sizeof(T)), "upload x");
+  else
+    check(bmfp_fill(dx.p, dtype_of<T>(), xn, 2, 0, 0, 1.0, nullptr), "fill x");
+  // compile (hipRTC) and load, untimed, then one timed launch
+  check(bmfp_spmv(col.m, BMFP_BCSR_JIT, dx.p, dy.p, nullptr), "jit compile");
+  double t = 0;
+  check(bmfp_spmv_timed(col.m, BMFP_BCSR_JIT, dx.p, dy.p, 0, 1, nullptr, &t), "jit");
+  std::vector<T> y(yn);
+  check(bmfp_memcpy_d2h(y.data(), dy.p, yn * sizeof(T)), "download y");
+  if (reference_y) compare_results(yn, reference_y, y.data());
+  double bytes = 0, flops = 0;
+  check(bmfp_spmv_traffic(col.m, &bytes, &flops), "traffic");
+  return measurement_class("jit", t, bytes, flops);
+}
+
 // Instantiations (reference: gpu_matrix_multiplier.cu:1333-1341).
 #define BMFP_INSTANTIATE(T, I)                                                                                  \
   template measurement_class gpu_csr_spmv<T, I>(const csr_matrix_class<T, I> &, const T *);                   \
@@ -202,7 +225,8 @@ std::vector<measurement_class> gpu_bcsr_spmv(bcsr_matrix_class<T, I> &m, const T
                                                        T *);                                                    \
   template std::vector<measurement_class> gpu_bcsr_spmv<T, I>(bcsr_matrix_class<T, I> &, const T *, const T *); \
   template std::vector<measurement_class> gpu_bcsr_spmv<T, I>(bcsr_matrix_class<T, I> &, const T *, const T *,  \
-                                                              const T *);
+                                                              const T *);                                        \
+  template measurement_class gpu_bcsr_jit_spmv<T, I>(bcsr_matrix_class<T, I> &, const T *, const T *, const T *);
 BMFP_INSTANTIATE(float, int)
 BMFP_INSTANTIATE(double, int)
 #undef BMFP_INSTANTIATE

@@ -58,7 +58,11 @@ enum {
                                  /*  shared across a block row by lane shuffles       */
   BMFP_BCSR_BLOCK_ROW_GROUP = 6, /* [..._column_by_column(_fastdiv,_po2,_template)]   */
                                  /*  :485-673, lane group per block row               */
-  BMFP_BCSR_LIBRARY = 7          /* [cusparse?bsrmv] :676-734 -> rocSPARSE bsrmv      */
+  BMFP_BCSR_LIBRARY = 7,         /* [cusparse?bsrmv] :676-734 -> rocSPARSE bsrmv      */
+  BMFP_BCSR_JIT = 8              /* [bcsr_jit, NVRTC] main.cpp:170-294 -> a kernel    */
+                                 /*  compiled by hipRTC for A's bs (F64 / F32, either */
+                                 /*  layout); the AUTO choice for bs outside the      */
+                                 /*  precompiled {1,2,3,4,6,8,12,16,24,32}            */
 };
 
 typedef struct bmfp_matrix_t *bmfp_matrix;
@@ -176,6 +180,11 @@ int bmfp_spmv(bmfp_matrix A, int variant, const void *x, void *y,
 int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y,
                     int warmup, int reps, bmfp_stream stream,
                     double *avg_seconds);
+/* Compiles with hipRTC (no device needed) the kernel BMFP_BCSR_JIT uses for
+ * (bs, dtype, layout) on gfx950; reports its code size and compile time.
+ * The SpMV entry points compile and cache the kernel themselves on first
+ * use; this call checks the toolchain and measures the compile cost. */
+int bmfp_jit_build(int bs, int dtype, int layout, size_t *code_bytes, double *seconds);
 /* The family BMFP_SPMV_AUTO resolves to for A. */
 int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant);
 

@@ -58,4 +58,13 @@ std::vector<measurement_class> gpu_bcsr_spmv(bcsr_matrix_class<data_type, index_
 // Measurement names of gpu_bcsr_spmv, in order.
 const std::vector<std::string> &gpu_bcsr_spmv_names();
 
+// "jit": the reference's run-time-specialised BCSR kernel (NVRTC, inline in
+// main.cpp:170-294) as the hipRTC-compiled kernel for the matrix's bs on the
+// column-major values; compiled first, then one timed launch, as there.
+// data_type float or double (the reference's is float only).
+template <typename data_type, typename index_type>
+measurement_class gpu_bcsr_jit_spmv(bcsr_matrix_class<data_type, index_type> &matrix,
+                                    const data_type *transpose_matrix_data, const data_type *x,
+                                    const data_type *reference_y);
+
 #endif

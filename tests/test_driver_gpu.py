@@ -1,0 +1,42 @@
+"""The drop-in benchmark driver (bmfp_main, the reference's main.cpp flow) on
+the GPU: every measurement row of the reference is present in result.json,
+including the run-time-specialised "jit" row, and no GPU result disagrees
+with the CPU CSR reference (compare_results prints "ERROR" on a mismatch)."""
+import json
+import os
+import subprocess
+
+import pytest
+
+import block_matrix_format_performance_amd as bm
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("typ,bs", [("float", "2,5"), ("double", "4,7")])
+def test_driver_rows_and_agreement(tmp_path, typ, bs):
+    print(f"\n[gpu-case] driver --type {typ} --bs {bs}", flush=True)
+    out = tmp_path / "result.json"
+    p = subprocess.run([bm.MAIN_PATH, "--type", typ, "--bs", bs, "--n-rows", "3000", "--bpr", "5", "--reps", "2",
+                        "--x", "random", "--out", str(out)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert "ERROR" not in p.stdout + p.stderr, p.stdout + p.stderr
+    res = json.loads(out.read_text())
+    assert sorted(res) == sorted(bs.split(","))
+    for b, rows in res.items():
+        for name in ["CPU CSR", "GPU CSR", "GPU CSR-Vector", "jit"] + bm_rows():
+            assert name in rows, (b, name)
+        assert rows["jit"] > 0
+
+
+def bm_rows():
+    return ["GPU BCSR (row major, thread per row)", "GPU BCSR (column major, thread per row, coal x)",
+            "GPU BSR (rocSPARSE, row major)"]
+
+
+def test_driver_solve():
+    print("\n[gpu-case] driver --solve", flush=True)
+    p = subprocess.run([bm.MAIN_PATH, "--bs", "2", "--n-rows", "500", "--reps", "1", "--out", os.devnull, "--solve"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "BiCGStab 256x256 bs=2" in p.stdout

@@ -85,3 +85,20 @@ def test_init_without_device_reports_no_device():
     if bm.device_count() > 0:
         pytest.skip("a device is present")
     assert bm.lib().bmfp_init(0) == bm.ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("bs,dtype,layout", [(5, bm.F64, bm.ROW_MAJOR), (7, bm.F32, bm.COL_MAJOR),
+                                             (48, bm.F64, bm.COL_MAJOR), (100, bm.F32, bm.ROW_MAJOR)])
+def test_jit_kernel_source_compiles_for_gfx950(bs, dtype, layout):
+    # hipRTC needs no device: the specialised kernel source must compile here
+    code, seconds = bm.jit_build(bs, dtype, layout)
+    assert code > 1000 and seconds > 0
+
+
+def test_jit_build_argument_errors():
+    with pytest.raises(bm.BmfpError) as e:
+        bm.jit_build(0)
+    assert e.value.status == bm.ERR_INVALID
+    with pytest.raises(bm.BmfpError) as e:
+        bm.jit_build(5, bm.F16)
+    assert e.value.status == bm.ERR_UNSUPPORTED

@@ -244,3 +244,31 @@ def test_device_stencil_matches_host_generator(bs, kind, layout, dtype):
     assert np.array_equal(D.download_values(), H.download_values())
     x = oracle.rng_unit_vec(2, 4, m.n_cols * bs)
     assert np.array_equal(bm.spmv_host(D, x), bm.spmv_host(H, x))
+
+
+@pytest.mark.parametrize("bs", [5, 7, 9, 10, 13, 20, 40, 70])
+@pytest.mark.parametrize("dtype", [bm.F64, bm.F32])
+def test_bcsr_jit_kernel(bs, dtype):
+    # hipRTC-specialised kernel (the AUTO choice for these block sizes)
+    m, x = stencil_case(5, 4, bs, kind=1, seed=bs)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        A, y = check_bcsr(m, x, layout, bm.BCSR_JIT, dtype=dtype)
+        assert A.auto_variant() == bm.BCSR_JIT
+        y2 = bm.spmv_host(A, x, bm.SPMV_AUTO)
+        assert np.array_equal(y, y2)               # cached module, same kernel
+
+
+@pytest.mark.parametrize("bs", [1, 4, 12])
+def test_bcsr_jit_on_precompiled_sizes_and_ragged_rows(bs):
+    m, x = ragged_case(61, 57, bs, max_len=30, seed=7 + bs)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        check_bcsr(m, x, layout, bm.BCSR_JIT)
+
+
+def test_bcsr_jit_unsupported_for_f16():
+    m, x = stencil_case(4, 4, 5, kind=0, seed=1)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 5, dtype=bm.F16)
+    assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW
+    with pytest.raises(bm.BmfpError) as e:
+        bm.spmv_host(A, x, bm.BCSR_JIT)
+    assert e.value.status == bm.ERR_UNSUPPORTED
