@@ -229,19 +229,22 @@ def plain_one_at_a_time(reqs, max_iter):
 
 
 def request_bytes_moved(r, outer, inner):
-    """Compulsory HBM bytes of one served request: every refinement step reads
-    the FP64 matrix once (x+d, b read, x', r32 written per right-hand side);
-    every inner iteration reads the FP32 matrix twice plus ~16 FP32 vector
-    accesses per right-hand side and row (the right-hand sides share the
-    matrix reads)."""
+    """Compulsory HBM bytes of one served request (right-hand sides share every
+    matrix read).  Refinement step 0 (x = 0): b read, x' and r32 written, no
+    matrix.  Later steps: the FP64 matrix once, x, d, b read and x', r32
+    written.  Inner iteration: the FP32 matrix twice and 68 bytes of FP32
+    vector traffic per right-hand side and row (v = A p: ri, p, v, r32 read,
+    p, v written; t = A s: ri, v read, s, t written; updates: s, p, d, t, r32
+    read, d, ri written)."""
     side, bs, nr = r["side"], r["bs"], r["nrhs"]
     nodes = side * side
     nnzb = (3 * side - 2) ** 2
     n = nodes * bs
     idx = 4.0 * nnzb + 4.0 * (nodes + 1)
+    first = nr * n * (8 + 8 + 4)
     resid = 8.0 * nnzb * bs * bs + idx + nr * n * (8 + 4 + 8 + 8 + 4)
-    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 4.0 * 16
-    return outer * resid + inner * inner_it
+    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 68.0
+    return (first if outer > 0 else 0.0) + max(outer - 1, 0) * resid + inner * inner_it
 
 
 def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):

@@ -481,7 +481,7 @@ extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int
   const int64_t rows = nodes * bs;
   k_gen_stencil_rows<<<(unsigned) ((rows + 255) / 256), 256, 0, nullptr>>>(nx, ny, bs, kind, seed,
                                                                           m->row_ptr->as<int>(),
-                                                                          m->columns->as<int>(), gen);
+                                                                          m->columns->as<int>(), gen, nullptr);
   BMFP_LAUNCHED();
   if (!direct) BMFP_TRY(convert_values(*m, gen, BMFP_ROW_MAJOR, nullptr));
   BMFP_HIP(hipDeviceSynchronize());

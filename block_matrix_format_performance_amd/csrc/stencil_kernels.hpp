@@ -34,18 +34,20 @@ inline void stencil_row_ptr(int nx, int ny, std::vector<int> &rp)
 namespace {
 
 // One thread per scalar row t = node*bs + r: writes row r of each of the
-// node's blocks (row-major blocks) and, for r == 0, the block columns.
+// node's blocks (row-major blocks) and, for r == 0, the block columns; with
+// vals32 also the FP32 rounding of every value.  Grids up to 2^31 scalar rows
+// (32-bit index arithmetic).
 __global__ __launch_bounds__(256) void k_gen_stencil_rows(int nx, int ny, int bs, int kind, uint64_t seed,
                                                           const int *__restrict__ rp, int *__restrict__ cols,
-                                                          double *__restrict__ vals)
+                                                          double *__restrict__ vals, float *__restrict__ vals32)
 {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
-  const int64_t n = (int64_t) nx * ny * bs;
-  if (t >= n) return;
-  const int64_t i = t / bs;
-  const int r = (int) (t - i * bs);
-  const int iy = (int) (i / nx), ix = (int) (i - (int64_t) iy * nx);
+  const int64_t t64 = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (t64 >= (int64_t) nx * ny * bs) return;
+  const int t = (int) t64;
+  const int i = t / bs;
+  const int r = t - i * bs;
+  const int iy = i / nx, ix = i - iy * nx;
   const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
   const int64_t bs2 = (int64_t) bs * bs;
   int64_t nb = rp[i];
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows(int nx, int ny, int bs
     if (jy < 0 || jy >= ny || jx < 0 || jx >= nx) continue;
     if (r == 0) cols[nb] = jy * nx + jx;
     const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) bs2;
-    double *row = vals + nb * bs2 + (int64_t) r * bs;
+    const int64_t at = nb * bs2 + (int64_t) r * bs;
     if (s == 4) diag = nb;
     for (int c = 0; c < bs; c++) {
       double v;
@@ -72,14 +74,20 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows(int nx, int ny, int bs
         else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * bs + r);
         v = -(k + 0.3 * cc);
       }
-      row[c] = v;
-      if (!(s == 4 && c == r)) sum += fabs(v);
+      if (!(s == 4 && c == r)) {
+        sum += fabs(v);
+        vals[at + c] = v;
+        if (vals32) vals32[at + c] = (float) v;
+      } else if (kind == 0) {
+        vals[at + c] = v;  // the random part of the diagonal entry; the shift is added below
+      }
     }
     nb++;
   }
-  double *dg = vals + diag * bs2 + (int64_t) r * bs + r;
-  if (kind == 0) *dg = *dg + (sum + 1.0);
-  else *dg = 1.02 * sum + 1.0;
+  const int64_t dg = diag * bs2 + (int64_t) r * bs + r;
+  const double dv = kind == 0 ? vals[dg] + (sum + 1.0) : 1.02 * sum + 1.0;
+  vals[dg] = dv;
+  if (vals32) vals32[dg] = (float) dv;
 }
 
 }  // namespace
