@@ -213,3 +213,22 @@ def test_autotuned_schedule_is_reported():
     groups, segments, per_bs = S.schedule()
     assert groups in (2048, 4096, 8192, 16384) and segments in (1, 8)
     assert sorted(per_bs) == [1, 2, 3, 4, 6, 8] and set(per_bs.values()) <= {1, 2, 4}
+
+
+def test_nonfinite_rhs_fails_alone_inside_a_request():
+    # one NaN right-hand side ends at the first residual with ERR_NONFINITE; the
+    # request's other right-hand sides come out bit-identical to solving them alone
+    nx, ny, bs = 16, 13, 3
+    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, 0, seed=2)
+    n = nx * ny * bs
+    B = np.random.default_rng(5).uniform(-1, 1, (3, n))
+    B[1, 17] = np.nan
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    multi = S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=3, b=B))
+    singles = {k: S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=1, b=B[k])) for k in (0, 2)}
+    out = S.drain()
+    res, X = out[multi]
+    assert res.status == bm.ERR_NONFINITE
+    for k, rid in singles.items():
+        assert out[rid][0].status == bm.OK
+        assert np.array_equal(X[k], out[rid][1][0]), k
