@@ -284,10 +284,53 @@ struct GroupEngine {
   }
 };
 
+// Lane c of each group of BS consecutive lanes, broadcast to the group: no
+// exchange for BS = 1, DPP quad permutations (VALU) for BS = 2 and 4, a
+// bit-mode swizzle for BS = 8, a general permute otherwise.
+template <int BS> __device__ __forceinline__ int group_bcast_i32(int v, int c, int base_lane)
+{
+  if constexpr (BS == 1) {
+    return v;
+  } else if constexpr (BS == 2) {
+    return c == 0 ? __builtin_amdgcn_mov_dpp(v, 0xA0, 0xF, 0xF, false) : __builtin_amdgcn_mov_dpp(v, 0xF5, 0xF, 0xF, false);
+  } else if constexpr (BS == 4) {
+    switch (c) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false);
+    }
+  } else if constexpr (BS == 8) {
+    switch (c) {  // lane (i & 0x18) | c within each 32-lane half
+    case 0: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (0 << 5));
+    case 1: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (1 << 5));
+    case 2: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (2 << 5));
+    case 3: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (3 << 5));
+    case 4: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (4 << 5));
+    case 5: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (5 << 5));
+    case 6: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (6 << 5));
+    default: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (7 << 5));
+    }
+  } else {
+    return __shfl(v, base_lane + c);
+  }
+}
+
+template <int BS> __device__ __forceinline__ float group_bcast(float v, int c, int base_lane)
+{
+  return __int_as_float(group_bcast_i32<BS>(__float_as_int(v), c, base_lane));
+}
+template <int BS> __device__ __forceinline__ double group_bcast(double v, int c, int base_lane)
+{
+  return __hiloint2double(group_bcast_i32<BS>(__double2hiint(v), c, base_lane),
+                          group_bcast_i32<BS>(__double2loint(v), c, base_lane));
+}
+
 // ShflEngine: one lane per scalar row, 64/BS block rows per wavefront.  For
 // each block, lane r forms only operand element r of the block's x segment
 // (the operands here cost 2-4 loads each) and reads the other BS-1 from its
-// block-row neighbours by shuffles; loads of U blocks are issued together.
+// block-row neighbours by group broadcasts; loads of U blocks are issued
+// together.
 template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
 struct ShflEngine {
   static constexpr int U = BS <= 2 ? 9 : BS <= 8 ? 3 : 1;
@@ -325,14 +368,14 @@ struct ShflEngine {
 #pragma unroll
         for (int k = 0; k < U; ++k)
 #pragma unroll
-          for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[k][c]), __shfl(xe[k], base_lane + c), acc);
+          for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[k][c]), group_bcast<BS>(xe[k], c, base_lane), acc);
       }
       for (; b < b1; ++b) {
         VT v[BS];
         load_block_row<VT, BS, LAYOUT>(M.vals + (int64_t) b * (BS * BS), r, v);
         const AT xe = op((int64_t) M.cols[b] * BS + r);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[c]), __shfl(xe, base_lane + c), acc);
+        for (int c = 0; c < BS; ++c) acc = fmadd(cvt<AT>(v[c]), group_bcast<BS>(xe, c, base_lane), acc);
       }
       if (active) {
         if constexpr (SCALED) acc *= (AT) M.scale[br];
