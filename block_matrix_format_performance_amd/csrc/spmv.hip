@@ -79,10 +79,17 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
     break;
   }
   case BMFP_BCSR_GROUP_PER_ROW: {
-    constexpr int VEC = gpr_vec<VT>(BS);
-    constexpr int G = pow2ceil(BS / VEC);
-    Grid g = make_grid(nb * BS * G);
-    k_bcsr_gpr<VT, AT, BS, VEC, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    constexpr DenseShape D = dense_shape<VT, BS>();
+    if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
+      constexpr int RPW = 64 / (BS / D.vec);
+      Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
+      k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    } else {
+      constexpr int VEC = gpr_vec<VT>(BS);
+      constexpr int G = pow2ceil(BS / VEC);
+      Grid g = make_grid(nb * BS * G);
+      k_bcsr_gpr<VT, AT, BS, VEC, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    }
     break;
   }
   case BMFP_BCSR_SHUFFLE_X: {

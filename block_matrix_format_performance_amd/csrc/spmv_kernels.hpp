@@ -243,6 +243,91 @@ __global__ __launch_bounds__(256) void k_bcsr_gpr(BcsrArgs<VT, AT> a)
 }
 
 // ============================================================================
+// Dense lane group per scalar row (row-major blocks): G = BS/VEC lanes, so
+// every lane of a group works (64/G rows per wavefront; the 64 mod G spare
+// lanes idle).  Lane g owns columns [g*VEC, g*VEC+VEC) of every block of its
+// row; loads of U blocks are issued together; partial sums are combined by a
+// fixed shuffle tree over the group.  Shapes per block size (FP64) from
+// timing on the HBM-resident stencils: dense_shape().
+// ============================================================================
+struct DenseShape {
+  int vec, u;
+};
+template <class VT, int BS> constexpr DenseShape dense_shape()
+{
+  if constexpr (sizeof(VT) != 8) return {0, 0};
+  switch (BS) {
+  case 6: return {2, 3};
+  case 8: return {2, 3};
+  case 12: return {4, 1};
+  case 16: return {4, 2};
+  case 24: return {4, 1};
+  case 32: return {4, 2};
+  default: return {0, 0};
+  }
+}
+
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
+{
+  constexpr int G = BS / VEC;
+  constexpr int RPW = 64 / G;
+  static_assert(BS % VEC == 0 && G <= 64, "bad group shape");
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int lr = lane / G, g = lane - lr * G;
+  const int64_t row = wave * RPW + lr;
+  const int64_t rows = (int64_t) a.n_brows * BS;
+  const bool on = lane < RPW * G && row < rows;
+  const int64_t br = on ? row / BS : 0;
+  const int r = (int) (row - br * BS);
+  int b = 0, b1 = 0;
+  if (on) {
+    b = a.rp[br];
+    b1 = a.rp[br + 1];
+  }
+  constexpr int ALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
+  constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * BS), lowbit((int) sizeof(AT) * VEC));
+  AT acc = 0;
+  for (; b + U <= b1; b += U) {
+    int col[U];
+    VT v[U][VEC];
+    AT xv[U][VEC];
+#pragma unroll
+    for (int k = 0; k < U; ++k) col[k] = a.cols[b + k];
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN>(a.vals + ((int64_t) (b + k) * BS + r) * BS + g * VEC, v[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_n<AT, VEC, XALIGN>(a.x + (int64_t) col[k] * BS + g * VEC, xv[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[k][c]), xv[k][c], acc);
+  }
+  for (; U > 1 && b < b1; ++b) {
+    VT v[VEC];
+    AT xv[VEC];
+    load_n<VT, VEC, ALIGN>(a.vals + ((int64_t) b * BS + r) * BS + g * VEC, v);
+    load_n<AT, VEC, XALIGN>(a.x + (int64_t) a.cols[b] * BS + g * VEC, xv);
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+  }
+  // group sum: lane g += lane g + s for s = 1, 2, 4, ... (all lanes take part;
+  // the spare lanes past the last whole group read only each other)
+  const int end = lr * G + G < 64 ? lr * G + G : 64;
+#pragma unroll
+  for (int s2 = 1; s2 < G; s2 *= 2) {
+    const int src = lane + s2 < end ? lane + s2 : lane;
+    const AT o = __shfl(acc, src);
+    if (g % (2 * s2) == 0 && g + s2 < G) acc += o;
+  }
+  if (on && g == 0) {
+    if constexpr (SCALED) acc *= (AT) a.scale[br];
+    a.y[row] = acc;
+  }
+}
+
+// ============================================================================
 // Thread per scalar row with the x segment shared by shuffles: the BS lanes
 // of a block row each load one element of x[col*BS .. col*BS+BS) (one
 // coalesced load instead of BS broadcast loads per lane) and read the others
