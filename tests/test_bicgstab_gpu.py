@@ -51,7 +51,7 @@ def test_config0_like_b_is_A_times_ones(jacobi):
 @pytest.mark.parametrize("bs", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32])
 @pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
 def test_convective_system_all_block_sizes(bs, layout):
-    # config-2 type matrix (-(K+0.3C) off-diagonal blocks), x* uniform[-1,1]
+    # convective block stencil (-(K+0.3C) off-diagonal blocks), x* uniform[-1,1]
     side = max(3, 48 // bs)
     m = oracle.gen_stencil9(side, side - 1, bs, 1, 0)
     c = oracle.bcsr_to_csr(m)

@@ -102,3 +102,35 @@ def test_jit_build_argument_errors():
     with pytest.raises(bm.BmfpError) as e:
         bm.jit_build(5, bm.F16)
     assert e.value.status == bm.ERR_UNSUPPORTED
+
+
+def test_stencil_create_argument_errors():
+    # rejected before any device call
+    for args in [(0, 4, 2, 0), (4, 4, 0, 0), (4, 4, 2, 3)]:
+        with pytest.raises(bm.BmfpError) as e:
+            bm.Matrix.stencil(*args)
+        assert e.value.status == bm.ERR_INVALID
+    with pytest.raises(bm.BmfpError) as e:
+        bm.Matrix.stencil(4, 4, 2, dtype=7)
+    assert e.value.status == bm.ERR_INVALID
+
+
+@pytest.mark.parametrize("bad", [dict(max_systems=0), dict(iterations_per_round=0), dict(slot_groups=-1),
+                                 dict(chunks_per_tile=-2), dict(inner_tol=0.0), dict(max_outer=0),
+                                 dict(max_inner=0)])
+def test_server_config_validation(bad):
+    with pytest.raises(bm.BmfpError) as e:
+        bm.Server(**bad)
+    assert e.value.status == bm.ERR_INVALID
+
+
+def test_server_request_footprint():
+    # FP64 + FP32 matrix, 3 FP64 and 9 FP32 vectors over all right-hand sides, 256-B aligned pieces
+    def a256(b):
+        return (b + 255) // 256 * 256
+    for nx, ny, bs, nrhs in [(16, 16, 2, 1), (100, 37, 6, 3), (512, 512, 8, 4)]:
+        nb, nnzb = nx * ny, (3 * nx - 2) * (3 * ny - 2)
+        n, nv = nb * bs, nnzb * bs * bs
+        want = (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + a256(4 * nv) + 3 * a256(8 * n * nrhs)
+                + 9 * a256(4 * n * nrhs))
+        assert bm.Server.request_bytes(bm.Server.stencil_request(nx, ny, bs, nrhs=nrhs)) == want
