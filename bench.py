@@ -303,7 +303,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
            "busy_GBs": round(moved / busy / 1e9, 1),
            "max_rel_residual": worst_res,
            "mean_outer": round(float(np.mean(outer)), 2), "mean_inner": round(float(np.mean(inner)), 1),
-           "schedule": {"slot_groups": schedule[0], "segments": schedule[1], "chunks_per_tile": schedule[2]}}
+           "schedule": {"slot_groups": schedule[0], "xcd_run": schedule[1], "chunks_per_tile": schedule[2]}}
     if ref_x is not None:
         out["max_err_vs_fp64"] = max(float((np.linalg.norm(xs[j] - ref_x[j], axis=1) /
                                             np.linalg.norm(ref_x[j], axis=1)).max()) for j in xs)

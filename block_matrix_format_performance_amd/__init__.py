@@ -605,7 +605,7 @@ class Server:
         return res, (x.reshape(res.nrhs, res.n) if want_x else None)
 
     def schedule(self):
-        """(slot_groups, segments, {bs: chunks per tile}) from the configuration or the autotuner."""
+        """(slot_groups, xcd_run, {bs: chunks per tile}) from the configuration or the autotuner."""
         g, seg = ctypes.c_int(0), ctypes.c_int(0)
         per = (ctypes.c_int * 9)()
         check(lib().bmfp_server_schedule(self.handle, ctypes.byref(g), ctypes.byref(seg), per))

@@ -515,21 +515,37 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
 }
 
 // ---- slot kernels: every wavefront walks its own tiles -------------------------
-// With `segments` = 8 the tile table is cut into 8 contiguous ranges and the
-// workgroups of each XCD (workgroups are dispatched to the XCDs round-robin)
-// walk one range, so neighbouring chunks of a request -- which gather the
-// same operand rows -- share one L2.  With 1 all waves walk the whole table.
+// With xcd_run = K > 0 the tile table is dealt to the 8 XCDs in runs of K
+// consecutive tiles (run j to XCD j mod 8; workgroups are dispatched to the
+// XCDs round-robin): neighbouring chunks of a request, which gather the same
+// operand rows, then share one L2, while every XCD still sees tiles from the
+// whole table (and so a mix of phases).  With 0 every wavefront walks the
+// table round-robin.  The grid is a multiple of 8 workgroups.
 template <int SLOT>
 __global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                              double *__restrict__ part, int segments)
+                                              double *__restrict__ part, int xcd_run)
 {
   const int nt = *ntp;
-  const int seg = (int) blockIdx.x % segments;
-  const int wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x / segments) * 4 + (int) (threadIdx.x >> 6));
-  const int nw = (int) (gridDim.x / segments) * 4;
-  const int t1 = (int) ((int64_t) nt * (seg + 1) / segments);
-  for (int ti = (int) ((int64_t) nt * seg / segments) + wave; ti < t1; ti += nw) {
+  if (xcd_run > 0) {
+    const int xcd = (int) blockIdx.x & 7;
+    const int lw = __builtin_amdgcn_readfirstlane((int) (blockIdx.x >> 3) * 4 + (int) (threadIdx.x >> 6));
+    const int nwx = (int) (gridDim.x >> 3) * 4;
+    for (int j = lw;; j += nwx) {
+      const int64_t ti64 = ((int64_t) (j / xcd_run) * 8 + xcd) * xcd_run + j % xcd_run;
+      if (ti64 >= nt) break;
+      const TileDesc T = tiles[ti64];
+      const ReqState &S = rs[T.req];
+      const int phase = S.phase;
+      if (!slot_owns<SLOT>(phase)) continue;
+      const ReqDesc &D = rd[T.req];
+      BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+    }
+    return;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane((int) blockIdx.x * 4 + (int) (threadIdx.x >> 6));
+  const int nw = (int) gridDim.x * 4;
+  for (int ti = wave; ti < nt; ti += nw) {
     const TileDesc T = tiles[ti];
     const ReqState &S = rs[T.req];
     const int phase = S.phase;

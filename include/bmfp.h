@@ -309,10 +309,10 @@ int bmfp_server_poll(bmfp_server s, bmfp_request_result *res, double *x,
 /* The result bmfp_server_poll would return next, without removing it. */
 int bmfp_server_peek(bmfp_server s, bmfp_request_result *res, int *got);
 /* The tile schedule in use (from the configuration or the autotuner run by
- * bmfp_server_create): slot-kernel workgroups, tile-table segments (8: one
- * contiguous range per XCD, 1: one shared range) and chunks per tile, the
- * latter indexed by block size (9 entries, 0 for unsupported sizes). */
-int bmfp_server_schedule(bmfp_server s, int *slot_groups, int *segments,
+ * bmfp_server_create): slot-kernel workgroups, tiles per run dealt to one XCD
+ * (0: round-robin over all wavefronts) and chunks per tile, the latter
+ * indexed by block size (9 entries, 0 for unsupported sizes). */
+int bmfp_server_schedule(bmfp_server s, int *slot_groups, int *xcd_run,
                          int *chunks_per_tile_by_bs);
 int bmfp_server_stats(bmfp_server s, size_t *used_bytes, size_t *budget_bytes,
                       int *active_systems, int *queued, double *busy_seconds);

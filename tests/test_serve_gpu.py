@@ -201,7 +201,7 @@ def test_results_independent_of_tile_schedule(groups, chunks):
     ids = [ref.submit(ref.stencil_request(**s)) for s in spec]
     base = ref.drain()
     S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8, slot_groups=groups, chunks_per_tile=chunks)
-    assert S.schedule() == (groups, 8, {b: chunks for b in (1, 2, 3, 4, 6, 8)})
+    assert S.schedule() == (groups, 0, {b: chunks for b in (1, 2, 3, 4, 6, 8)})
     ids2 = [S.submit(S.stencil_request(**s)) for s in spec]
     out = S.drain()
     for a, b in zip(ids, ids2):
@@ -210,8 +210,8 @@ def test_results_independent_of_tile_schedule(groups, chunks):
 
 def test_autotuned_schedule_is_reported():
     S = bm.Server(memory_budget_bytes=8 << 30, max_systems=8)
-    groups, segments, per_bs = S.schedule()
-    assert groups in (2048, 4096, 8192, 16384) and segments in (1, 8)
+    groups, xcd_run, per_bs = S.schedule()
+    assert groups in (2048, 4096, 8192, 16384) and xcd_run in (0, 8, 32, 128)
     assert sorted(per_bs) == [1, 2, 3, 4, 6, 8] and set(per_bs.values()) <= {1, 2, 4}
 
 
