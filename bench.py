@@ -200,8 +200,10 @@ def serving_stream(n_sample, seed=2):
 def plain_one_at_a_time(reqs, max_iter):
     """Each request's right-hand sides solved one after another by the plain
     FP64 BiCGStab (x0 = 0, 1e-10).  Timed: the solve calls only (matrix
-    generation, solver set-up and its first graph build are excluded)."""
-    total, per_req, xs, iters = 0.0, [], [], []
+    generation, solver set-up and its first graph build are excluded).
+    Compulsory bytes per iteration of the fused solver: two FP64 SpMVs (matrix,
+    x read, y written) and 15 further FP64 vector passes."""
+    total, per_req, xs, iters, moved = 0.0, [], [], [], 0.0
     for r in reqs:
         A = bm.Matrix.stencil(r["side"], r["side"], r["bs"], 0, seed=r["seed"])
         n = A.y_len
@@ -219,13 +221,15 @@ def plain_one_at_a_time(reqs, max_iter):
             if info.status != bm.OK:
                 raise RuntimeError(f"plain FP64 BiCGStab failed on {r}: status {info.status}")
             iters.append(info.iterations)
+            moved += info.iterations * (2.0 * (8.0 * A.nnzb * r["bs"] ** 2 + 4.0 * A.nnzb + 4.0 * (A.n_rows + 1)
+                                               + 16.0 * n) + 15 * 8.0 * n)
             xr.append(x.download())
         total += t_req
         per_req.append(t_req)
         xs.append(np.stack(xr))
         S.destroy()
         A.destroy()
-    return total, np.array(per_req), xs, iters
+    return total, np.array(per_req), xs, iters, moved
 
 
 def request_bytes_moved(r, outer, inner):
@@ -301,6 +305,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
            "max_ms": round(float(lat.max()) * 1e3, 3),
            "gpu_busy_frac": round(busy / span, 3),
            "busy_GBs": round(moved / busy / 1e9, 1),
+           "compulsory_GB": round(moved / 1e9, 2),
            "max_rel_residual": worst_res,
            "mean_outer": round(float(np.mean(outer)), 2), "mean_inner": round(float(np.mean(inner)), 1),
            "schedule": {"slot_groups": schedule[0], "xcd_run": schedule[1], "chunks_per_tile": schedule[2]}}
@@ -316,11 +321,11 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
         f"{sum(r['nrhs'] for r in reqs)} right-hand sides")
     t0 = time.perf_counter()
     if plain:
-        plain_s, plain_per, ref_x, plain_it = plain_one_at_a_time(reqs, max_iter)
+        plain_s, plain_per, ref_x, plain_it, plain_bytes = plain_one_at_a_time(reqs, max_iter)
         log(f"[bench] plain FP64 one at a time: {plain_s:.3f} s of solves "
             f"({n_sample / plain_s:.1f} solves/s), wall {time.perf_counter() - t0:.1f} s")
     else:
-        plain_s, plain_per, ref_x, plain_it = float("nan"), np.array([np.nan]), None, [np.nan]
+        plain_s, plain_per, ref_x, plain_it, plain_bytes = float("nan"), np.array([np.nan]), None, [np.nan], 0.0
     cfg = {"memory_budget_bytes": int(200e9)}
     cfg.update(cfg_overrides or {})
     runs = []
@@ -330,9 +335,13 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
         # bitwise those of the timed runs (independent of mix and schedule)
         acc = serve_replay(reqs, arrivals, float("inf"), cfg, ref_x)
         accuracy = {"max_err_vs_fp64": acc["max_err_vs_fp64"], "max_rel_residual": acc["max_rel_residual"],
-                    "reference": "plain FP64 GPU BiCGStab solutions (1e-10)"}
+                    "reference": "plain FP64 GPU BiCGStab solutions (1e-10)",
+                    # speedup the served loop would reach moving its compulsory bytes at the
+                    # plain solver's rate (outer/inner counts are the same in every replay)
+                    "compulsory_bytes_ratio_plain_over_served": round(plain_bytes / (acc["compulsory_GB"] * 1e9), 3)}
         log(f"[bench] served solutions vs plain FP64: max rel err {acc['max_err_vs_fp64']:.2e}, "
-            f"max rel residual {acc['max_rel_residual']:.2e}")
+            f"max rel residual {acc['max_rel_residual']:.2e}; compulsory bytes plain / served "
+            f"{accuracy['compulsory_bytes_ratio_plain_over_served']:.2f} (plain at {plain_bytes / plain_s / 1e9:.0f} GB/s)")
     for k, lam in enumerate(lambdas):
         r = serve_replay(reqs, arrivals, lam, cfg)
         r["speedup_vs_plain"] = round(r["solves_per_s"] / (n_sample / plain_s), 3) if plain else None
@@ -345,6 +354,7 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
                     f"9-point stencils; served by FP64 refinement around FP32 BiCGStab",
         "config": {k: v for k, v in cfg.items()},
         "plain_fp64": {"solves_per_s": round(n_sample / plain_s, 2), "solve_seconds": round(plain_s, 4),
+                       "compulsory_GB": round(plain_bytes / 1e9, 2), "GBs": round(plain_bytes / plain_s / 1e9, 1),
                        "p50_ms": round(float(np.percentile(plain_per, 50)) * 1e3, 3),
                        "p99_ms": round(float(np.percentile(plain_per, 99)) * 1e3, 3),
                        "mean_iterations": round(float(np.mean(plain_it)), 1),

@@ -605,7 +605,10 @@ class Server:
         return res, (x.reshape(res.nrhs, res.n) if want_x else None)
 
     def schedule(self):
-        """(slot_groups, xcd_run, {bs: chunks per tile}) from the configuration or the autotuner."""
+        """(slot_groups, xcd_run, {bs: chunks per tile}) from the configuration or the autotuner.
+
+        slot_groups > 0 is a persistent grid of that many workgroups; -1 means one
+        wavefront per tile, the grid following the tile table each round."""
         g, seg = ctypes.c_int(0), ctypes.c_int(0)
         per = (ctypes.c_int * 9)()
         check(lib().bmfp_server_schedule(self.handle, ctypes.byref(g), ctypes.byref(seg), per))

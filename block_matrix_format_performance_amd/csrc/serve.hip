@@ -514,45 +514,59 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
   }
 }
 
-// ---- slot kernels: every wavefront walks its own tiles -------------------------
-// With xcd_run = K > 0 the tile table is dealt to the 8 XCDs in runs of K
+// ---- slot kernels ----------------------------------------------------------------
+// Two launch shapes, results identical (partial sums are per chunk):
+//  * persistent = 0: one wavefront per tile, the grid sized to the tile table
+//    of the round (the hardware overlaps starting and finishing wavefronts);
+//  * persistent = 1: a fixed grid whose wavefronts each walk the table.
+// With xcd_run = K > 0 the table is dealt to the 8 XCDs in runs of K
 // consecutive tiles (run j to XCD j mod 8; workgroups are dispatched to the
-// XCDs round-robin): neighbouring chunks of a request, which gather the same
-// operand rows, then share one L2, while every XCD still sees tiles from the
-// whole table (and so a mix of phases).  With 0 every wavefront walks the
-// table round-robin.  The grid is a multiple of 8 workgroups.
+// XCDs round-robin), so neighbouring chunks of a request, which gather the
+// same operand rows, share one L2 while every XCD still sees tiles from the
+// whole table.  With 0 the tiles go to wavefronts in table order.  Grids are
+// multiples of 8 workgroups when K > 0.
+template <int SLOT>
+__device__ __forceinline__ void slot_tile(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
+                                          const TileDesc *__restrict__ tiles, double *__restrict__ part, int ti)
+{
+  const TileDesc T = tiles[ti];
+  const ReqState &S = rs[T.req];
+  const int phase = S.phase;
+  if (!slot_owns<SLOT>(phase)) return;
+  const ReqDesc &D = rd[T.req];
+  BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+}
+
 template <int SLOT>
 __global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                              double *__restrict__ part, int xcd_run)
+                                              double *__restrict__ part, int xcd_run, int persistent)
 {
   const int nt = *ntp;
+  const int wib = (int) (threadIdx.x >> 6);
+  if (!persistent) {
+    int64_t ti = (int64_t) blockIdx.x * 4 + wib;
+    if (xcd_run > 0) {
+      const int64_t l = (int64_t) (blockIdx.x >> 3) * 4 + wib;
+      ti = (l / xcd_run * 8 + (blockIdx.x & 7)) * xcd_run + l % xcd_run;
+    }
+    if (ti < nt) slot_tile<SLOT>(rd, rs, tiles, part, __builtin_amdgcn_readfirstlane((int) ti));
+    return;
+  }
   if (xcd_run > 0) {
     const int xcd = (int) blockIdx.x & 7;
-    const int lw = __builtin_amdgcn_readfirstlane((int) (blockIdx.x >> 3) * 4 + (int) (threadIdx.x >> 6));
+    const int lw = __builtin_amdgcn_readfirstlane((int) (blockIdx.x >> 3) * 4 + wib);
     const int nwx = (int) (gridDim.x >> 3) * 4;
     for (int j = lw;; j += nwx) {
       const int64_t ti64 = ((int64_t) (j / xcd_run) * 8 + xcd) * xcd_run + j % xcd_run;
       if (ti64 >= nt) break;
-      const TileDesc T = tiles[ti64];
-      const ReqState &S = rs[T.req];
-      const int phase = S.phase;
-      if (!slot_owns<SLOT>(phase)) continue;
-      const ReqDesc &D = rd[T.req];
-      BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+      slot_tile<SLOT>(rd, rs, tiles, part, (int) ti64);
     }
     return;
   }
-  const int wave = __builtin_amdgcn_readfirstlane((int) blockIdx.x * 4 + (int) (threadIdx.x >> 6));
+  const int wave = __builtin_amdgcn_readfirstlane((int) blockIdx.x * 4 + wib);
   const int nw = (int) gridDim.x * 4;
-  for (int ti = wave; ti < nt; ti += nw) {
-    const TileDesc T = tiles[ti];
-    const ReqState &S = rs[T.req];
-    const int phase = S.phase;
-    if (!slot_owns<SLOT>(phase)) continue;
-    const ReqDesc &D = rd[T.req];
-    BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
-  }
+  for (int ti = wave; ti < nt; ti += nw) slot_tile<SLOT>(rd, rs, tiles, part, ti);
 }
 
 // ---- finalize: one wavefront per request, lane k owns right-hand side k --------

@@ -245,7 +245,7 @@ int bmfp_refine_destroy(bmfp_solver s);
  * ones compute, then join the batch: each request is solved by FP64
  * iterative refinement around FP32 BiCGStab, its right-hand sides in lockstep
  * (one read of the matrix per SpMV for all of them), all requests advancing
- * together through batched persistent kernels replayed from a HIP graph
+ * together through batched kernels replayed from a HIP graph
  * (iterations_per_round batched iterations per bmfp_server_step).  A
  * request's arithmetic is independent of the batch it shares and of the tile
  * schedule: results are bitwise identical alone and in any mix.  Block sizes
@@ -256,7 +256,9 @@ typedef struct {
   size_t memory_budget_bytes; /* device bytes for admitted requests (capped at 90% free) */
   int max_systems;            /* resident requests (active + finished, not yet polled) */
   int iterations_per_round;   /* batched iterations per graph replay                   */
-  int slot_groups;            /* persistent workgroups of the slot kernels; 0: tuned   */
+  int slot_groups;            /* slot-kernel grid: > 0 persistent workgroups walking the
+                                 tile table; -1 one wavefront per tile (grid sized to
+                                 the tables each round); 0: tuned                     */
   double inner_tol;           /* FP32 BiCGStab relative tolerance per refinement step  */
   int max_outer;              /* refinement steps per right-hand side                  */
   int max_inner;              /* inner iterations per refinement step                  */
@@ -309,9 +311,10 @@ int bmfp_server_poll(bmfp_server s, bmfp_request_result *res, double *x,
 /* The result bmfp_server_poll would return next, without removing it. */
 int bmfp_server_peek(bmfp_server s, bmfp_request_result *res, int *got);
 /* The tile schedule in use (from the configuration or the autotuner run by
- * bmfp_server_create): slot-kernel workgroups, tiles per run dealt to one XCD
- * (0: round-robin over all wavefronts) and chunks per tile, the latter
- * indexed by block size (9 entries, 0 for unsupported sizes). */
+ * bmfp_server_create): slot-kernel grid (as slot_groups in the configuration,
+ * -1 = one wavefront per tile), tiles per run dealt to one XCD (0: table
+ * order) and chunks per tile, the latter indexed by block size (9 entries, 0
+ * for unsupported sizes). */
 int bmfp_server_schedule(bmfp_server s, int *slot_groups, int *xcd_run,
                          int *chunks_per_tile_by_bs);
 int bmfp_server_stats(bmfp_server s, size_t *used_bytes, size_t *budget_bytes,

@@ -115,7 +115,7 @@ def test_stencil_create_argument_errors():
     assert e.value.status == bm.ERR_INVALID
 
 
-@pytest.mark.parametrize("bad", [dict(max_systems=0), dict(iterations_per_round=0), dict(slot_groups=-1),
+@pytest.mark.parametrize("bad", [dict(max_systems=0), dict(iterations_per_round=0), dict(slot_groups=-2),
                                  dict(chunks_per_tile=-2), dict(inner_tol=0.0), dict(max_outer=0),
                                  dict(max_inner=0)])
 def test_server_config_validation(bad):
