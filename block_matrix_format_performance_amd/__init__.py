@@ -41,6 +41,7 @@ BCSR_SHUFFLE_X = 5
 BCSR_BLOCK_ROW_GROUP = 6
 BCSR_LIBRARY = 7
 BCSR_JIT = 8
+CSR_MERGE = 9
 
 BCSR_VARIANTS = (BCSR_THREAD_PER_ROW, BCSR_GROUP_PER_ROW, BCSR_SHUFFLE_X, BCSR_BLOCK_ROW_GROUP)
 PRECOMPILED_BS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)

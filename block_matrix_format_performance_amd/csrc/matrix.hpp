@@ -20,6 +20,7 @@ struct bmfp_matrix_t {
   std::shared_ptr<bmfp::DeviceBuffer> columns;  // int[nnzb]
   bmfp::DeviceBuffer values;                    // dtype[nnzb*bs*bs]
   bmfp::DeviceBuffer scale;                     // float[n_rows] (BMFP_F16 only)
+  bmfp::DeviceBuffer merge_carry;               // per-wavefront carries of BMFP_CSR_MERGE (CSR only)
 
   // rocSPARSE state for BMFP_BCSR_LIBRARY, created on first use.
   void *lib_handle = nullptr;
@@ -32,6 +33,15 @@ struct bmfp_matrix_t {
 };
 
 namespace bmfp {
+// Merge-path CSR SpMV: wavefront tiles of 512 path items (row ends +
+// nonzeros) and the scratch they need: per wavefront a carry row and value
+// (8-byte slots each), then the n_waves + 1 tile boundaries (int).
+constexpr int64_t MERGE_TILE_ITEMS = 512;
+inline int64_t merge_waves(int64_t n_rows, int64_t nnz) { return (n_rows + nnz + MERGE_TILE_ITEMS - 1) / MERGE_TILE_ITEMS; }
+inline size_t merge_carry_bytes(int64_t n_rows, int64_t nnz)
+{
+  return (size_t) merge_waves(n_rows, nnz) * 16 + (size_t) (merge_waves(n_rows, nnz) + 1) * 4;
+}
 int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
 // The hipRTC-specialised kernel for A's block size (F64 / F32).
 int launch_bcsr_jit(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);

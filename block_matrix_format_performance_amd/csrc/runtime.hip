@@ -427,6 +427,7 @@ extern "C" int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_co
   m->csr_group = csr_group_for(m->mean_row_blocks);
   BMFP_HIP(hipGetDevice(&m->device));
   BMFP_TRY(upload_indices(*m, row_ptr, columns));
+  BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(n_rows, nnz)));
   if (dtype == BMFP_F64) {
     BMFP_TRY(m->values.alloc(sizeof(double) * (size_t) nnz));
     if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, values, sizeof(double) * (size_t) nnz, hipMemcpyHostToDevice));
@@ -578,13 +579,12 @@ static int spmv_dispatch(bmfp_matrix A, int variant, const void *x, void *y, hip
 {
   if (variant == BMFP_SPMV_AUTO) variant = auto_variant(*A);
   if (variant == BMFP_BCSR_LIBRARY) return launch_library(*A, x, y, s);
+  const bool csr_family = variant == BMFP_CSR_SCALAR || variant == BMFP_CSR_VECTOR || variant == BMFP_CSR_MERGE;
   if (A->is_bcsr) {
-    if (variant == BMFP_CSR_SCALAR || variant == BMFP_CSR_VECTOR)
-      return fail(BMFP_ERR_INVALID, "CSR variant on a BCSR matrix");
+    if (csr_family) return fail(BMFP_ERR_INVALID, "CSR variant on a BCSR matrix");
     return launch_bcsr(*A, variant, x, y, s);
   }
-  if (variant != BMFP_CSR_SCALAR && variant != BMFP_CSR_VECTOR)
-    return fail(BMFP_ERR_INVALID, "BCSR variant on a CSR matrix");
+  if (!csr_family) return fail(BMFP_ERR_INVALID, "BCSR variant on a CSR matrix");
   return launch_csr(*A, variant, x, y, s);
 }
 

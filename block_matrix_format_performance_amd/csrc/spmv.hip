@@ -6,6 +6,8 @@
 #include "matrix.hpp"
 #include "spmv_kernels.hpp"
 
+static_assert(bmfp::MERGE_TILE == bmfp::MERGE_TILE_ITEMS, "merge tile and carry sizing disagree");
+
 namespace bmfp {
 
 namespace {
@@ -177,7 +179,16 @@ bool bcsr_supported(int variant, int dtype, int layout, int bs)
 
 int auto_variant(const bmfp_matrix_t &A)
 {
-  if (!A.is_bcsr) return A.mean_row_blocks >= 4.0 ? BMFP_CSR_VECTOR : BMFP_CSR_SCALAR;
+  if (!A.is_bcsr) {
+    // measured on ~1.4e8-nonzero matrices: thread per row for short rows,
+    // lane groups for long rows, merge-path tiles for ragged and mid-length
+    // rows and whenever one row dwarfs the mean (a lane group would serialise
+    // it)
+    const double mean = A.mean_row_blocks;
+    if (A.max_row_blocks > 8.0 * mean + 64.0) return BMFP_CSR_MERGE;
+    if (mean < 12.0) return BMFP_CSR_SCALAR;
+    return mean >= 128.0 ? BMFP_CSR_VECTOR : BMFP_CSR_MERGE;
+  }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
   // matrices (FP64, ~4.4e8 stored values): batched thread-per-row for small
   // blocks; 16-byte lane slices (row-major) or shuffle-shared x segments
@@ -239,6 +250,19 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
     case 32: k_csr_vector<VT, AT, 32><<<g.groups, 256, 0, s>>>(a); break;
     default: k_csr_vector<VT, AT, 64><<<g.groups, 256, 0, s>>>(a); break;
     }
+  } else if (variant == BMFP_CSR_MERGE) {
+    const int64_t waves = merge_waves(A.n_rows, A.nnzb);
+    if (A.merge_carry.bytes < merge_carry_bytes(A.n_rows, A.nnzb))
+      return fail(BMFP_ERR_INVALID, "matrix has no merge carry buffer");
+    int *carry_row = A.merge_carry.as<int>();
+    AT *carry_val = reinterpret_cast<AT *>(A.merge_carry.as<char>() + (size_t) waves * 8);
+    int *bounds = reinterpret_cast<int *>(A.merge_carry.as<char>() + (size_t) waves * 16);
+    k_csr_merge_bounds<<<(unsigned) ((waves + 1 + 255) / 256), 256, 0, s>>>(A.n_rows, A.nnzb, waves, a.rp, bounds);
+    BMFP_LAUNCHED();
+    k_csr_merge<VT, AT><<<(unsigned) ((waves + 3) / 4), 256, 0, s>>>(a, A.nnzb, waves, bounds, carry_row, carry_val);
+    BMFP_LAUNCHED();
+    k_csr_merge_fixup<AT><<<(unsigned) ((waves + 255) / 256), 256, 0, s>>>(waves, A.n_rows, carry_row, carry_val,
+                                                                          a.y);
   } else {
     return fail(BMFP_ERR_INVALID, "variant %d is not a CSR kernel family", variant);
   }

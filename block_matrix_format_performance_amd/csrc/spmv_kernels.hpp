@@ -17,6 +17,9 @@
 //                (column-major streaming)    bs in {3,5,6,9,10}; here: exact)
 //   k_csr_scalar thread per row              K2 :65-86
 //   k_csr_vector lane group per row          K3 :165-196
+//   k_csr_merge  merge-path tiles            (no kernel in the tree; the
+//                                             "GPU Merge SpMV" rows of
+//                                             results/bs_range.json)
 #pragma once
 
 #include "common.hpp"
@@ -487,6 +490,128 @@ __global__ __launch_bounds__(256) void k_csr_vector(CsrArgs<VT, AT> a)
 #pragma unroll
   for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
   if (g == 0) a.y[row] = acc;
+}
+
+// ============================================================================
+// Merge-path CSR SpMV: load-balanced for any row-length distribution.
+// The merge of the row ends (row i ends before nonzero j iff rp[i+1] <= j)
+// with the nonzeros 0..nnz-1 is a path of n + nnz items, cut into tiles of
+// MERGE_IPT items per lane, 64 lanes per wavefront.  k_csr_merge_bounds
+// finds where every tile starts (one thread per tile boundary, a binary
+// search over row_ptr); each wavefront of k_csr_merge stages its tile's
+// products vals[j] * x[cols[j]] and row ends in LDS with coalesced loads, and
+// each lane walks its MERGE_IPT items from there.  A row that crosses lanes
+// is summed by a segmented scan over the lanes (keyed by the row each lane
+// ends in); a row that crosses wavefronts leaves a per-wavefront carry that
+// k_csr_merge_fixup adds in wavefront order.  Every row is written once by
+// the lane that reaches its end, so y needs no clearing, and the order of
+// additions is fixed: results are bitwise reproducible.
+// ============================================================================
+constexpr int MERGE_IPT = 8;
+constexpr int MERGE_TILE = 64 * MERGE_IPT;
+
+// Rows consumed at merge-path diagonal d: the first x in [max(0, d-nnz),
+// min(d, n)] whose row end comes after nonzero d-x-1.
+__device__ __forceinline__ int merge_search(int64_t d, int n, int nnz, const int *__restrict__ rp)
+{
+  int64_t lo = d - nnz > 0 ? d - nnz : 0, hi = d < n ? d : n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t) rp[mid + 1] <= d - mid - 1) lo = mid + 1;
+    else hi = mid;
+  }
+  return (int) lo;
+}
+
+// Row coordinate of every tile boundary t = 0..n_waves (diagonal t * MERGE_TILE,
+// clamped to the end of the path).
+static __global__ __launch_bounds__(256) void k_csr_merge_bounds(int n, int nnz, int64_t n_waves, const int *__restrict__ rp,
+                                                          int *__restrict__ bounds)
+{
+  const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (t > n_waves) return;
+  const int64_t total = (int64_t) n + nnz;
+  bounds[t] = merge_search(t * MERGE_TILE < total ? t * MERGE_TILE : total, n, nnz, rp);
+}
+
+template <class VT, class AT>
+__global__ __launch_bounds__(256) void k_csr_merge(CsrArgs<VT, AT> a, int nnz, int64_t n_waves,
+                                                   const int *__restrict__ bounds, int *__restrict__ carry_row,
+                                                   AT *__restrict__ carry_val)
+{
+  __shared__ AT prod_s[4][MERGE_TILE];
+  __shared__ int ends_s[4][MERGE_TILE + 1];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int n = a.n_rows;
+  const int64_t total = (int64_t) n + nnz;
+  const int64_t w = (int64_t) blockIdx.x * 4 + wib;
+  // wavefronts past the path take part in the barrier with an empty tile
+  const int64_t d0 = w * MERGE_TILE < total ? w * MERGE_TILE : total;
+  const int64_t d1 = d0 + MERGE_TILE < total ? d0 + MERGE_TILE : total;
+  const int i0 = w < n_waves ? bounds[w] : n, i1 = w < n_waves ? bounds[w + 1] : n;
+  const int j0 = (int) (d0 - i0), j1 = (int) (d1 - i1);
+  for (int k = lane; k < j1 - j0; k += 64) prod_s[wib][k] = cvt<AT>(a.vals[j0 + k]) * a.x[a.cols[j0 + k]];
+  for (int k = lane; k <= i1 - i0; k += 64) ends_s[wib][k] = i0 + k < n ? a.rp[i0 + k + 1] : nnz;
+  __syncthreads();
+  if (w >= n_waves) return;
+  // this lane's items [dl, de) start at (i, j): search the staged row ends
+  const int64_t dl = d0 + lane * MERGE_IPT < d1 ? d0 + lane * MERGE_IPT : d1;
+  const int64_t de = dl + MERGE_IPT < d1 ? dl + MERGE_IPT : d1;
+  int lo = (int) (dl - j1 > i0 ? dl - j1 : i0), hi = (int) (dl - j0 < i1 ? dl - j0 : i1);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t) ends_s[wib][mid - i0] <= dl - mid - 1) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo, j = (int) (dl - lo);
+  AT acc = 0, first_part = 0;
+  int first_row = -1;
+  for (int64_t t = dl; t < de; ++t) {
+    if (j < ends_s[wib][i - i0]) {
+      acc += prod_s[wib][j - j0];
+      ++j;
+    } else {
+      if (first_row < 0) {
+        first_row = i;
+        first_part = acc;
+      } else {
+        a.y[i] = acc;  // the whole row lies in this lane's items
+      }
+      ++i;
+      acc = 0;
+    }
+  }
+  // segmented inclusive scan of the lanes' open-row sums, keyed by the row
+  // each lane ends in (keys ascend with the lane, so equal keys are adjacent)
+  AT sum = acc;
+#pragma unroll
+  for (int off = 1; off < 64; off *= 2) {
+    const AT o = __shfl_up(sum, off);
+    const int oi = __shfl_up(i, off);
+    if (lane >= off && oi == i) sum += o;
+  }
+  const AT prev_sum = __shfl_up(sum, 1);
+  const int prev_row = __shfl_up(i, 1);
+  if (first_row >= 0) a.y[first_row] = (lane > 0 && prev_row == first_row ? prev_sum : (AT) 0) + first_part;
+  if (lane == 63) {
+    carry_row[w] = i;
+    carry_val[w] = sum;
+  }
+}
+
+// Adds the wavefront carries: each run of wavefronts ending in the same row
+// is summed in wavefront order by the thread of its first wavefront.
+template <class AT>
+__global__ __launch_bounds__(256) void k_csr_merge_fixup(int64_t n_waves, int n, const int *__restrict__ carry_row,
+                                                         const AT *__restrict__ carry_val, AT *__restrict__ y)
+{
+  const int64_t w = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (w >= n_waves) return;
+  const int r = carry_row[w];
+  if (r >= n || (w > 0 && carry_row[w - 1] == r)) return;
+  AT s = 0;
+  for (int64_t k = w; k < n_waves && carry_row[k] == r; ++k) s += carry_val[k];
+  y[r] += s;
 }
 
 }  // namespace bmfp

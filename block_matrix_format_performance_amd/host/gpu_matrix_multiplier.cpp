@@ -173,6 +173,15 @@ measurement_class gpu_csr_vector_spmv(const csr_matrix_class<T, I> &m, const T *
 }
 
 template <typename T, typename I>
+measurement_class gpu_csr_merge_spmv(const csr_matrix_class<T, I> &m, const T *x, const T *reference_y, T *y_out)
+{
+  Handle h;
+  csr_handle(h, m);
+  return time_variant<T>(h.m, BMFP_CSR_MERGE, "GPU Merge SpMV", x, (size_t) m.n_cols, (size_t) m.n_rows,
+                         reference_y, y_out);
+}
+
+template <typename T, typename I>
 std::vector<measurement_class> gpu_bcsr_spmv(bcsr_matrix_class<T, I> &m, const T *transposed, const T *x,
                                              const T *reference_y)
 {
@@ -223,6 +232,7 @@ measurement_class gpu_bcsr_jit_spmv(bcsr_matrix_class<T, I> &m, const T *transpo
   template measurement_class gpu_csr_vector_spmv<T, I>(const csr_matrix_class<T, I> &, const T *);            \
   template measurement_class gpu_csr_vector_spmv<T, I>(const csr_matrix_class<T, I> &, const T *, const T *,  \
                                                        T *);                                                    \
+  template measurement_class gpu_csr_merge_spmv<T, I>(const csr_matrix_class<T, I> &, const T *, const T *, T *); \
   template std::vector<measurement_class> gpu_bcsr_spmv<T, I>(bcsr_matrix_class<T, I> &, const T *, const T *); \
   template std::vector<measurement_class> gpu_bcsr_spmv<T, I>(bcsr_matrix_class<T, I> &, const T *, const T *,  \
                                                               const T *);                                        \

@@ -129,6 +129,8 @@ std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matri
       [&] { return gpu_csr_spmv<T, I>(matrix, xp, reference.data(), nullptr); }));
   printer.print_time(measure_multiple_times(
       [&] { return gpu_csr_vector_spmv<T, I>(matrix, xp, reference.data(), nullptr); }));
+  printer.print_time(measure_multiple_times(
+      [&] { return gpu_csr_merge_spmv<T, I>(matrix, xp, reference.data(), nullptr); }));
 
   std::vector<T> transposed((size_t) block_matrix.size());
   block_matrix.transpose_blocks(transposed.data());

@@ -59,10 +59,15 @@ enum {
   BMFP_BCSR_BLOCK_ROW_GROUP = 6, /* [..._column_by_column(_fastdiv,_po2,_template)]   */
                                  /*  :485-673, lane group per block row               */
   BMFP_BCSR_LIBRARY = 7,         /* [cusparse?bsrmv] :676-734 -> rocSPARSE bsrmv      */
-  BMFP_BCSR_JIT = 8              /* [bcsr_jit, NVRTC] main.cpp:170-294 -> a kernel    */
+  BMFP_BCSR_JIT = 8,             /* [bcsr_jit, NVRTC] main.cpp:170-294 -> a kernel    */
                                  /*  compiled by hipRTC for A's bs (F64 / F32, either */
                                  /*  layout); the AUTO choice for bs outside the      */
                                  /*  precompiled {1,2,3,4,6,8,12,16,24,32}            */
+  BMFP_CSR_MERGE = 9             /* ["GPU Merge SpMV" rows of results/bs_range.json;  */
+                                 /*  no kernel in the tree] merge-path tiles: equal   */
+                                 /*  work per wavefront for any row lengths.  Uses    */
+                                 /*  the matrix's carry buffer: merge SpMVs on one    */
+                                 /*  handle must not overlap in time                  */
 };
 
 typedef struct bmfp_matrix_t *bmfp_matrix;

@@ -55,6 +55,13 @@ std::vector<measurement_class> gpu_bcsr_spmv(bcsr_matrix_class<data_type, index_
                                              const data_type *transpose_matrix_data, const data_type *x,
                                              const data_type *reference_y);
 
+// "GPU Merge SpMV": merge-path tiles, equal work per wavefront whatever the
+// row lengths (the name of the rows in the reference's results/bs_range.json,
+// whose kernel is not in the tree).
+template <typename data_type, typename index_type>
+measurement_class gpu_csr_merge_spmv(const csr_matrix_class<data_type, index_type> &matrix, const data_type *x,
+                                     const data_type *reference_y, data_type *y_out);
+
 // Measurement names of gpu_bcsr_spmv, in order.
 const std::vector<std::string> &gpu_bcsr_spmv_names();
 

@@ -24,7 +24,7 @@ def test_driver_rows_and_agreement(tmp_path, typ, bs):
     res = json.loads(out.read_text())
     assert sorted(res) == sorted(bs.split(","))
     for b, rows in res.items():
-        for name in ["CPU CSR", "GPU CSR", "GPU CSR-Vector", "jit"] + bm_rows():
+        for name in ["CPU CSR", "GPU CSR", "GPU CSR-Vector", "GPU Merge SpMV", "jit"] + bm_rows():
             assert name in rows, (b, name)
         assert rows["jit"] > 0
 

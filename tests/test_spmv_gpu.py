@@ -217,6 +217,71 @@ def test_csr_ragged_rows_every_group_width():
             check_csr(c, x, v)
 
 
+def csr_from_lengths(lengths, n_cols, seed):
+    """CSR with the given row lengths, random sorted columns (with repeats
+    allowed across rows), values uniform[-1,1]; x uniform[0,1]."""
+    rng = np.random.default_rng(seed)
+    lengths = np.asarray(lengths, np.int64)
+    rp = np.concatenate([[0], np.cumsum(lengths)])
+    cols = np.concatenate([np.sort(rng.choice(n_cols, ln, replace=ln > n_cols)) if ln else np.zeros(0, np.int64)
+                           for ln in lengths]).astype(np.int32) if lengths.sum() else np.zeros(0, np.int32)
+    vals = rng.uniform(-1, 1, int(rp[-1]))
+    x = rng.uniform(0, 1, n_cols)
+    return oracle.CSR(len(lengths), n_cols, rp, cols, vals), x
+
+
+@pytest.mark.parametrize("bs", [1, 2, 4, 8, 32])
+def test_csr_merge_stencil_expansion(bs):
+    m, x = stencil_case(12, 10, bs, kind=1, seed=bs + 20)
+    c = oracle.bcsr_to_csr(m)
+    check_csr(c, x, bm.CSR_MERGE)
+    check_csr(c, x, bm.CSR_MERGE, dtype=bm.F32)
+
+
+MERGE_PATTERNS = {
+    # a merge tile is 512 path items (row ends + nonzeros)
+    "one_row_over_200_tiles": [100000],
+    "long_rows_between_empty_runs": [0] * 700 + [3000] + [0] * 1300 + [1] + [2500, 7] + [0] * 600,
+    "all_rows_empty": [0] * 3000,
+    "power_law": None,
+    "uniform_short": None,
+    "tile_boundary_rows": [511, 1, 510, 2, 512, 0, 1023, 1, 1, 511],
+}
+
+
+@pytest.mark.parametrize("name", sorted(MERGE_PATTERNS))
+@pytest.mark.parametrize("dtype", [bm.F64, bm.F32])
+def test_csr_merge_irregular_rows(name, dtype):
+    rng = np.random.default_rng(len(name))
+    lengths = MERGE_PATTERNS[name]
+    if name == "power_law":
+        lengths = np.minimum((rng.pareto(1.1, 6000) * 3).astype(np.int64), 40000)
+    elif name == "uniform_short":
+        lengths = rng.integers(0, 6, 20000)
+    c, x = csr_from_lengths(lengths, 5000, seed=len(name) + 1)
+    check_csr(c, x, bm.CSR_MERGE, dtype=dtype)
+
+
+def test_csr_auto_choice_follows_row_lengths():
+    cases = {bm.CSR_SCALAR: [9] * 500, bm.CSR_VECTOR: [300] * 50, bm.CSR_MERGE: [40] * 50 + [10] * 50}
+    for want, lengths in cases.items():
+        c, x = csr_from_lengths(lengths, 4000, seed=want)
+        A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
+        assert A.auto_variant() == want
+        check_csr(c, x, bm.SPMV_AUTO)
+    c, x = csr_from_lengths([2] * 2000 + [5000], 6000, seed=9)  # one row dwarfs the mean
+    assert bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols).auto_variant() == bm.CSR_MERGE
+
+
+def test_csr_merge_is_bitwise_reproducible():
+    lengths = np.minimum((np.random.default_rng(3).pareto(1.1, 20000) * 4).astype(np.int64), 60000)
+    c, x = csr_from_lengths(lengths, 8000, seed=4)
+    A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
+    y1 = bm.spmv_host(A, x, bm.CSR_MERGE)
+    y2 = bm.spmv_host(A, x, bm.CSR_MERGE)
+    assert np.array_equal(y1.view(np.uint64), y2.view(np.uint64))
+
+
 def test_fill_matches_host_rng():
     n = 1000
     d = bm.DeviceArray(n)
