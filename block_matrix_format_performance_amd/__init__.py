@@ -101,6 +101,9 @@ def _declare(L):
     L.bmfp_stencil_nnzb.restype = ctypes.c_int64
     L.bmfp_stencil_nnzb.argtypes = [c_int, c_int]
     L.bmfp_gen_stencil9.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_uint64, vp, vp, vp]
+    L.bmfp_mm_read.argtypes = [ctypes.c_char_p, ip, ip, ip, vp, vp, vp]
+    L.bmfp_mm_write.argtypes = [ctypes.c_char_p, c_int, c_int, vp, vp, vp]
+    L.bmfp_block_size_detect.argtypes = [c_int, c_int, vp, vp, c_int, vp, c_int, ip, ip, dp]
     L.bmfp_bcsr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, c_int, c_int, vp, vp, vp]
     L.bmfp_csr_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, vp, vp, vp]
     L.bmfp_stencil_create.argtypes = [ctypes.POINTER(vp), c_int, c_int, c_int, c_int, ctypes.c_uint64, c_int, c_int]
@@ -296,6 +299,43 @@ def gen_stencil9(nx, ny, bs, kind=0, seed=0):
     vals = np.zeros(nnzb * bs * bs, np.float64)
     check(L.bmfp_gen_stencil9(nx, ny, bs, kind, seed, _ptr(rp), _ptr(cols), _ptr(vals)))
     return rp, cols, vals
+
+
+def read_mm(path):
+    """Matrix Market file -> CSR (row_ptr, columns, values, n_rows, n_cols): columns
+    ascending within a row, duplicates summed, symmetric halves mirrored."""
+    L = lib()
+    p = os.fsencode(path)
+    n, m, nnz = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    check(L.bmfp_mm_read(p, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nnz), None, None, None))
+    rp = np.zeros(n.value + 1, np.int32)
+    cols = np.zeros(nnz.value, np.int32)
+    vals = np.zeros(nnz.value, np.float64)
+    check(L.bmfp_mm_read(p, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nnz), _ptr(rp), _ptr(cols), _ptr(vals)))
+    return rp, cols, vals, n.value, m.value
+
+
+def write_mm(path, row_ptr, columns, values, n_rows, n_cols):
+    """CSR -> "coordinate real general" Matrix Market text (csr_matrix_class::write_mm)."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    cols = np.ascontiguousarray(columns, np.int32)
+    vals = np.ascontiguousarray(values, np.float64)
+    check(lib().bmfp_mm_write(os.fsencode(path), n_rows, n_cols, _ptr(rp), _ptr(cols), _ptr(vals)))
+
+
+BLOCK_SIZE_CANDIDATES = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+
+
+def detect_block_size(row_ptr, columns, n_rows, n_cols, value_bytes=8, candidates=BLOCK_SIZE_CANDIDATES):
+    """(bs, nnzb, fill): the candidate block size whose BCSR SpMV moves the fewest
+    compulsory bytes for this CSR pattern."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    cols = np.ascontiguousarray(columns, np.int32)
+    cand = np.ascontiguousarray(candidates, np.int32)
+    bs, nnzb, fill = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+    check(lib().bmfp_block_size_detect(n_rows, n_cols, _ptr(rp), _ptr(cols), value_bytes, _ptr(cand), len(cand),
+                                       ctypes.byref(bs), ctypes.byref(nnzb), ctypes.byref(fill)))
+    return bs.value, nnzb.value, fill.value
 
 
 def gen_n_diag_bcsr(n_rows, blocks_per_row, bs):

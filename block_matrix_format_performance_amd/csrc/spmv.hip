@@ -257,7 +257,7 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
     int *carry_row = A.merge_carry.as<int>();
     AT *carry_val = reinterpret_cast<AT *>(A.merge_carry.as<char>() + (size_t) waves * 8);
     int *bounds = reinterpret_cast<int *>(A.merge_carry.as<char>() + (size_t) waves * 16);
-    k_csr_merge_bounds<<<(unsigned) ((waves + 1 + 255) / 256), 256, 0, s>>>(A.n_rows, A.nnzb, waves, a.rp, bounds);
+    k_csr_merge_bounds<MERGE_TILE><<<(unsigned) ((waves + 1 + 255) / 256), 256, 0, s>>>(A.n_rows, A.nnzb, waves, a.rp, bounds);
     BMFP_LAUNCHED();
     k_csr_merge<VT, AT><<<(unsigned) ((waves + 3) / 4), 256, 0, s>>>(a, A.nnzb, waves, bounds, carry_row, carry_val);
     BMFP_LAUNCHED();

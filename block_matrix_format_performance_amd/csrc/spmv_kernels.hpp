@@ -525,13 +525,14 @@ __device__ __forceinline__ int merge_search(int64_t d, int n, int nnz, const int
 
 // Row coordinate of every tile boundary t = 0..n_waves (diagonal t * MERGE_TILE,
 // clamped to the end of the path).
-static __global__ __launch_bounds__(256) void k_csr_merge_bounds(int n, int nnz, int64_t n_waves, const int *__restrict__ rp,
+template <int TILE>
+__global__ __launch_bounds__(256) void k_csr_merge_bounds(int n, int nnz, int64_t n_waves, const int *__restrict__ rp,
                                                           int *__restrict__ bounds)
 {
   const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
   if (t > n_waves) return;
   const int64_t total = (int64_t) n + nnz;
-  bounds[t] = merge_search(t * MERGE_TILE < total ? t * MERGE_TILE : total, n, nnz, rp);
+  bounds[t] = merge_search(t * TILE < total ? t * TILE : total, n, nnz, rp);
 }
 
 template <class VT, class AT>

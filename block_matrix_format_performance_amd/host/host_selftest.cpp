@@ -7,7 +7,11 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+#include <cstdlib>
+#include <unistd.h>
 #include "bmfp/matrix_converters.h"
+#include "bmfp/matrix_market.h"
 #include "bmfp/measurement_class.h"
 
 template <class T>
@@ -33,6 +37,43 @@ static std::string capture_compare(unsigned n, const double *a, const double *b)
   return out;
 }
 
+// Text of a file as a JSON string body.
+static std::string file_json(const std::string &path)
+{
+  std::string out;
+  FILE *f = std::fopen(path.c_str(), "rb");
+  int c;
+  while (f && (c = std::fgetc(f)) != EOF) {
+    if (c == '\n') out += "\\n";
+    else if (c == '"' || c == '\\') (out += '\\') += (char) c;
+    else out += (char) c;
+  }
+  if (f) std::fclose(f);
+  return out;
+}
+
+// Matrix Market text of csr through matrix_converters.h's write_mm and through
+// the C ABI, and whether reading it back restores the matrix exactly.
+static void put_mm(const csr_matrix_class<double, int> &csr)
+{
+  char tmpl[] = "/tmp/bmfp_mm_XXXXXX";
+  const int fd = mkstemp(tmpl);
+  if (fd < 0) std::exit(1);
+  close(fd);
+  const std::string path = tmpl;
+  csr.write_mm(path);
+  std::printf("\"mm_text\": \"%s\",\n", file_json(path).c_str());
+  write_matrix_market(csr, path);
+  std::printf("\"mm_text_abi\": \"%s\",\n", file_json(path).c_str());
+  auto back = read_matrix_market<double, int>(path);
+  bool same = back->n_rows == csr.n_rows && back->n_cols == csr.n_cols && back->nnz == csr.nnz &&
+              std::equal(csr.row_ptr.get(), csr.row_ptr.get() + csr.n_rows + 1, back->row_ptr.get()) &&
+              std::equal(csr.columns.get(), csr.columns.get() + csr.nnz, back->columns.get()) &&
+              std::equal(csr.values.get(), csr.values.get() + csr.nnz, back->values.get());
+  std::printf("\"mm_readback_exact\": %s,\n", same ? "true" : "false");
+  std::remove(path.c_str());
+}
+
 int main()
 {
   std::printf("{\n\"matrices\": [\n");
@@ -55,6 +96,7 @@ int main()
     put("csr_row_ptr", csr.row_ptr.get(), csr.n_rows + 1);
     put("csr_columns", csr.columns.get(), csr.nnz);
     put("block_lookup_mid", lookup.data(), (long) lookup.size());
+    put_mm(csr);
     put("roundtrip_row_ptr", back->row_ptr.get(), back->n_rows + 1);
     put("roundtrip_columns", back->columns.get(), back->nnzb);
     put("roundtrip_values", back->values.get(), back->size());

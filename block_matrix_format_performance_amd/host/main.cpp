@@ -15,6 +15,10 @@
 //   --x ones|random            x = 1 (reference) or uniform[0,1] (seed 0)
 //   --reps R                   repetitions averaged (default 10)
 //   --out FILE                 JSON output (default result.json)
+//   --mm FILE                  a Matrix Market file instead of a generated
+//                              matrix: CSR rows on it as read, BCSR rows at
+//                              the detected block size (or each --bs that
+//                              divides its dimensions)
 //   --solve                    also run the FP64 BiCGStab of the 256x256
 //                              bs=2 stencil (b = A*1, x0 = 0, eps 1e-8) and
 //                              report its iterations, error and time
@@ -38,6 +42,7 @@
 #include "bmfp/bicgstab.h"
 #include "bmfp/gpu_matrix_multiplier.h"
 #include "bmfp/matrix_converters.h"
+#include "bmfp/matrix_market.h"
 #include "bmfp/measurement_class.h"
 
 namespace {
@@ -45,6 +50,8 @@ namespace {
 struct Options {
   std::string type = "float";
   std::vector<int> bs = {2, 4, 8, 16, 32};
+  bool bs_given = false;
+  std::string mm;
   std::string matrix = "ndiag";
   int n_rows = 50000;
   int bpr = 6;
@@ -190,6 +197,31 @@ std::map<std::string, double> run_bs(int bs, const Options &o)
   return perform_measurements(matrix, *block_matrix, o);
 }
 
+// Matrix Market input: result maps keyed by block size.
+template <typename T>
+std::map<int, std::map<std::string, double>> run_mm(const Options &o)
+{
+  auto csr = read_matrix_market<T, int>(o.mm);
+  std::cout << o.mm << ": " << csr->n_rows << " x " << csr->n_cols << ", " << csr->nnz << " nonzeros\n";
+  std::vector<int> sizes;
+  if (o.bs_given) {
+    for (int bs : o.bs)
+      if (bs >= 1 && csr->n_rows % bs == 0 && csr->n_cols % bs == 0) sizes.push_back(bs);
+      else std::cout << "bs " << bs << " does not divide the matrix; skipped\n";
+  } else {
+    const block_size_choice c = detect_block_size(*csr);
+    std::cout << "detected block size " << c.bs << " (" << c.nnzb << " blocks, fill " << c.fill << ")\n";
+    sizes.push_back(c.bs);
+  }
+  std::map<int, std::map<std::string, double>> out;
+  for (int bs : sizes) {
+    std::cout << "\nBS: " << bs << "\n";
+    auto block_matrix = csr_to_bcsr<T, int>(*csr, bs);
+    out[bs] = perform_measurements(*csr, *block_matrix, o);
+  }
+  return out;
+}
+
 int run_solve()
 {
   // 256x256 node grid, bs = 2, diagonally dominant, b = A*1, x0 = 0, eps 1e-8
@@ -233,7 +265,11 @@ int main(int argc, char **argv)
       return argv[++i];
     };
     if (a == "--type") o.type = next();
-    else if (a == "--bs") o.bs = parse_list(next());
+    else if (a == "--bs") {
+      o.bs = parse_list(next());
+      o.bs_given = true;
+    }
+    else if (a == "--mm") o.mm = next();
     else if (a == "--matrix") o.matrix = next();
     else if (a == "--n-rows") o.n_rows = std::atoi(next());
     else if (a == "--bpr") o.bpr = std::atoi(next());
@@ -252,7 +288,16 @@ int main(int argc, char **argv)
     return 1;
   }
   std::map<int, std::map<std::string, double>> all;
-  for (int bs : o.bs) all[bs] = o.type == "double" ? run_bs<double>(bs, o) : run_bs<float>(bs, o);
+  if (!o.mm.empty()) {
+    try {
+      all = o.type == "double" ? run_mm<double>(o) : run_mm<float>(o);
+    } catch (const std::exception &e) {
+      std::cerr << "--mm " << o.mm << ": " << e.what() << std::endl;
+      return 1;
+    }
+  } else {
+    for (int bs : o.bs) all[bs] = o.type == "double" ? run_bs<double>(bs, o) : run_bs<float>(bs, o);
+  }
 
   std::ofstream os(o.out);
   os << "{\n";

@@ -167,6 +167,30 @@ int bmfp_matrix_shape(bmfp_matrix m, int *is_bcsr, int *dtype, int *layout,
  * indices once, x once, y once (never the reference's CPU model, which
  * counts x per nonzero). */
 int bmfp_spmv_traffic(bmfp_matrix m, double *bytes, double *flops);
+/* ---- Matrix Market I/O and block-size detection (host only) ---------------
+ * Replace common/mmio.{h,c} and csr_matrix_class::write_mm
+ * (matrix_converters.h:119-144).  bmfp_mm_read parses coordinate (real,
+ * integer, pattern; general, symmetric, skew-symmetric) and array (real,
+ * integer; general) files into CSR: columns ascending within a row, duplicate
+ * entries summed, the mirrored half of symmetric storage added, zeros of
+ * array files dropped.  Call it with row_ptr == NULL for the sizes, then with
+ * arrays of n_rows+1, nnz and nnz elements (the sizes as returned). */
+int bmfp_mm_read(const char *path, int *n_rows, int *n_cols, int *nnz,
+                 int *row_ptr, int *columns, double *values);
+/* "coordinate real general", 1-based "%d %d %10.20g" entries in CSR order:
+ * the text write_mm produces through mmio. */
+int bmfp_mm_write(const char *path, int n_rows, int n_cols, const int *row_ptr,
+                  const int *columns, const double *values);
+/* Block size for BCSR storage of a CSR pattern (no reference counterpart):
+ * of the candidates dividing both dimensions, the one whose BCSR SpMV with
+ * value_bytes-wide values moves the fewest compulsory bytes (values, block
+ * columns, block row pointers, x, y).  Also the block count and the fill
+ * (stored values / nonzeros) of the choice. */
+int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr,
+                           const int *columns, int value_bytes,
+                           const int *candidates, int n_candidates, int *bs,
+                           int *nnzb, double *fill);
+
 /* Download values (as doubles, in the handle's layout, unscaled). */
 int bmfp_matrix_download_values(bmfp_matrix m, double *values);
 

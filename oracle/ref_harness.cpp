@@ -11,6 +11,7 @@
 //   get_block_data_by_column   matrix_converters.h:61-70
 //   measurement_class          measurement_class.h:11-58, measurement_class.cpp:7-16
 //   compare_results            measurement_class.h:60-89 (its stderr text)
+//   csr_matrix_class::write_mm matrix_converters.h:119-144 (through common/mmio.c)
 #include <string>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +19,7 @@
 #include <vector>
 #include <sstream>
 #include <iostream>
+#include <unistd.h>
 
 #include "matrix_converters.h"
 #include "measurement_class.h"
@@ -77,6 +79,27 @@ int main()
       lookup.push_back(*m->get_block_data_by_column(row, col));
     }
     put_doubles("block_lookup_mid", lookup.data(), (long) lookup.size());
+    {
+      // the Matrix Market text write_mm produces for this CSR matrix
+      char path[] = "/tmp/ref_mm_XXXXXX";
+      const int fd = mkstemp(path);
+      if (fd < 0) return 1;
+      close(fd);
+      csr.write_mm(path);
+      FILE *f = std::fopen(path, "rb");
+      std::string text;
+      int c;
+      while (f && (c = std::fgetc(f)) != EOF) text += (char) c;
+      if (f) std::fclose(f);
+      std::remove(path);
+      std::printf("\"mm_text\": \"");
+      for (char ch : text) {
+        if (ch == '\n') std::printf("\\n");
+        else if (ch == '"' || ch == '\\') std::printf("\\%c", ch);
+        else std::putchar(ch);
+      }
+      std::printf("\",\n");
+    }
     put_doubles("csr_values", csr.values.get(), csr.nnz, false);
     std::printf("}%s\n", ci + 1 < ncases ? "," : "");
   }

@@ -38,6 +38,15 @@ def test_converters_match_reference_exactly(selftest, golden):
             assert mine[key] == ref[key], key
 
 
+def test_write_mm_text_matches_reference_exactly(selftest, golden):
+    # the reference's write_mm (through common/mmio.c) vs matrix_converters.h's
+    # write_mm and bmfp_mm_write, byte for byte; reading it back is exact
+    for mine, ref in zip(selftest["matrices"], golden["matrices"]):
+        assert mine["mm_text"] == ref["mm_text"]
+        assert mine["mm_text_abi"] == ref["mm_text"]
+        assert mine["mm_readback_exact"] is True
+
+
 def test_csr_to_bcsr_round_trip(selftest):
     for m in selftest["matrices"]:
         assert m["roundtrip_row_ptr"] == m["bcsr_row_ptr"]

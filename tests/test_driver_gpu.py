@@ -9,6 +9,7 @@ import subprocess
 import pytest
 
 import block_matrix_format_performance_amd as bm
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +33,24 @@ def test_driver_rows_and_agreement(tmp_path, typ, bs):
 def bm_rows():
     return ["GPU BCSR (row major, thread per row)", "GPU BCSR (column major, thread per row, coal x)",
             "GPU BSR (rocSPARSE, row major)"]
+
+
+def test_driver_matrix_market_input(tmp_path):
+    print("\n[gpu-case] driver --mm (4x4-block stencil file)", flush=True)
+    m = oracle.gen_stencil9(30, 20, 4, 0, 5)
+    c = oracle.bcsr_to_csr(m)
+    path = tmp_path / "s4.mtx"
+    bm.write_mm(path, c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
+    out = tmp_path / "result.json"
+    p = subprocess.run([bm.MAIN_PATH, "--mm", str(path), "--type", "double", "--reps", "2", "--x", "random",
+                        "--out", str(out)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert "detected block size 4" in p.stdout
+    assert "ERROR" not in p.stdout + p.stderr, p.stdout + p.stderr
+    res = json.loads(out.read_text())
+    assert list(res) == ["4"]
+    for name in ["CPU CSR", "GPU CSR", "GPU CSR-Vector", "GPU Merge SpMV", "jit"] + bm_rows():
+        assert name in res["4"], name
 
 
 def test_driver_solve():

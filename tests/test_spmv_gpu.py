@@ -282,6 +282,22 @@ def test_csr_merge_is_bitwise_reproducible():
     assert np.array_equal(y1.view(np.uint64), y2.view(np.uint64))
 
 
+def test_matrix_market_file_through_every_path(tmp_path):
+    # a file's CSR through AUTO and merge, and its detected blocks through BCSR AUTO
+    m, x = stencil_case(14, 11, 3, kind=1, seed=31)
+    c = oracle.bcsr_to_csr(m)
+    path = tmp_path / "s3.mtx"
+    bm.write_mm(path, c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
+    rp, cols, vals, n, ncol = bm.read_mm(path)
+    r = oracle.CSR(n, ncol, rp, cols, vals)
+    for v in (bm.SPMV_AUTO, bm.CSR_MERGE):
+        check_csr(r, x, v)
+    bs, nnzb, fill = bm.detect_block_size(rp, cols, n, ncol)
+    assert (bs, nnzb, fill) == (3, m.nnzb, 1.0)
+    b = oracle.csr_to_bcsr(r, bs)
+    check_bcsr(b, x, bm.ROW_MAJOR, bm.SPMV_AUTO)
+
+
 def test_fill_matches_host_rng():
     n = 1000
     d = bm.DeviceArray(n)
