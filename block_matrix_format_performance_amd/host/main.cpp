@@ -15,6 +15,8 @@
 //   --x ones|random            x = 1 (reference) or uniform[0,1] (seed 0)
 //   --reps R                   repetitions averaged (default 10)
 //   --out FILE                 JSON output (default result.json)
+//   --out-gbs FILE             also each row's effective bandwidth (GB/s of
+//                              its compulsory bytes), same layout
 //   --mm FILE                  a Matrix Market file instead of a generated
 //                              matrix: CSR rows on it as read, BCSR rows at
 //                              the detected block size (or each --bs that
@@ -59,6 +61,7 @@ struct Options {
   std::string x = "ones";
   int reps = 10;
   std::string out = "result.json";
+  std::string out_gbs;
   bool solve = false;
 };
 
@@ -99,11 +102,20 @@ public:
   }
 };
 
+// Per format: mean seconds and effective bandwidth (GB/s).
+struct Rows {
+  std::map<std::string, double> seconds, gbs;
+  void add(const measurement_class &m)
+  {
+    seconds[m.get_format()] = m.get_elapsed();
+    gbs[m.get_format()] = m.get_effective_bandwidth();
+  }
+};
+
 template <typename T, typename I>
-std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matrix,
-                                                   bcsr_matrix_class<T, I> &block_matrix, const Options &o)
+Rows perform_measurements(csr_matrix_class<T, I> &matrix, bcsr_matrix_class<T, I> &block_matrix, const Options &o)
 {
-  std::map<std::string, double> results;
+  Rows results;
   const size_t n = (size_t) matrix.n_rows;
   std::vector<T> x(n), reference(n);
   if (o.x == "random") {
@@ -124,7 +136,7 @@ std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matri
     measurement_class acc;
     for (int i = 0; i < o.reps; i++) acc += action();
     acc.finalize();
-    results[acc.get_format()] = acc.get_elapsed();
+    results.add(acc);
     return acc;
   };
 
@@ -144,7 +156,7 @@ std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matri
   {
     // "jit" (main.cpp:170-294): one timed launch of the run-time-specialised kernel
     auto jit = gpu_bcsr_jit_spmv<T, I>(block_matrix, transposed.data(), xp, reference.data());
-    results[jit.get_format()] = jit.get_elapsed();
+    results.add(jit);
     printer.print_time(jit);
   }
   // o.reps calls, averaged by a default-constructed accumulator per format
@@ -157,7 +169,7 @@ std::map<std::string, double> perform_measurements(csr_matrix_class<T, I> &matri
   }
   for (auto &m : acc) {
     m.finalize();
-    results[m.get_format()] = m.get_elapsed();
+    results.add(m);
     printer.print_time(m);
   }
   return results;
@@ -182,7 +194,7 @@ std::string json_number(double v)
 }
 
 template <typename T>
-std::map<std::string, double> run_bs(int bs, const Options &o)
+Rows run_bs(int bs, const Options &o)
 {
   std::unique_ptr<bcsr_matrix_class<T, int>> block_matrix;
   if (o.matrix == "stencil") {
@@ -199,7 +211,7 @@ std::map<std::string, double> run_bs(int bs, const Options &o)
 
 // Matrix Market input: result maps keyed by block size.
 template <typename T>
-std::map<int, std::map<std::string, double>> run_mm(const Options &o)
+std::map<int, Rows> run_mm(const Options &o)
 {
   auto csr = read_matrix_market<T, int>(o.mm);
   std::cout << o.mm << ": " << csr->n_rows << " x " << csr->n_cols << ", " << csr->nnz << " nonzeros\n";
@@ -213,7 +225,7 @@ std::map<int, std::map<std::string, double>> run_mm(const Options &o)
     std::cout << "detected block size " << c.bs << " (" << c.nnzb << " blocks, fill " << c.fill << ")\n";
     sizes.push_back(c.bs);
   }
-  std::map<int, std::map<std::string, double>> out;
+  std::map<int, Rows> out;
   for (int bs : sizes) {
     std::cout << "\nBS: " << bs << "\n";
     auto block_matrix = csr_to_bcsr<T, int>(*csr, bs);
@@ -277,6 +289,7 @@ int main(int argc, char **argv)
     else if (a == "--x") o.x = next();
     else if (a == "--reps") o.reps = std::atoi(next());
     else if (a == "--out") o.out = next();
+    else if (a == "--out-gbs") o.out_gbs = next();
     else if (a == "--solve") o.solve = true;
     else {
       std::cerr << "unknown option " << a << std::endl;
@@ -287,7 +300,7 @@ int main(int argc, char **argv)
     std::cerr << "bmfp_init: " << bmfp_last_error() << std::endl;
     return 1;
   }
-  std::map<int, std::map<std::string, double>> all;
+  std::map<int, Rows> all;
   if (!o.mm.empty()) {
     try {
       all = o.type == "double" ? run_mm<double>(o) : run_mm<float>(o);
@@ -299,19 +312,23 @@ int main(int argc, char **argv)
     for (int bs : o.bs) all[bs] = o.type == "double" ? run_bs<double>(bs, o) : run_bs<float>(bs, o);
   }
 
-  std::ofstream os(o.out);
-  os << "{\n";
-  size_t i = 0;
-  for (const auto &kv : all) {
-    os << "  \"" << kv.first << "\": {\n";
-    size_t j = 0;
-    for (const auto &r : kv.second)
-      os << "    \"" << json_escape(r.first) << "\": " << json_number(r.second)
-         << (++j < kv.second.size() ? ",\n" : "\n");
-    os << "  }" << (++i < all.size() ? ",\n" : "\n");
-  }
-  os << "}\n";
-  os.close();
+  // result.json layout { "<bs>": { "<format>": value } }
+  auto write_json = [&](const std::string &path, bool gbs) {
+    std::ofstream os(path);
+    os << "{\n";
+    size_t i = 0;
+    for (const auto &kv : all) {
+      const auto &rows = gbs ? kv.second.gbs : kv.second.seconds;
+      os << "  \"" << kv.first << "\": {\n";
+      size_t j = 0;
+      for (const auto &r : rows)
+        os << "    \"" << json_escape(r.first) << "\": " << json_number(r.second) << (++j < rows.size() ? ",\n" : "\n");
+      os << "  }" << (++i < all.size() ? ",\n" : "\n");
+    }
+    os << "}\n";
+  };
+  write_json(o.out, false);
+  if (!o.out_gbs.empty()) write_json(o.out_gbs, true);
   std::cout << "wrote " << o.out << std::endl;
   return o.solve ? run_solve() : 0;
 }

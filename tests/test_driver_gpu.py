@@ -10,6 +10,7 @@ import pytest
 
 import block_matrix_format_performance_amd as bm
 import oracle
+from block_matrix_format_performance_amd import report
 
 pytestmark = pytest.mark.gpu
 
@@ -17,9 +18,10 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("typ,bs", [("float", "2,5"), ("double", "4,7")])
 def test_driver_rows_and_agreement(tmp_path, typ, bs):
     print(f"\n[gpu-case] driver --type {typ} --bs {bs}", flush=True)
-    out = tmp_path / "result.json"
+    out, gbs = tmp_path / "result.json", tmp_path / "gbs.json"
     p = subprocess.run([bm.MAIN_PATH, "--type", typ, "--bs", bs, "--n-rows", "3000", "--bpr", "5", "--reps", "2",
-                        "--x", "random", "--out", str(out)], capture_output=True, text=True, timeout=300)
+                        "--x", "random", "--out", str(out), "--out-gbs", str(gbs)],
+                       capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stderr
     assert "ERROR" not in p.stdout + p.stderr, p.stdout + p.stderr
     res = json.loads(out.read_text())
@@ -28,6 +30,16 @@ def test_driver_rows_and_agreement(tmp_path, typ, bs):
         for name in ["CPU CSR", "GPU CSR", "GPU CSR-Vector", "GPU Merge SpMV", "jit"] + bm_rows():
             assert name in rows, (b, name)
         assert rows["jit"] > 0
+    # the report: speed-up over CPU CSR and the fraction of the HBM roofline
+    bw = json.loads(gbs.read_text())
+    assert sorted(bw) == sorted(res)
+    for b in bw:
+        assert 0 < bw[b]["GPU Merge SpMV"] < 8000 and 0 < bw[b]["GPU CSR"] < 8000
+    sp = report.speedup(report.load(out))
+    # rows a family cannot run (null in result.json) come out as NaN
+    assert all(v > 0 for rows in sp.values() for v in rows.values() if v == v)
+    assert all(sp[int(b)]["GPU Merge SpMV"] > 0 and sp[int(b)]["jit"] > 0 for b in res)
+    assert report.main([str(out), "--gbs", str(gbs)]) == 0
 
 
 def bm_rows():
