@@ -13,6 +13,7 @@
 #include "bmfp/matrix_converters.h"
 #include "bmfp/matrix_market.h"
 #include "bmfp/measurement_class.h"
+#include "bmfp/truss_bridge.h"
 
 template <class T>
 static void put(const char *name, const T *a, long n, bool comma = true)
@@ -124,6 +125,35 @@ int main()
   std::printf("\"compare\": {\"same\": \"%s\", \"bad\": \"%s\", \"tiny\": \"%s\"},\n",
               capture_compare(4, y_same, r1).c_str(), capture_compare(4, y_bad, r1).c_str(),
               capture_compare(4, y_tiny, r1).c_str());
+  // a small suspension-bridge truss: geometry, supports, K and f
+  {
+    auto load = [](double x) { return std::pair<double, double>(x > 80 && x < 100 ? 3e5 : 0.0, x > 80 && x < 100 ? -1e6 : 0.0); };
+    golden_gate_bridge_2d<double, int> br(load, 100.0, 40.0, 100.0, 10.0);
+    const int nn = br.nodes_count(), ne = br.elements_count();
+    std::vector<double> xy, ea;
+    std::vector<int> ends, fixed;
+    for (int i = 0; i < nn; i++) {
+      xy.push_back(br.node_x(i));
+      xy.push_back(br.node_y(i));
+    }
+    for (int e = 0; e < ne; e++) {
+      ends.push_back(br.element_nodes(e).first);
+      ends.push_back(br.element_nodes(e).second);
+      ea.push_back(br.element_ea(e));
+    }
+    for (int d = 0; d < 2 * nn; d++) fixed.push_back(br.dof_fixed(d) ? 1 : 0);
+    const auto &K = *br.matrix;
+    std::printf("\"bridge\": {\"nodes\": %d,\n", nn);
+    put("xy", xy.data(), (long) xy.size());
+    put("ends", ends.data(), (long) ends.size());
+    put("ea", ea.data(), (long) ea.size());
+    put("fixed", fixed.data(), (long) fixed.size());
+    put("row_ptr", K.row_ptr.get(), K.n_rows + 1);
+    put("columns", K.columns.get(), K.nnzb);
+    put("values", K.values.get(), K.size());
+    put("rhs", br.forces_rhs.get(), 2L * nn, false);
+    std::printf("},\n");
+  }
   // 9-point stencil generated through the library (for the bitwise check)
   auto st = gen_stencil9_bcsr<double, int>(5, 4, 3, 1, 7);
   std::printf("\"stencil\": {\"nx\": 5, \"ny\": 4, \"bs\": 3, \"kind\": 1, \"seed\": 7,\n");

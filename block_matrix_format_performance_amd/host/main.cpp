@@ -21,6 +21,17 @@
 //                              matrix: CSR rows on it as read, BCSR rows at
 //                              the detected block size (or each --bs that
 //                              divides its dimensions)
+//   --bridge                   the suspension-bridge truss (bs = 2) instead of
+//                              a generated matrix: every SpMV row on it
+//   --bridge-solve             build the bridge, write matrix.mtx and
+//                              output_1.vtk, solve K u = f with the Jacobi-
+//                              preconditioned FP64 BiCGStab and write
+//                              output_2.vtk (displaced), bridge_rhs.txt and
+//                              bridge_solution.txt (main.cpp:349-381)
+//   --bridge-span M            main span in metres (default 1280; sides 345)
+//   --bridge-eps E             relative residual target (default 1e-8: the
+//                              slender truss is ill-conditioned, and its
+//                              true residual stalls near 5e-9 in FP64)
 //   --solve                    also run the FP64 BiCGStab of the 256x256
 //                              bs=2 stencil (b = A*1, x0 = 0, eps 1e-8) and
 //                              report its iterations, error and time
@@ -46,6 +57,7 @@
 #include "bmfp/matrix_converters.h"
 #include "bmfp/matrix_market.h"
 #include "bmfp/measurement_class.h"
+#include "bmfp/truss_bridge.h"
 
 namespace {
 
@@ -54,6 +66,8 @@ struct Options {
   std::vector<int> bs = {2, 4, 8, 16, 32};
   bool bs_given = false;
   std::string mm;
+  bool bridge = false, bridge_solve = false;
+  double bridge_span = 1280.0, bridge_eps = 1e-8;
   std::string matrix = "ndiag";
   int n_rows = 50000;
   int bpr = 6;
@@ -234,6 +248,46 @@ std::map<int, Rows> run_mm(const Options &o)
   return out;
 }
 
+// The bridge of main.cpp:349-381: towers 260 m high, 7.62 m panels, 2 MN
+// downward on every deck node within 450 m of mid-span.
+golden_gate_bridge_2d<double, int> make_bridge(const Options &o)
+{
+  const double side = 345.0, main_span = o.bridge_span;
+  auto load = [=](double x) {
+    const double mid = (main_span + 2 * side) / 2, window = 450;
+    return std::pair<double, double>(0.0, x > mid - window && x < mid + window ? -2000000.0 : 0.0);
+  };
+  return golden_gate_bridge_2d<double, int>(load, main_span, side, 260.0, 7.62);
+}
+
+int run_bridge_solve(const Options &o)
+{
+  auto bridge = make_bridge(o);
+  const bcsr_matrix_class<double, int> &K = *bridge.matrix;
+  const size_t n = (size_t) K.n_rows * K.bs;
+  std::cout << "bridge: " << bridge.nodes_count() << " nodes, " << bridge.elements_count() << " bars, n = " << n
+            << ", " << K.nnzb << " blocks\n";
+  csr_matrix_class<double, int>(K).write_mm("matrix.mtx");
+  bridge.write_vtk("output_1.vtk");
+  gpu_bicgstab<double, int> solver(K, true);
+  const double *u = solver.solve(K, bridge.forces_rhs.get(), o.bridge_eps, 200000);
+  const bmfp_solve_info &info = solver.last_info();
+  bridge.write_vtk("output_2.vtk", u);
+  std::ofstream rhs("bridge_rhs.txt"), sol("bridge_solution.txt");
+  rhs << std::setprecision(17);
+  sol << std::setprecision(17);
+  double umax = 0;
+  for (size_t i = 0; i < n; i++) {
+    rhs << bridge.forces_rhs[i] << "\n";
+    sol << u[i] << "\n";
+    umax = std::max(umax, std::fabs(u[i]));
+  }
+  std::cout << "Jacobi BiCGStab: status " << info.status << ", iterations " << info.iterations << ", ||r||/||b|| "
+            << info.rel_residual << ", true " << info.true_rel_residual << ", " << info.seconds * 1e3
+            << " ms; max |u| " << umax << " m" << std::endl;
+  return info.status == BMFP_OK ? 0 : 1;
+}
+
 int run_solve()
 {
   // 256x256 node grid, bs = 2, diagonally dominant, b = A*1, x0 = 0, eps 1e-8
@@ -282,6 +336,10 @@ int main(int argc, char **argv)
       o.bs_given = true;
     }
     else if (a == "--mm") o.mm = next();
+    else if (a == "--bridge") o.bridge = true;
+    else if (a == "--bridge-solve") o.bridge_solve = true;
+    else if (a == "--bridge-span") o.bridge_span = std::atof(next());
+    else if (a == "--bridge-eps") o.bridge_eps = std::atof(next());
     else if (a == "--matrix") o.matrix = next();
     else if (a == "--n-rows") o.n_rows = std::atoi(next());
     else if (a == "--bpr") o.bpr = std::atoi(next());
@@ -300,8 +358,23 @@ int main(int argc, char **argv)
     std::cerr << "bmfp_init: " << bmfp_last_error() << std::endl;
     return 1;
   }
+  if (o.bridge_solve) return run_bridge_solve(o);
   std::map<int, Rows> all;
-  if (!o.mm.empty()) {
+  if (o.bridge) {
+    auto bridge = make_bridge(o);
+    std::cout << "\nBS: 2 (suspension bridge, " << bridge.nodes_count() << " nodes)\n";
+    csr_matrix_class<double, int> csr(*bridge.matrix);
+    if (o.type == "double") {
+      all[2] = perform_measurements(csr, *bridge.matrix, o);
+    } else {
+      auto b32 = csr_to_bcsr<float, int>(
+          csr_matrix_class<float, int>(csr.n_rows, csr.n_cols, csr.nnz, csr.row_ptr.get(), csr.columns.get(),
+                                       std::vector<float>(csr.values.get(), csr.values.get() + csr.nnz).data()),
+          2);
+      csr_matrix_class<float, int> c32(*b32);
+      all[2] = perform_measurements(c32, *b32, o);
+    }
+  } else if (!o.mm.empty()) {
     try {
       all = o.type == "double" ? run_mm<double>(o) : run_mm<float>(o);
     } catch (const std::exception &e) {
