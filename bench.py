@@ -364,6 +364,49 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
     }
 
 
+def csr_shape_comparison(reps=20):
+    """The CSR families (scalar, lane group, merge path) on ~1.4e8-nonzero
+    matrices of different row-length shapes, FP64, x uniform[0,1]: GB/s of
+    the compulsory bytes (12 nnz + 4 (n+1) + 16 n)."""
+    rng = np.random.default_rng(0)
+    out = []
+
+    def run(name, rp, cols, n_cols):
+        rp = np.ascontiguousarray(rp, np.int32)
+        n, nnz = len(rp) - 1, int(rp[-1])
+        A = bm.Matrix.csr(rp, cols, np.ones(nnz), n, n_cols)
+        x = bm.DeviceArray(n_cols)
+        x.fill(0, seed=1, stream_id=4)
+        y = bm.DeviceArray(n)
+        by = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n
+        row = {"matrix": name, "n": n, "nnz": nnz, "auto": {1: "scalar", 2: "vector", 9: "merge"}[A.auto_variant()]}
+        for v, vn in ((bm.CSR_SCALAR, "scalar"), (bm.CSR_VECTOR, "vector"), (bm.CSR_MERGE, "merge")):
+            t = A.spmv_timed(x, y, v, warmup=3, reps=reps)
+            row[vn + "_GBs"] = round(by / t / 1e9, 1)
+        log(f"[bench] csr {name:26s} " + " ".join(f"{k} {row[k]}" for k in row if k.endswith("GBs"))
+            + f" (auto: {row['auto']})")
+        out.append(row)
+        A.destroy()
+
+    for L in (4, 16, 256):
+        n = int(1.4e8 // L)
+        rp = np.arange(n + 1, dtype=np.int64) * L
+        cols = np.clip(np.repeat(np.arange(n, dtype=np.int64), L) + np.tile(np.arange(L) - L // 2, n) * 3,
+                       0, n - 1).astype(np.int32)
+        run(f"equal rows of {L}", rp, cols, n)
+    for mean in (8, 40):
+        n = int(1.4e8 // mean)
+        L = rng.integers(1, 2 * mean, n)
+        rp = np.concatenate([[0], np.cumsum(L)])
+        cols = np.clip(np.repeat(np.arange(n), L) + rng.integers(-50, 50, int(rp[-1])), 0, n - 1).astype(np.int32)
+        run(f"ragged rows, mean {mean}", rp, cols, n)
+    n = 2_000_000
+    L = np.minimum((rng.pareto(1.2, n) * 6).astype(np.int64), 500_000)
+    rp = np.concatenate([[0], np.cumsum(L)])
+    run("power-law rows (max 500k)", rp, rng.integers(0, n, int(rp[-1]), dtype=np.int32), n)
+    return out
+
+
 def parse_overrides(text):
     out = {}
     for kv in filter(None, text.split(",")):
@@ -395,6 +438,8 @@ def main():
                     help="requests of the serving stream replayed (0 = skip the serving leg)")
     ap.add_argument("--serve-lambdas", default="200,1000,5000")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--csr-only", action="store_true",
+                    help="run only the CSR family comparison (scalar / lane group / merge path) over row shapes")
     ap.add_argument("--serve-no-plain", action="store_true",
                     help="skip the one-at-a-time FP64 comparator of the serving leg")
     ap.add_argument("--serve-config", default="",
@@ -415,6 +460,9 @@ def main():
 
     if args.only:
         run_only(args, total_values, stream)
+        return
+    if args.csr_only:
+        print(json.dumps({"csr_shapes": csr_shape_comparison()}), flush=True)
         return
     if args.serve_only:
         print(json.dumps(serving_comparison(args.serve_requests,
