@@ -11,7 +11,8 @@ kernel family/layout (chosen by timing every family first).  value = compulsory
 bytes of the step / step time (GB/s); the timed region is K steps between two
 HIP events on the work stream, inputs already resident in HBM.
 
-Also reported: per-(bs, layout, family) GB/s, the dominant kernel's roofline
+Also reported: per-(bs, layout, family) GB/s (with the CSR expansion of each
+matrix under the three CSR families), the dominant kernel's roofline
 (HIP-event launch time; compulsory bytes per launch), and the CPU baseline:
 the oracle's single-threaded CSR SpMV (main.cpp:36-45 restated) on a bounded
 sample, timed on this host.
@@ -438,6 +439,8 @@ def main():
                     help="requests of the serving stream replayed (0 = skip the serving leg)")
     ap.add_argument("--serve-lambdas", default="200,1000,5000")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--no-csr-rows", action="store_true",
+                    help="skip the CSR rows (scalar / lane group / merge) of each bs in the sweep")
     ap.add_argument("--csr-only", action="store_true",
                     help="run only the CSR family comparison (scalar / lane group / merge path) over row shapes")
     ap.add_argument("--serve-no-plain", action="store_true",
@@ -501,6 +504,18 @@ def main():
                 log(f"[bench]   {key:28s} {t*1e6:9.1f} us  {gbs:8.1f} GB/s  {gbs/HBM_PEAK_GBS*100:5.1f}%")
                 if fam != bm.BCSR_LIBRARY and (best is None or t < best[0]):
                     best = (t, layout, fam, key)
+        if not args.no_csr_rows:
+            # the same matrix as CSR (expanded in device memory): thread per row,
+            # lane group per row and merge path, over the CSR compulsory bytes
+            C = A0.to_csr()
+            cbytes, _ = C.traffic()
+            for fam, name in ((bm.CSR_SCALAR, "scalar"), (bm.CSR_VECTOR, "vector"), (bm.CSR_MERGE, "merge")):
+                t = C.spmv_timed(x, y, fam, warmup=5, reps=args.tune_reps, stream=stream.handle)
+                key = f"csr/{name}"
+                results[key] = round(cbytes / t / 1e9, 1)
+                log(f"[bench]   {key:28s} {t*1e6:9.1f} us  {cbytes / t / 1e9:8.1f} GB/s  "
+                    f"{cbytes / t / 1e9 / HBM_PEAK_GBS * 100:5.1f}%")
+            C.destroy()
         sweep[str(bs)] = results
         keep = mats[best[1]]
         for layout in list(mats):

@@ -423,6 +423,19 @@ class Matrix:
         check(lib().bmfp_spmv_auto_variant(self.handle, ctypes.byref(v)))
         return v.value
 
+    def to_csr(self, dtype=F64, stream=None):
+        """CSR expansion in device memory (csr_matrix_class(bcsr) entry order)."""
+        h = ctypes.c_void_p()
+        check(lib().bmfp_matrix_to_csr(self.handle, dtype, ctypes.byref(h), stream))
+        return Matrix(h)
+
+    def download_structure(self):
+        """(row_ptr, columns) as int32 arrays."""
+        rp = np.zeros(self.n_rows + 1, np.int32)
+        cols = np.zeros(self.nnzb, np.int32)
+        check(lib().bmfp_matrix_download_structure(self.handle, _ptr(rp), _ptr(cols)))
+        return rp, cols
+
     def download_values(self):
         out = np.empty(self.nnzb * self.bs * self.bs, np.float64)
         check(lib().bmfp_matrix_download_values(self.handle, _ptr(out)))

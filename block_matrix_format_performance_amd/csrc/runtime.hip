@@ -340,6 +340,39 @@ int upload_indices(bmfp_matrix_t &m, const int *row_ptr, const int *columns)
 }
 
 // Converts device F64 values `src` (layout src_layout) into m.values / m.scale.
+// BCSR -> CSR in device memory (csr_matrix_class(const bcsr_matrix_class&),
+// matrix_converters.h:88-117): scalar row br*bs + r lists, block by block in
+// the block row's order, the bs entries (col*bs + c, value of (r, c)).  One
+// thread per scalar row writes its len*bs entries.
+template <class VT>
+__global__ __launch_bounds__(256) void k_bcsr_to_csr(int n_brows, int bs, int layout, const int *__restrict__ rp,
+                                                     const int *__restrict__ cols, const double *__restrict__ vals,
+                                                     int *__restrict__ out_rp, int *__restrict__ out_cols,
+                                                     VT *__restrict__ out_vals)
+{
+  const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  const int64_t rows = (int64_t) n_brows * bs;
+  if (i > rows) return;
+  if (i == rows) {
+    out_rp[rows] = (int) ((int64_t) rp[n_brows] * bs * bs);
+    return;
+  }
+  const int br = (int) (i / bs), r = (int) (i - (int64_t) br * bs);
+  const int b0 = rp[br], len = rp[br + 1] - b0;
+  const int64_t bs2 = (int64_t) bs * bs;
+  const int64_t base = (int64_t) b0 * bs2 + (int64_t) r * len * bs;
+  out_rp[i] = (int) base;
+  for (int k = 0; k < len; k++) {
+    const int64_t b = b0 + k;
+    const int col = cols[b];
+    for (int c = 0; c < bs; c++) {
+      out_cols[base + (int64_t) k * bs + c] = col * bs + c;
+      out_vals[base + (int64_t) k * bs + c] =
+          (VT) vals[b * bs2 + (layout == BMFP_ROW_MAJOR ? (int64_t) r * bs + c : (int64_t) c * bs + r)];
+    }
+  }
+}
+
 int convert_values(bmfp_matrix_t &m, const double *src, int src_layout, hipStream_t s)
 {
   const int64_t nv = m.n_vals();
@@ -545,6 +578,55 @@ extern "C" int bmfp_spmv_traffic(bmfp_matrix m, double *bytes, double *flops)
   if (m->dtype == BMFP_F16) b += 4.0 * m->n_rows;
   if (bytes) *bytes = b;
   if (flops) *flops = 2.0 * nv;
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_to_csr(bmfp_matrix src, int dtype, bmfp_matrix *out, bmfp_stream stream)
+{
+  if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (!src->is_bcsr || src->dtype != BMFP_F64)
+    return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_to_csr needs an F64 BCSR source");
+  if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
+  const int64_t nnz = (int64_t) src->nnzb * src->bs * src->bs, rows = src->y_len();
+  if (nnz > INT_MAX || rows > INT_MAX) return fail(BMFP_ERR_INVALID, "expanded matrix too large for int32");
+  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+  m->is_bcsr = 0;
+  m->dtype = dtype;
+  m->n_rows = (int) rows;
+  m->n_cols = (int) src->x_len();
+  m->bs = 1;
+  m->nnzb = (int) nnz;
+  m->max_row_blocks = src->max_row_blocks * src->bs;
+  m->mean_row_blocks = src->mean_row_blocks * src->bs;
+  m->csr_group = csr_group_for(m->mean_row_blocks);
+  m->device = src->device;
+  m->row_ptr = std::make_shared<DeviceBuffer>();
+  m->columns = std::make_shared<DeviceBuffer>();
+  BMFP_TRY(m->row_ptr->alloc(sizeof(int) * (size_t) (rows + 1)));
+  BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnz));
+  BMFP_TRY(m->values.alloc((dtype == BMFP_F64 ? sizeof(double) : sizeof(float)) * (size_t) nnz));
+  BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(rows, nnz)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned groups = (unsigned) ((rows + 1 + 255) / 256);
+  const int *rp = src->row_ptr->as<int>(), *cols = src->columns->as<int>();
+  if (dtype == BMFP_F64)
+    k_bcsr_to_csr<double><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
+                                                 m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<double>());
+  else
+    k_bcsr_to_csr<float><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
+                                                m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<float>());
+  BMFP_LAUNCHED();
+  BMFP_HIP(hipStreamSynchronize(s));
+  *out = m.release();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_matrix_download_structure(bmfp_matrix m, int *row_ptr, int *columns)
+{
+  if (!m || !row_ptr || (m->nnzb && !columns)) return fail(BMFP_ERR_INVALID, "NULL argument");
+  BMFP_HIP(hipMemcpy(row_ptr, m->row_ptr->ptr, sizeof(int) * ((size_t) m->n_rows + 1), hipMemcpyDeviceToHost));
+  if (m->nnzb) BMFP_HIP(hipMemcpy(columns, m->columns->ptr, sizeof(int) * (size_t) m->nnzb, hipMemcpyDeviceToHost));
   return BMFP_OK;
 }
 

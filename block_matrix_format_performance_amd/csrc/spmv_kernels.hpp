@@ -540,7 +540,9 @@ __global__ __launch_bounds__(256) void k_csr_merge(CsrArgs<VT, AT> a, int nnz, i
                                                    const int *__restrict__ bounds, int *__restrict__ carry_row,
                                                    AT *__restrict__ carry_val)
 {
-  __shared__ AT prod_s[4][MERGE_TILE];
+  // products padded by one slot per 8: lanes walk runs of ~8 consecutive
+  // products, which would otherwise fall on the same LDS banks
+  __shared__ AT prod_s[4][MERGE_TILE + MERGE_TILE / 8];
   __shared__ int ends_s[4][MERGE_TILE + 1];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int n = a.n_rows;
@@ -551,8 +553,35 @@ __global__ __launch_bounds__(256) void k_csr_merge(CsrArgs<VT, AT> a, int nnz, i
   const int64_t d1 = d0 + MERGE_TILE < total ? d0 + MERGE_TILE : total;
   const int i0 = w < n_waves ? bounds[w] : n, i1 = w < n_waves ? bounds[w + 1] : n;
   const int j0 = (int) (d0 - i0), j1 = (int) (d1 - i1);
-  for (int k = lane; k < j1 - j0; k += 64) prod_s[wib][k] = cvt<AT>(a.vals[j0 + k]) * a.x[a.cols[j0 + k]];
-  for (int k = lane; k <= i1 - i0; k += 64) ends_s[wib][k] = i0 + k < n ? a.rp[i0 + k + 1] : nnz;
+  {
+    // all index and value loads first, then all x gathers, then the products:
+    // no load waits on another except each gather on its own index
+    const int cnt = j1 - j0, nends = i1 - i0 + 1;
+    int ci[MERGE_IPT];
+    VT v[MERGE_IPT];
+    AT xv[MERGE_IPT];
+#pragma unroll
+    for (int m = 0; m < MERGE_IPT; ++m) {
+      const int k = lane + 64 * m;
+      if (k < cnt) {
+        ci[m] = a.cols[j0 + k];
+        v[m] = a.vals[j0 + k];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MERGE_IPT + 1; ++m) {
+      const int k = lane + 64 * m;
+      if (k < nends) ends_s[wib][k] = i0 + k < n ? a.rp[i0 + k + 1] : nnz;
+    }
+#pragma unroll
+    for (int m = 0; m < MERGE_IPT; ++m)
+      if (lane + 64 * m < cnt) xv[m] = a.x[ci[m]];
+#pragma unroll
+    for (int m = 0; m < MERGE_IPT; ++m) {
+      const int k = lane + 64 * m;
+      if (k < cnt) prod_s[wib][k + (k >> 3)] = cvt<AT>(v[m]) * xv[m];
+    }
+  }
   __syncthreads();
   if (w >= n_waves) return;
   // this lane's items [dl, de) start at (i, j): search the staged row ends
@@ -569,7 +598,7 @@ __global__ __launch_bounds__(256) void k_csr_merge(CsrArgs<VT, AT> a, int nnz, i
   int first_row = -1;
   for (int64_t t = dl; t < de; ++t) {
     if (j < ends_s[wib][i - i0]) {
-      acc += prod_s[wib][j - j0];
+      acc += prod_s[wib][(j - j0) + ((j - j0) >> 3)];
       ++j;
     } else {
       if (first_row < 0) {

@@ -193,6 +193,13 @@ int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr,
 
 /* Download values (as doubles, in the handle's layout, unscaled). */
 int bmfp_matrix_download_values(bmfp_matrix m, double *values);
+/* Download row pointers (n_rows+1) and (block) column indices (nnzb). */
+int bmfp_matrix_download_structure(bmfp_matrix m, int *row_ptr, int *columns);
+/* CSR expansion of an F64 BCSR matrix in device memory, into F64 or F32
+ * values: replaces csr_matrix_class(const bcsr_matrix_class&)
+ * (matrix_converters.h:88-117) with the same entry order, bit-exact for F64. */
+int bmfp_matrix_to_csr(bmfp_matrix src, int dtype, bmfp_matrix *out,
+                       bmfp_stream stream);
 
 /* ---- SpMV ----------------------------------------------------------------
  * y = A x.  x, y are device vectors of n_cols*bs / n_rows*bs elements in the

@@ -263,8 +263,9 @@ def test_csr_merge_irregular_rows(name, dtype):
 
 
 def test_csr_auto_choice_follows_row_lengths():
-    cases = {bm.CSR_SCALAR: [9] * 500, bm.CSR_VECTOR: [300] * 50, bm.CSR_MERGE: [40] * 50 + [10] * 50}
-    for want, lengths in cases.items():
+    cases = [(bm.CSR_SCALAR, [9] * 500), (bm.CSR_VECTOR, [300] * 50), (bm.CSR_MERGE, [40] * 50 + [10] * 50),
+             (bm.CSR_MERGE, list(range(1, 16)) * 30)]                   # short but ragged
+    for want, lengths in cases:
         c, x = csr_from_lengths(lengths, 4000, seed=want)
         A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
         assert A.auto_variant() == want
@@ -280,6 +281,24 @@ def test_csr_merge_is_bitwise_reproducible():
     y1 = bm.spmv_host(A, x, bm.CSR_MERGE)
     y2 = bm.spmv_host(A, x, bm.CSR_MERGE)
     assert np.array_equal(y1.view(np.uint64), y2.view(np.uint64))
+
+
+@pytest.mark.parametrize("bs,layout", [(1, bm.ROW_MAJOR), (2, bm.ROW_MAJOR), (3, bm.COL_MAJOR), (5, bm.ROW_MAJOR),
+                                       (8, bm.COL_MAJOR)])
+def test_device_csr_expansion_is_bit_exact(bs, layout):
+    m, x = ragged_case(40, 33, bs, max_len=7, seed=bs)
+    vals = m.values if layout == bm.ROW_MAJOR else oracle.transpose_blocks(m)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, vals, m.n_rows, m.n_cols, bs, layout=layout)
+    c = oracle.bcsr_to_csr(m)
+    C = A.to_csr()
+    assert (C.is_bcsr, C.n_rows, C.n_cols, C.nnzb) == (0, c.n_rows, c.n_cols, len(c.columns))
+    rp, cols = C.download_structure()
+    np.testing.assert_array_equal(rp, c.row_ptr)
+    np.testing.assert_array_equal(cols, c.columns)
+    assert np.array_equal(C.download_values().view(np.uint64), c.values.view(np.uint64))
+    assert np.array_equal(A.to_csr(bm.F32).download_values(), c.values.astype(np.float32).astype(np.float64))
+    for v in (bm.SPMV_AUTO, bm.CSR_MERGE):
+        check_csr(c, x, v)
 
 
 def test_matrix_market_file_through_every_path(tmp_path):
