@@ -180,14 +180,17 @@ bool bcsr_supported(int variant, int dtype, int layout, int bs)
 int auto_variant(const bmfp_matrix_t &A)
 {
   if (!A.is_bcsr) {
-    // measured on ~1.4e8-nonzero matrices: thread per row for short rows of
-    // (nearly) equal length, lane groups for long rows, merge-path tiles for
-    // everything else and whenever one row dwarfs the mean (a lane group
-    // would serialise it)
+    // measured on ~1.4e8-nonzero matrices (equal, ragged and power-law rows,
+    // and the CSR expansions of the 9-point block stencils): merge-path tiles
+    // whenever one row dwarfs the mean (a lane group would serialise it) and
+    // for mid-length rows, whose lengths rarely fill whole lane groups; lane
+    // groups for long rows and for short ragged rows; thread per row for
+    // short rows of (nearly) equal length
     const double mean = A.mean_row_blocks;
     if (A.max_row_blocks > 8.0 * mean + 64.0) return BMFP_CSR_MERGE;
     if (mean >= 128.0) return BMFP_CSR_VECTOR;
-    return mean < 12.0 && A.max_row_blocks <= mean + 2.0 ? BMFP_CSR_SCALAR : BMFP_CSR_MERGE;
+    if (mean < 12.0) return A.max_row_blocks <= mean + 2.0 ? BMFP_CSR_SCALAR : BMFP_CSR_VECTOR;
+    return BMFP_CSR_MERGE;
   }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
   // matrices (FP64, ~4.4e8 stored values): batched thread-per-row for small

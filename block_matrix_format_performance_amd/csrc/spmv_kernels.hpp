@@ -464,16 +464,56 @@ struct CsrArgs {
   AT *__restrict__ y;
 };
 
+// Sum over e = e0, e0 + stride, ... < e1 of vals[e] * x[cols[e]], in that
+// order.  Batches of U entries issue their index and value loads together,
+// then their x gathers, so a lane waits on one gather latency per batch
+// instead of one per entry.  TAIL_ONLY: full batches run unpredicated (guards
+// inside a batch serialise its loads) and only the last, partial batch is
+// predicated -- best for a thread walking a whole row; otherwise every batch
+// is predicated -- best for lane groups, whose lanes see a few entries each.
+template <class VT, class AT, int U, bool TAIL_ONLY>
+__device__ __forceinline__ AT csr_dot(const CsrArgs<VT, AT> &a, int e, int e1, int stride)
+{
+  AT acc = 0;
+  int c[U];
+  VT v[U];
+  AT xv[U];
+  if constexpr (TAIL_ONLY) {
+    for (; e + (U - 1) * stride < e1; e += U * stride) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = a.cols[e + u * stride];
+        v[u] = a.vals[e + u * stride];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = a.x[c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = fmadd(cvt<AT>(v[u]), xv[u], acc);
+    }
+  }
+  for (; e < e1; e += U * stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e + u * stride < e1) {
+        c[u] = a.cols[e + u * stride];
+        v[u] = a.vals[e + u * stride];
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e + u * stride < e1) xv[u] = a.x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e + u * stride < e1) acc = fmadd(cvt<AT>(v[u]), xv[u], acc);
+  }
+  return acc;
+}
+
 template <class VT, class AT>
 __global__ __launch_bounds__(256) void k_csr_scalar(CsrArgs<VT, AT> a)
 {
   const int64_t row = tile_id(a.xcd_per) * 256 + threadIdx.x;
   if (row >= a.n_rows) return;
-  const int e0 = a.rp[row], e1 = a.rp[row + 1];
-  AT acc = 0;
-#pragma unroll 4
-  for (int e = e0; e < e1; ++e) acc = fmadd(cvt<AT>(a.vals[e]), a.x[a.cols[e]], acc);
-  a.y[row] = acc;
+  a.y[row] = csr_dot<VT, AT, 8, true>(a, a.rp[row], a.rp[row + 1], 1);
 }
 
 template <class VT, class AT, int G>
@@ -483,10 +523,7 @@ __global__ __launch_bounds__(256) void k_csr_vector(CsrArgs<VT, AT> a)
   const int g = (int) (t % G);
   const int64_t row = t / G;
   if (row >= a.n_rows) return;  // whole groups leave together
-  const int e0 = a.rp[row], e1 = a.rp[row + 1];
-  AT acc = 0;
-#pragma unroll 4
-  for (int e = e0 + g; e < e1; e += G) acc = fmadd(cvt<AT>(a.vals[e]), a.x[a.cols[e]], acc);
+  AT acc = csr_dot<VT, AT, 4, false>(a, a.rp[row] + g, a.rp[row + 1], G);
 #pragma unroll
   for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
   if (g == 0) a.y[row] = acc;

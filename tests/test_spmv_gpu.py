@@ -263,8 +263,8 @@ def test_csr_merge_irregular_rows(name, dtype):
 
 
 def test_csr_auto_choice_follows_row_lengths():
-    cases = [(bm.CSR_SCALAR, [9] * 500), (bm.CSR_VECTOR, [300] * 50), (bm.CSR_MERGE, [40] * 50 + [10] * 50),
-             (bm.CSR_MERGE, list(range(1, 16)) * 30)]                   # short but ragged
+    cases = [(bm.CSR_SCALAR, [9] * 500), (bm.CSR_VECTOR, [300] * 50), (bm.CSR_MERGE, [20] * 100),
+             (bm.CSR_MERGE, [40] * 50 + [10] * 50), (bm.CSR_VECTOR, list(range(1, 16)) * 30)]
     for want, lengths in cases:
         c, x = csr_from_lengths(lengths, 4000, seed=want)
         A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
