@@ -82,7 +82,11 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
   }
   case BMFP_BCSR_GROUP_PER_ROW: {
     constexpr DenseShape D = dense_shape<VT, BS>();
-    if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
+    if constexpr (BS == 2 || BS == 3) {
+      constexpr int RPW = 64 / BS;
+      Grid g = make_grid((nb + RPW - 1) / RPW * 64);
+      k_bcsr_split<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    } else if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
       constexpr int RPW = 64 / (BS / D.vec);
       Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
       k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
@@ -193,13 +197,15 @@ int auto_variant(const bmfp_matrix_t &A)
     return BMFP_CSR_MERGE;
   }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
-  // matrices (FP64, ~4.4e8 stored values): batched thread-per-row for small
-  // blocks; 16-byte lane slices (row-major) or shuffle-shared x segments
-  // (column-major) from bs = 4 up.
+  // matrices (FP64, ~4.4e8 stored values): batched thread per row for bs 1;
+  // whole blocks split over the block row's lanes for bs 2 and 3; 16-byte
+  // lane slices (row-major) or shuffle-shared x segments (column-major) from
+  // bs = 4 up.
   // block sizes without a precompiled instantiation: a kernel specialised by
   // hipRTC (F64 / F32), the runtime-bs thread-per-row kernel for F16
   if (!precompiled_bs(A.bs)) return A.dtype == BMFP_F16 ? BMFP_BCSR_THREAD_PER_ROW : BMFP_BCSR_JIT;
-  if (A.bs <= 3) return BMFP_BCSR_THREAD_PER_ROW;
+  if (A.bs == 1) return BMFP_BCSR_THREAD_PER_ROW;
+  if (A.bs <= 3) return BMFP_BCSR_GROUP_PER_ROW;  // blocks split over the block row's lanes
   return A.layout == BMFP_COL_MAJOR ? BMFP_BCSR_SHUFFLE_X : BMFP_BCSR_GROUP_PER_ROW;
 }
 

@@ -11,6 +11,8 @@
 // families here:
 //   k_bcsr_tpr   thread per scalar row       K4 :275-304, K8 :341-370, K9 :372-400
 //   k_bcsr_gpr   lane group per scalar row   K5 :306-339 (32 fixed lanes there)
+//   k_bcsr_split lane group per block row,   the same family for bs 2 and 3
+//                blocks split over its lanes
 //   k_bcsr_shfx  thread per row, x segment   K10/K11 :402-469 (there: shared memory,
 //                shared by lane shuffles      wrong for bs<32; here: per block row)
 //   k_bcsr_bgr   lane group per block row    K12-K16 :485-673 (there: wrong for
@@ -327,6 +329,75 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
   if (on && g == 0) {
     if constexpr (SCALED) acc *= (AT) a.scale[br];
     a.y[row] = acc;
+  }
+}
+
+// ============================================================================
+// Lane group per block row with the blocks split across the group (bs 2, 3):
+// the BS lanes of a block row each take whole blocks b0+h, b0+h+BS, ... in
+// predicated batches of P (all index, value and x loads of a batch issued
+// together) and accumulate all BS output rows; lane r then sums the group's
+// partials of row r in BS shuffle rounds (round k: lane h offers its partial
+// of row (h - k) mod BS, lane r reads lane (r + k) mod BS), a fixed order.
+// For a 9-block row a lane issues about 9/BS x (block + index + x segment)
+// requests instead of 9 x (block row + index + x segment).
+// ============================================================================
+template <int BS> constexpr int split_batch() { return BS == 2 ? 5 : 3; }
+
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_split(BcsrArgs<VT, AT> a)
+{
+  constexpr int P = split_batch<BS>(), RPW = 64 / BS;
+  constexpr int VALIGN = lowbit((int) sizeof(VT) * BS * BS), XALIGN = lowbit((int) sizeof(AT) * BS);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int q = lane / BS, h = lane - q * BS, base = q * BS;
+  const int64_t br64 = wave * RPW + q;
+  const bool on = lane < RPW * BS && br64 < a.n_brows;
+  const int br = on ? (int) br64 : 0;
+  int b = 0, b1 = 0;
+  if (on) {
+    b = a.rp[br] + h;
+    b1 = a.rp[br + 1];
+  }
+  AT part[BS];
+#pragma unroll
+  for (int r = 0; r < BS; ++r) part[r] = 0;
+  for (; b < b1; b += BS * P) {
+    int c[P];
+    VT v[P][BS * BS];
+    AT xv[P][BS];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) c[u] = a.cols[b + BS * u];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) load_n<VT, BS * BS, VALIGN>(a.vals + (int64_t) (b + BS * u) * (BS * BS), v[u]);
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) load_n<AT, BS, XALIGN>(a.x + (int64_t) c[u] * BS, xv[u]);
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+          for (int cc = 0; cc < BS; ++cc)
+            part[r] = fmadd(cvt<AT>(v[u][LAYOUT == BMFP_ROW_MAJOR ? r * BS + cc : cc * BS + r]), xv[u][cc], part[r]);
+      }
+  }
+  AT acc = 0;
+#pragma unroll
+  for (int k = 0; k < BS; ++k) {
+    const int si = (h - k + BS) % BS;
+    AT send = part[0];
+#pragma unroll
+    for (int r = 1; r < BS; ++r) send = si == r ? part[r] : send;
+    acc += __shfl(send, lane < RPW * BS ? base + (h + k) % BS : lane);
+  }
+  if (on) {
+    if constexpr (SCALED) acc *= (AT) a.scale[br];
+    a.y[(int64_t) br * BS + h] = acc;
   }
 }
 
