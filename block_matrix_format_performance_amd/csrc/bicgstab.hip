@@ -216,16 +216,24 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   BMFP_HIP(hipStreamWaitEvent(s, S.ev_sync, 0));
   BMFP_HIP(hipEventRecord(S.ev0, s));
   BMFP_TRY(S.ops_init(S, b, eps, max_iterations, s));
-  // Launch graph chunks; read the 64-byte state back between batches.  The
-  // batch grows while the solve runs so that the host round trip stays a
-  // small fraction; kernels after the stop are no-ops.
+  // Launch graph chunks of GRAPH_ITERS iterations; read the state back
+  // between batches.  Kernels after the stop are no-ops but still cost a
+  // launch each, so the next batch covers the iterations the residual's
+  // decrease so far predicts are left (doubling while it has not decreased).
   int batch = 1;
   for (;;) {
     for (int k = 0; k < batch; k++) BMFP_HIP(hipGraphLaunch(S.graph, s));
     BMFP_HIP(hipMemcpyAsync(S.h_state, S.state.ptr, sizeof(SolverState), hipMemcpyDeviceToHost, s));
     BMFP_HIP(hipStreamSynchronize(s));
-    if (S.h_state->done) break;
+    const SolverState &st = *S.h_state;
+    if (st.done) break;
     batch = std::min(batch * 2, 8);
+    const double r0 = std::sqrt(st.norm_b2), rk = std::sqrt(st.rr), goal = st.eps * r0;
+    if (st.iter > 0 && rk > 0 && rk < r0 && goal > 0) {
+      const double per_iter = std::log(rk / r0) / st.iter;  // < 0
+      const double left = std::log(goal / rk) / per_iter;
+      batch = (int) std::min(8.0, std::max(1.0, std::ceil(left / GRAPH_ITERS)));
+    }
   }
   const size_t vb = vec_size(S.A->dtype);
   BMFP_HIP(hipMemcpyAsync(x, S.vecs.ptr, (size_t) S.n * vb, hipMemcpyDeviceToDevice, s));
