@@ -11,8 +11,11 @@
 // families here:
 //   k_bcsr_tpr   thread per scalar row       K4 :275-304, K8 :341-370, K9 :372-400
 //   k_bcsr_gpr   lane group per scalar row   K5 :306-339 (32 fixed lanes there)
-//   k_bcsr_split lane group per block row,   the same family for bs 2 and 3
+//   k_bcsr_stage lane group per block row,   the same family for bs 1-3
+//                wavefront's blocks in LDS,
 //                blocks split over its lanes
+//   k_bcsr_split as k_bcsr_stage without     the same family (blocks of a
+//                the LDS copy                size that is not 8-byte aligned)
 //   k_bcsr_shfx  thread per row, x segment   K10/K11 :402-469 (there: shared memory,
 //                shared by lane shuffles      wrong for bs<32; here: per block row)
 //   k_bcsr_bgr   lane group per block row    K12-K16 :485-673 (there: wrong for
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 // For a 9-block row a lane issues about 9/BS x (block + index + x segment)
 // requests instead of 9 x (block row + index + x segment).
 // ============================================================================
-template <int BS> constexpr int split_batch() { return BS == 2 ? 5 : 3; }
+template <int BS> constexpr int split_batch() { return BS == 1 ? 9 : BS == 2 ? 5 : 3; }
 
 template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
 __global__ __launch_bounds__(256) void k_bcsr_split(BcsrArgs<VT, AT> a)
@@ -385,6 +388,158 @@ __global__ __launch_bounds__(256) void k_bcsr_split(BcsrArgs<VT, AT> a)
           for (int cc = 0; cc < BS; ++cc)
             part[r] = fmadd(cvt<AT>(v[u][LAYOUT == BMFP_ROW_MAJOR ? r * BS + cc : cc * BS + r]), xv[u][cc], part[r]);
       }
+  }
+  AT acc = 0;
+#pragma unroll
+  for (int k = 0; k < BS; ++k) {
+    const int si = (h - k + BS) % BS;
+    AT send = part[0];
+#pragma unroll
+    for (int r = 1; r < BS; ++r) send = si == r ? part[r] : send;
+    acc += __shfl(send, lane < RPW * BS ? base + (h + k) % BS : lane);
+  }
+  if (on) {
+    if constexpr (SCALED) acc *= (AT) a.scale[br];
+    a.y[(int64_t) br * BS + h] = acc;
+  }
+}
+
+// ============================================================================
+// k_bcsr_split with the wavefront's blocks staged in LDS first.  The values
+// and column indices of the wavefront's RPW block rows are one contiguous
+// range; it is copied to LDS with 16-byte loads at consecutive addresses
+// across the lanes (one load instruction moves 1 KB), then each lane reads
+// its whole blocks from LDS and gathers x from global memory as in
+// k_bcsr_split.  A wavefront whose range exceeds CAP blocks (rows longer than
+// 9 blocks on average) reads global memory directly.  No workgroup barrier:
+// each wavefront stages and reads only its own LDS slice.  For bs = 1 this is
+// thread per row (RPW = 64, P = 9: a whole 9-point row in one batch).
+// ============================================================================
+template <int BS> constexpr int stage_cap() { return 9 * (64 / BS); }
+
+// LDS operations of one wavefront complete in issue order, so a compiler
+// barrier is all that lanes need to read what other lanes of the same
+// wavefront wrote.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Accumulates blocks b, b + BS, ... < b1 of one block row into part[] in
+// predicated batches of P: col_at(k) and block_at(k, v) fetch block k's column
+// and values.  X_FIRST issues the x gathers before the block loads (blocks in
+// LDS: the gathers are the long-latency loads); otherwise the global block
+// loads go out first, before the gathers wait on their column indices.
+template <class VT, class AT, int BS, int LAYOUT, int P, bool X_FIRST, class ColF, class BlkF>
+__device__ __forceinline__ void split_accumulate(int b, int b1, const AT *__restrict__ x, AT (&part)[BS],
+                                                 ColF col_at, BlkF block_at)
+{
+  constexpr int XALIGN = lowbit((int) sizeof(AT) * BS);
+  for (; b < b1; b += BS * P) {
+    int c[P];
+    VT v[P][BS * BS];
+    AT xv[P][BS];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) c[u] = col_at(b + BS * u);
+    if constexpr (!X_FIRST) {
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+        if (b + BS * u < b1) block_at(b + BS * u, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) load_n<AT, BS, XALIGN>(x + (int64_t) c[u] * BS, xv[u]);
+    if constexpr (X_FIRST) {
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+        if (b + BS * u < b1) block_at(b + BS * u, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (b + BS * u < b1) {
+#pragma unroll
+        for (int r = 0; r < BS; ++r)
+#pragma unroll
+          for (int cc = 0; cc < BS; ++cc)
+            part[r] = fmadd(cvt<AT>(v[u][LAYOUT == BMFP_ROW_MAJOR ? r * BS + cc : cc * BS + r]), xv[u][cc], part[r]);
+      }
+  }
+}
+
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_bcsr_stage(BcsrArgs<VT, AT> a)
+{
+  constexpr int P = split_batch<BS>(), RPW = 64 / BS, CAP = stage_cap<BS>();
+  constexpr int BB = BS * BS * (int) sizeof(VT);       // bytes per block
+  static_assert(BB % 8 == 0, "staging copies 16-byte chunks with an 8-byte tail");
+  constexpr int SLACK = BB % 16 == 0 ? 0 : 16;         // range start rounded down to 16 bytes
+  constexpr int VBYTES = (CAP * BB + SLACK + 15) / 16 * 16;
+  constexpr int NCH = (VBYTES / 16 + 63) / 64, NCOL = (CAP + 63) / 64;
+  constexpr int VALIGN = lowbit(BB);
+  __shared__ __align__(16) char vals_s[WPG][VBYTES];
+  __shared__ int cols_s[WPG][CAP];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int64_t r0_64 = ((int64_t) blockIdx.x * WPG + wib) * RPW;
+  if (r0_64 >= a.n_brows) return;  // the whole wavefront leaves; no barrier follows
+  const int r0 = (int) r0_64, r1 = r0 + RPW < a.n_brows ? r0 + RPW : a.n_brows;
+  const int B0 = a.rp[r0], B1 = a.rp[r1];
+  const int q = lane / BS, h = lane - q * BS, base = q * BS;
+  const bool on = lane < RPW * BS && r0 + q < r1;
+  const int br = on ? r0 + q : r0;
+  int b = 0, b1 = 0;
+  if (on) {
+    b = a.rp[br] + h;
+    b1 = a.rp[br + 1];
+  }
+  AT part[BS];
+#pragma unroll
+  for (int r = 0; r < BS; ++r) part[r] = 0;
+  if (B1 - B0 <= CAP) {
+    const char *vbase = reinterpret_cast<const char *>(a.vals);
+    const int64_t s_byte = (int64_t) B0 * BB;
+    const int64_t s_al = s_byte & ~(int64_t) 15;           // values start 256-byte aligned
+    const int shift = SLACK ? (int) (s_byte - s_al) : 0;
+    const int e_rel = (int) ((int64_t) B1 * BB - s_al);    // staged bytes, a multiple of 8
+    const int nb = B1 - B0;
+    u32x4 tv[NCH];
+    int tc[NCOL];
+#pragma unroll
+    for (int m = 0; m < NCH; ++m) {
+      const int k = 16 * (lane + 64 * m);
+      if (k < e_rel) {
+        if (SLACK == 0 || k + 16 <= e_rel) {
+          tv[m] = *reinterpret_cast<const u32x4 *>(vbase + s_al + k);
+        } else {  // the last 8 bytes of the range
+          const u32x2 t = *reinterpret_cast<const u32x2 *>(vbase + s_al + k);
+          tv[m] = u32x4{t.x, t.y, 0u, 0u};
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NCOL; ++m)
+      if (lane + 64 * m < nb) tc[m] = a.cols[B0 + lane + 64 * m];
+#pragma unroll
+    for (int m = 0; m < NCH; ++m) {
+      const int k = 16 * (lane + 64 * m);
+      if (k < e_rel) *reinterpret_cast<u32x4 *>(&vals_s[wib][k]) = tv[m];
+    }
+#pragma unroll
+    for (int m = 0; m < NCOL; ++m)
+      if (lane + 64 * m < nb) cols_s[wib][lane + 64 * m] = tc[m];
+    wave_lds_sync();
+    const char *vs = &vals_s[wib][shift];
+    split_accumulate<VT, AT, BS, LAYOUT, P, true>(
+        b, b1, a.x, part, [&](int k) { return cols_s[wib][k - B0]; },
+        [&](int k, VT(&v)[BS * BS]) {
+          load_n<VT, BS * BS, VALIGN>(reinterpret_cast<const VT *>(vs + (k - B0) * BB), v);
+        });
+  } else {
+    split_accumulate<VT, AT, BS, LAYOUT, P, false>(
+        b, b1, a.x, part, [&](int k) { return a.cols[k]; },
+        [&](int k, VT(&v)[BS * BS]) { load_n<VT, BS * BS, VALIGN>(a.vals + (int64_t) k * (BS * BS), v); });
   }
   AT acc = 0;
 #pragma unroll

@@ -84,12 +84,33 @@ def test_bcsr_f64_stencil_all_variants(bs, layout):
     check_bcsr(m, x, layout, bm.SPMV_AUTO)
 
 
-@pytest.mark.parametrize("bs", [1, 3, 4, 8, 24])
+@pytest.mark.parametrize("bs", [1, 2, 3, 4, 8, 24])
 @pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
 def test_bcsr_f64_ragged_empty_rows(bs, layout):
     m, x = ragged_case(97, 83, bs, max_len=40, seed=100 + bs)
     for v in (VARIANTS_ROW if layout == bm.ROW_MAJOR else VARIANTS_COL):
         check_bcsr(m, x, layout, v)
+
+
+@pytest.mark.parametrize("bs", [1, 2, 3])
+@pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
+def test_bcsr_small_blocks_short_and_long_row_runs(bs, layout):
+    # wavefronts of 64/bs block rows: runs of short rows (at most 9 blocks
+    # each) alternate with runs holding a few long rows, so some wavefronts
+    # stage their blocks in LDS and others read them directly
+    rng = np.random.default_rng(40 + bs)
+    n_rows, n_cols = 700, 650
+    rp, cols = [0], []
+    for r in range(n_rows):
+        long_run = (r // (64 // bs)) % 3 == 1
+        ln = int(rng.integers(30, 120)) if long_run and r % 7 == 0 else int(rng.integers(0, 10))
+        cols.extend(sorted(rng.choice(n_cols, ln, replace=False).tolist()))
+        rp.append(len(cols))
+    m = oracle.BCSR(n_rows, n_cols, bs, np.array(rp), np.array(cols, dtype=np.int32),
+                    rng.uniform(-1, 1, len(cols) * bs * bs))
+    x = rng.uniform(0, 1, n_cols * bs)
+    check_bcsr(m, x, layout, bm.BCSR_GROUP_PER_ROW)
+    check_bcsr(m, x, layout, bm.SPMV_AUTO)
 
 
 @pytest.mark.parametrize("bs", [1, 4, 16])
@@ -260,6 +281,24 @@ def test_csr_merge_irregular_rows(name, dtype):
         lengths = rng.integers(0, 6, 20000)
     c, x = csr_from_lengths(lengths, 5000, seed=len(name) + 1)
     check_csr(c, x, bm.CSR_MERGE, dtype=dtype)
+
+
+def test_bcsr_auto_choice_small_blocks():
+    # bs 1-3: LDS-staged lane groups; bs 1 only while rows average <= 9 blocks
+    for bs, layout, want in [(1, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW), (2, bm.COL_MAJOR, bm.BCSR_GROUP_PER_ROW),
+                             (3, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW), (4, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW),
+                             (4, bm.COL_MAJOR, bm.BCSR_SHUFFLE_X)]:
+        m, x = stencil_case(9, 7, bs, kind=0, seed=bs)
+        vals = m.values if layout == bm.ROW_MAJOR else oracle.transpose_blocks(m)
+        A = bm.Matrix.bcsr(m.row_ptr, m.columns, vals, m.n_rows, m.n_cols, bs, layout=layout)
+        assert A.auto_variant() == want, (bs, layout)
+    m, x = stencil_case(9, 7, 1, kind=0, seed=1)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1, dtype=bm.F32)
+    assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW      # 4-byte blocks are not staged
+    m, x = ragged_case(200, 300, 1, max_len=40, seed=5, empty_every=0)   # mean ~20 blocks per row
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1)
+    assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW
+    check_bcsr(m, x, bm.ROW_MAJOR, bm.SPMV_AUTO)
 
 
 def test_csr_auto_choice_follows_row_lengths():
