@@ -37,6 +37,9 @@ HBM_PEAK_GBS = 8000.0
 # sweep matrices; profiles/spmv_hbm_traffic_pmc.md gives the runs and commands.
 PMC_TRAFFIC_BYTES = {
     ("hbm", 1, "thread_per_row"): 7.263e9,
+    ("hbm", 1, "group_per_row"): 7.259e9,
+    ("hbm", 2, "group_per_row"): 5.009e9,
+    ("hbm", 3, "group_per_row"): 4.197e9,
     ("hbm", 4, "group_per_row"): 4.134e9,
     ("hbm", 32, "group_per_row"): 3.639e9,
 }
