@@ -68,7 +68,8 @@ def build_case(bs, side, layouts):
 
 
 def measure_copy(nbytes, reps, stream):
-    """Plain streaming copy kernel: the measured-bandwidth yardstick."""
+    """Streaming copy kernel (nbytes read + nbytes written) and read-only
+    kernel (nbytes read): the measured-bandwidth yardsticks."""
     nbytes = int(nbytes) // 16 * 16
     src = bm.DeviceArray(nbytes // 8)
     dst = bm.DeviceArray(nbytes // 8)
@@ -82,9 +83,17 @@ def measure_copy(nbytes, reps, stream):
         bm.check(L.bmfp_stream_copy(dst.ptr, src.ptr, nbytes, stream.handle))
     e1.record(stream.handle)
     t = e1.elapsed_since(e0) / reps
+    for _ in range(3):
+        bm.check(L.bmfp_stream_read(src.ptr, nbytes, dst.ptr, stream.handle))
+    e0.record(stream.handle)
+    for _ in range(reps):
+        bm.check(L.bmfp_stream_read(src.ptr, nbytes, dst.ptr, stream.handle))
+    e1.record(stream.handle)
+    tr = e1.elapsed_since(e0) / reps
     src.free()
     dst.free()
-    return {"bytes_moved": 2 * nbytes, "us": round(t * 1e6, 2), "GB/s": round(2 * nbytes / t / 1e9, 1)}
+    return {"bytes_moved": 2 * nbytes, "us": round(t * 1e6, 2), "GB/s": round(2 * nbytes / t / 1e9, 1),
+            "read_only": {"bytes_read": nbytes, "us": round(tr * 1e6, 2), "GB/s": round(nbytes / tr / 1e9, 1)}}
 
 
 def cpu_baseline(seconds):

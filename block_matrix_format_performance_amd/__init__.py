@@ -96,6 +96,7 @@ def _declare(L):
     L.bmfp_event_record.argtypes = [vp, vp]
     L.bmfp_event_elapsed.argtypes = [vp, vp, dp]
     L.bmfp_stream_copy.argtypes = [vp, vp, sz, vp]
+    L.bmfp_stream_read.argtypes = [vp, sz, vp, vp]
     L.bmfp_fill.argtypes = [vp, c_int, sz, c_int, ctypes.c_uint64, ctypes.c_uint64, c_dbl, vp]
     L.bmfp_gen_n_diag_bcsr.argtypes = [c_int, c_int, c_int, vp, vp, vp]
     L.bmfp_stencil_nnzb.restype = ctypes.c_int64

@@ -51,11 +51,46 @@ __global__ __launch_bounds__(256) void k_fill(T *dst, int64_t n, int kind, uint6
 
 // ---- streaming copy (bandwidth yardstick) -------------------------------------
 typedef unsigned int u32x4c __attribute__((ext_vector_type(4)));
+// Grid-stride loops with STREAM_U 16-byte loads in flight per lane (all
+// issued before the first is used); consecutive lanes, consecutive addresses.
+constexpr int STREAM_U = 8;
+
 __global__ __launch_bounds__(256) void k_stream_copy(u32x4c *__restrict__ dst, const u32x4c *__restrict__ src,
                                                      int64_t n16)
 {
   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (STREAM_U - 1) * stride < n16; i += STREAM_U * stride) {
+    u32x4c v[STREAM_U];
+#pragma unroll
+    for (int k = 0; k < STREAM_U; ++k) v[k] = src[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < STREAM_U; ++k) dst[i + k * stride] = v[k];
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+// Reads every 16-byte chunk once and folds it into a per-lane XOR; the fold
+// is written only if it equals a fixed constant (a store the compiler cannot
+// drop, which practically never happens).
+__global__ __launch_bounds__(256) void k_stream_read(const u32x4c *__restrict__ src, int64_t n16,
+                                                     unsigned *__restrict__ sink)
+{
+  const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+  int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned acc = 0;
+  for (; i + (STREAM_U - 1) * stride < n16; i += STREAM_U * stride) {
+    u32x4c v[STREAM_U];
+#pragma unroll
+    for (int k = 0; k < STREAM_U; ++k) v[k] = src[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < STREAM_U; ++k) acc ^= v[k].x ^ v[k].y ^ v[k].z ^ v[k].w;
+  }
+  for (; i < n16; i += stride) {
+    const u32x4c v = src[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (acc == 0x9e3779b9u) sink[0] = acc;
 }
 
 // ---- value conversion (F64 -> F64/F32/F16, optional block transpose) --------
@@ -279,6 +314,19 @@ extern "C" int bmfp_stream_copy(void *dst, const void *src, size_t bytes, bmfp_s
   const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
   k_stream_copy<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<u32x4c *>(dst),
                                                                         static_cast<const u32x4c *>(src), n16);
+  BMFP_LAUNCHED();
+  return BMFP_OK;
+}
+
+extern "C" int bmfp_stream_read(const void *src, size_t bytes, void *sink, bmfp_stream stream)
+{
+  if (bytes == 0) return BMFP_OK;
+  if (!src || !sink || bytes % 16 || (uintptr_t) src % 16 || (uintptr_t) sink % 4)
+    return fail(BMFP_ERR_INVALID, "bmfp_stream_read: needs a 16-byte aligned source and size and a sink");
+  const int64_t n16 = (int64_t) (bytes / 16);
+  const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
+  k_stream_read<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const u32x4c *>(src), n16,
+                                                                        static_cast<unsigned *>(sink));
   BMFP_LAUNCHED();
   return BMFP_OK;
 }

@@ -80,8 +80,9 @@ void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 {
   constexpr int RPW = 64 / BS, WPG = stage_wpg<BS>();
   const int64_t waves = std::max<int64_t>((A.n_rows + RPW - 1) / RPW, 1);
-  k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG><<<(unsigned) ((waves + WPG - 1) / WPG), 64 * WPG, 0, s>>>(
-      bcsr_args<VT, AT, SCALED>(A, x, y, 0));
+  const Grid g = make_grid((waves + WPG - 1) / WPG * 256);  // one group per WPG wavefronts
+  k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG><<<g.groups, 64 * WPG, 0, s>>>(
+      bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
 template <class VT, class AT, bool SCALED, int BS, int LAYOUT>

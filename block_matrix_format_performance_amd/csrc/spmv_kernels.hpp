@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64 * WPG) void k_bcsr_stage(BcsrArgs<VT, AT> a)
   __shared__ __align__(16) char vals_s[WPG][VBYTES];
   __shared__ int cols_s[WPG][CAP];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
-  const int64_t r0_64 = ((int64_t) blockIdx.x * WPG + wib) * RPW;
+  const int64_t r0_64 = (tile_id(a.xcd_per) * WPG + wib) * RPW;
   if (r0_64 >= a.n_brows) return;  // the whole wavefront leaves; no barrier follows
   const int r0 = (int) r0_64, r1 = r0 + RPW < a.n_brows ? r0 + RPW : a.n_brows;
   const int B0 = a.rp[r0], B1 = a.rp[r1];

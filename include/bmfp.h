@@ -112,6 +112,11 @@ int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *seconds);
  * the DMA engines): the measured-bandwidth yardstick for the SpMV kernels.
  * bytes must be a multiple of 16 and both pointers 16-byte aligned. */
 int bmfp_stream_copy(void *dst, const void *src, size_t bytes, bmfp_stream stream);
+/* Reads `bytes` of device memory once with the same streaming loads and
+ * writes nothing but (practically never) one word to `sink` (device, 4-byte
+ * aligned): the read-only bandwidth yardstick.  No counterpart in the
+ * reference; it states what an HBM-read-bound kernel can reach on the GPU. */
+int bmfp_stream_read(const void *src, size_t bytes, void *sink, bmfp_stream stream);
 /* Deterministic device fill with the library's counter-based RNG:
  * kind 0 uniform[0,1), kind 1 uniform[-1,1), kind 2 constant `value`.
  * Element i of a vector filled with (seed, stream_id) is identical to the

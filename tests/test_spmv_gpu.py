@@ -411,3 +411,24 @@ def test_bcsr_jit_unsupported_for_f16():
     with pytest.raises(bm.BmfpError) as e:
         bm.spmv_host(A, x, bm.BCSR_JIT)
     assert e.value.status == bm.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("n16", [1, 255, 256 * 8 * 8192 + 37])
+def test_stream_yardsticks(n16):
+    # the copy kernel's unrolled loop and its tail both land every chunk; the
+    # read kernel reads and leaves its sink alone (its fold of these values
+    # is not the sentinel); misaligned sizes are rejected
+    L = bm.lib()
+    src = bm.DeviceArray.from_host(np.random.default_rng(n16).uniform(-1, 1, 2 * n16))
+    dst = bm.DeviceArray(2 * n16)
+    dst.zero()
+    bm.check(L.bmfp_stream_copy(dst.ptr, src.ptr, 16 * n16, None))
+    bm.synchronize()
+    assert np.array_equal(dst.download(), src.download())
+    sink = bm.DeviceArray(2)
+    sink.upload(np.array([7.0, 7.0]))
+    bm.check(L.bmfp_stream_read(src.ptr, 16 * n16, sink.ptr, None))
+    bm.synchronize()
+    assert np.array_equal(sink.download(), [7.0, 7.0])
+    assert L.bmfp_stream_read(src.ptr, 16 * n16 + 8, sink.ptr, None) == bm.ERR_INVALID
+    assert L.bmfp_stream_copy(dst.ptr, src.ptr, 8, None) == bm.ERR_INVALID
