@@ -204,14 +204,16 @@ int auto_variant(const bmfp_matrix_t &A)
     // measured on ~1.4e8-nonzero matrices (equal, ragged and power-law rows,
     // and the CSR expansions of the 9-point block stencils): merge-path tiles
     // whenever one row dwarfs the mean (a lane group would serialise it) and
-    // for mid-length rows, whose lengths rarely fill whole lane groups; lane
-    // groups for long rows and for short ragged rows; thread per row for
-    // short rows of (nearly) equal length
-    const double mean = A.mean_row_blocks;
-    if (A.max_row_blocks > 8.0 * mean + 64.0) return BMFP_CSR_MERGE;
+    // for ragged mid-length rows, whose lengths leave lanes of a group idle;
+    // LDS-staged lane groups for near-equal mid-length rows; lane groups for
+    // long rows and for short ragged rows; thread per row for short rows of
+    // (nearly) equal length and for short rows that fill a staged wavefront
+    const double mean = A.mean_row_blocks, mx = A.max_row_blocks;
+    if (mx > 8.0 * mean + 64.0) return BMFP_CSR_MERGE;
     if (mean >= 128.0) return BMFP_CSR_VECTOR;
-    if (mean < 12.0) return A.max_row_blocks <= mean + 2.0 ? BMFP_CSR_SCALAR : BMFP_CSR_VECTOR;
-    return BMFP_CSR_MERGE;
+    if (mean < 12.0)
+      return mx <= mean + 2.0 || csr_stage_group(mean, 1) == 1 ? BMFP_CSR_SCALAR : BMFP_CSR_VECTOR;
+    return mx <= 1.25 * mean + 2.0 ? BMFP_CSR_VECTOR : BMFP_CSR_MERGE;
   }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
   // matrices (FP64, ~4.4e8 stored values): for bs 1-3 the wavefront's blocks
@@ -252,6 +254,31 @@ int csr_group_for(double mean_len)
   return g;
 }
 
+// Lanes per row of k_csr_stage: the smallest power of two (>= lo) whose
+// wavefront of 64/G rows holds at most 576 nonzeros on average (9/10 of the
+// staging capacity, room for rows longer than the mean).  0 where staging
+// was measured not to pay: under 288 nonzeros per wavefront (rows of 4 with
+// one lane each: 4.7 vs 5.3 TB/s) or 32+ lanes per row (rows of 256: 5.0 vs
+// 5.6 TB/s for the lane-group kernel).
+int csr_stage_group(double mean_len, int lo)
+{
+  constexpr double FILL = 0.9 * CSR_STAGE_CAP;
+  int g = lo;
+  while (g < 64 && (64.0 / g) * mean_len > FILL) g *= 2;
+  const double fill = (64.0 / g) * mean_len;
+  return fill <= FILL && fill >= 0.45 * CSR_STAGE_CAP && g <= 16 ? g : 0;
+}
+
+template <class VT, class AT, int G>
+void launch_csr_stage(const bmfp_matrix_t &A, CsrArgs<VT, AT> a, hipStream_t s)
+{
+  constexpr int WPG = 2, RPW = 64 / G;
+  const int64_t waves = std::max<int64_t>((A.n_rows + RPW - 1) / RPW, 1);
+  const Grid g = make_grid((waves + WPG - 1) / WPG * 256);  // one group per WPG wavefronts
+  a.xcd_per = g.per;
+  k_csr_stage<VT, AT, G, WPG><<<g.groups, 64 * WPG, 0, s>>>(a);
+}
+
 template <class VT, class AT>
 int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
 {
@@ -263,9 +290,20 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
   a.x = static_cast<const AT *>(x);
   a.y = static_cast<AT *>(y);
   if (variant == BMFP_CSR_SCALAR) {
-    Grid g = make_grid(A.n_rows);
-    a.xcd_per = g.per;
-    k_csr_scalar<VT, AT><<<g.groups, 256, 0, s>>>(a);
+    if (csr_stage_group(A.mean_row_blocks, 1) == 1) {
+      launch_csr_stage<VT, AT, 1>(A, a, s);
+    } else {
+      Grid g = make_grid(A.n_rows);
+      a.xcd_per = g.per;
+      k_csr_scalar<VT, AT><<<g.groups, 256, 0, s>>>(a);
+    }
+  } else if (variant == BMFP_CSR_VECTOR && csr_stage_group(A.mean_row_blocks, 2) > 0) {
+    switch (csr_stage_group(A.mean_row_blocks, 2)) {
+    case 2: launch_csr_stage<VT, AT, 2>(A, a, s); break;
+    case 4: launch_csr_stage<VT, AT, 4>(A, a, s); break;
+    case 8: launch_csr_stage<VT, AT, 8>(A, a, s); break;
+    default: launch_csr_stage<VT, AT, 16>(A, a, s); break;
+    }
   } else if (variant == BMFP_CSR_VECTOR) {
     const int G = A.csr_group;
     Grid g = make_grid((int64_t) A.n_rows * G);

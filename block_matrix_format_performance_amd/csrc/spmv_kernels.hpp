@@ -22,6 +22,9 @@
 //                (column-major streaming)    bs in {3,5,6,9,10}; here: exact)
 //   k_csr_scalar thread per row              K2 :65-86
 //   k_csr_vector lane group per row          K3 :165-196
+//   k_csr_stage  thread / lane group per     K2 and K3 where a wavefront's
+//                row, wavefront's entries    rows fill its LDS slice
+//                in LDS
 //   k_csr_merge  merge-path tiles            (no kernel in the tree; the
 //                                             "GPU Merge SpMV" rows of
 //                                             results/bs_range.json)
@@ -753,6 +756,98 @@ __global__ __launch_bounds__(256) void k_csr_vector(CsrArgs<VT, AT> a)
 #pragma unroll
   for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
   if (g == 0) a.y[row] = acc;
+}
+
+// ============================================================================
+// CSR rows staged per wavefront (the CSR counterpart of k_bcsr_stage): RPW =
+// 64/G consecutive rows, G lanes per row.  The wavefront's nonzeros are one
+// contiguous range of values and of column indices; both are copied to the
+// wavefront's LDS slice with 16-byte loads at consecutive addresses, then
+// lane g of a row sums its entries g, g + G, ... from LDS (U x gathers from
+// global memory in flight) and the group adds its lanes' sums by xor
+// shuffles (a fixed order).  Ranges over CSR_STAGE_CAP nonzeros read global
+// memory directly (csr_dot).  No workgroup barrier.
+// ============================================================================
+constexpr int CSR_STAGE_CAP = 640;
+
+// Copies bytes [s, e) of the buffer at base (16-byte aligned) to LDS at dst:
+// the start rounded down to 16 bytes (up to 12 bytes before s come along),
+// 16 bytes per lane per step, the last partial chunk 4 bytes at a time (s and
+// e are multiples of 4; e - s <= CAPB).  Returns s's offset in dst.
+template <int CAPB>
+__device__ __forceinline__ int stage_range(const char *__restrict__ base, int64_t s, int64_t e, char *dst, int lane)
+{
+  constexpr int NCH = ((CAPB + 16) / 16 + 63) / 64;
+  const int64_t s_al = s & ~(int64_t) 15;
+  const int n = (int) (e - s_al);
+  u32x4 t[NCH];
+#pragma unroll
+  for (int m = 0; m < NCH; ++m) {
+    const int k = 16 * (lane + 64 * m);
+    if (k < n) {
+      if (k + 16 <= n) {
+        t[m] = *reinterpret_cast<const u32x4 *>(base + s_al + k);
+      } else {
+        const unsigned *w = reinterpret_cast<const unsigned *>(base + s_al + k);
+        t[m] = u32x4{w[0], n - k > 4 ? w[1] : 0u, n - k > 8 ? w[2] : 0u, 0u};
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NCH; ++m) {
+    const int k = 16 * (lane + 64 * m);
+    if (k < n) *reinterpret_cast<u32x4 *>(dst + k) = t[m];
+  }
+  return (int) (s - s_al);
+}
+
+template <class VT, class AT, int G, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_csr_stage(CsrArgs<VT, AT> a)
+{
+  constexpr int RPW = 64 / G, CAP = CSR_STAGE_CAP, U = G == 1 ? 8 : 4;
+  constexpr int VCAP = CAP * (int) sizeof(VT), CCAP = CAP * 4;
+  __shared__ __align__(16) char vals_s[WPG][VCAP + 16];
+  __shared__ __align__(16) char cols_s[WPG][CCAP + 16];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int64_t r0_64 = (tile_id(a.xcd_per) * WPG + wib) * RPW;
+  if (r0_64 >= a.n_rows) return;  // the whole wavefront leaves; no barrier follows
+  const int r0 = (int) r0_64, r1 = r0 + RPW < a.n_rows ? r0 + RPW : a.n_rows;
+  const int J0 = a.rp[r0], J1 = a.rp[r1];
+  const int q = lane / G, g = lane - q * G, row = r0 + q;
+  const bool on = row < r1;
+  int e = 0, e1 = 0;
+  if (on) {
+    e = a.rp[row] + g;
+    e1 = a.rp[row + 1];
+  }
+  AT acc = 0;
+  if (J1 - J0 <= CAP) {
+    const int vsh = stage_range<VCAP>(reinterpret_cast<const char *>(a.vals), (int64_t) J0 * (int) sizeof(VT),
+                                      (int64_t) J1 * (int) sizeof(VT), vals_s[wib], lane);
+    const int csh = stage_range<CCAP>(reinterpret_cast<const char *>(a.cols), (int64_t) J0 * 4, (int64_t) J1 * 4,
+                                      cols_s[wib], lane);
+    wave_lds_sync();
+    const VT *vs = reinterpret_cast<const VT *>(vals_s[wib] + vsh);
+    const int *cs = reinterpret_cast<const int *>(cols_s[wib] + csh);
+    for (; e < e1; e += U * G) {
+      int c[U];
+      AT xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u * G < e1) c[u] = cs[e + u * G - J0];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u * G < e1) xv[u] = a.x[c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u * G < e1) acc = fmadd(cvt<AT>(vs[e + u * G - J0]), xv[u], acc);
+    }
+  } else {
+    acc = csr_dot<VT, AT, U, G == 1>(a, e, e1, G);
+  }
+#pragma unroll
+  for (int off = G / 2; off >= 1; off /= 2) acc += __shfl_xor(acc, off, G);
+  if (on && g == 0) a.y[row] = acc;
 }
 
 // ============================================================================

@@ -226,6 +226,7 @@ def test_csr_stencil_expansion(bs):
     c = oracle.bcsr_to_csr(m)
     for v in (bm.CSR_SCALAR, bm.CSR_VECTOR, bm.SPMV_AUTO):
         check_csr(c, x, v)
+    check_csr(c, x, bm.CSR_SCALAR, dtype=bm.F32)
     check_csr(c, x, bm.CSR_VECTOR, dtype=bm.F32)
 
 
@@ -249,6 +250,19 @@ def csr_from_lengths(lengths, n_cols, seed):
     vals = rng.uniform(-1, 1, int(rp[-1]))
     x = rng.uniform(0, 1, n_cols)
     return oracle.CSR(len(lengths), n_cols, rp, cols, vals), x
+
+
+@pytest.mark.parametrize("dtype", [bm.F64, bm.F32])
+@pytest.mark.parametrize("n,mean_short,long_len", [(2000, 3, 700), (2000, 20, 2000), (2000, 300, 1500)])
+def test_csr_staged_and_direct_wavefronts(n, mean_short, long_len, dtype):
+    # short rows keep most wavefronts within the LDS staging capacity; a few
+    # long rows push theirs beyond it, and those read global memory directly
+    rng = np.random.default_rng(mean_short)
+    lengths = rng.integers(0, 2 * mean_short + 1, n)
+    lengths[rng.choice(n, 6, replace=False)] = long_len
+    c, x = csr_from_lengths(lengths, 3000, seed=mean_short)
+    for v in (bm.CSR_SCALAR, bm.CSR_VECTOR):
+        check_csr(c, x, v, dtype=dtype)
 
 
 @pytest.mark.parametrize("bs", [1, 2, 4, 8, 32])
@@ -302,8 +316,10 @@ def test_bcsr_auto_choice_small_blocks():
 
 
 def test_csr_auto_choice_follows_row_lengths():
-    cases = [(bm.CSR_SCALAR, [9] * 500), (bm.CSR_VECTOR, [300] * 50), (bm.CSR_MERGE, [20] * 100),
-             (bm.CSR_MERGE, [40] * 50 + [10] * 50), (bm.CSR_VECTOR, list(range(1, 16)) * 30)]
+    cases = [(bm.CSR_SCALAR, [9] * 500), (bm.CSR_VECTOR, [300] * 50), (bm.CSR_VECTOR, [20] * 100),
+             (bm.CSR_MERGE, [40] * 50 + [10] * 50), (bm.CSR_SCALAR, list(range(1, 16)) * 30),
+             (bm.CSR_VECTOR, list(range(0, 6)) * 80), (bm.CSR_MERGE, list(range(20, 61)) * 10),
+             (bm.CSR_SCALAR, [4] * 400)]
     for want, lengths in cases:
         c, x = csr_from_lengths(lengths, 4000, seed=want)
         A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
