@@ -50,6 +50,6 @@ int launch_library(bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
 void destroy_library(bmfp_matrix_t &A);
 int auto_variant(const bmfp_matrix_t &A);
 int csr_group_for(double mean_len);
-int csr_stage_group(double mean_len, int lo);
+int csr_stage_group(double mean_len, double max_len, int lo);
 bool bcsr_supported(int variant, int dtype, int layout, int bs);
 }  // namespace bmfp

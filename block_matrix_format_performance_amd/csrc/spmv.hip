@@ -212,7 +212,7 @@ int auto_variant(const bmfp_matrix_t &A)
     if (mx > 8.0 * mean + 64.0) return BMFP_CSR_MERGE;
     if (mean >= 128.0) return BMFP_CSR_VECTOR;
     if (mean < 12.0)
-      return mx <= mean + 2.0 || csr_stage_group(mean, 1) == 1 ? BMFP_CSR_SCALAR : BMFP_CSR_VECTOR;
+      return mx <= mean + 2.0 || csr_stage_group(mean, mx, 1) == 1 ? BMFP_CSR_SCALAR : BMFP_CSR_VECTOR;
     return mx <= 1.25 * mean + 2.0 ? BMFP_CSR_VECTOR : BMFP_CSR_MERGE;
   }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
@@ -258,11 +258,14 @@ int csr_group_for(double mean_len)
 // wavefront of 64/G rows holds at most 576 nonzeros on average (9/10 of the
 // staging capacity, room for rows longer than the mean).  0 where staging
 // was measured not to pay: under 288 nonzeros per wavefront (rows of 4 with
-// one lane each: 4.7 vs 5.3 TB/s) or 32+ lanes per row (rows of 256: 5.0 vs
-// 5.6 TB/s for the lane-group kernel).
-int csr_stage_group(double mean_len, int lo)
+// one lane each: 4.7 vs 5.3 TB/s), 32+ lanes per row (rows of 256: 5.0 vs
+// 5.6 TB/s for the lane-group kernel), or a longest row over 8x the mean +
+// 64 (its wavefront would walk it with the narrow group: power-law rows 26
+// vs 95 GB/s).
+int csr_stage_group(double mean_len, double max_len, int lo)
 {
   constexpr double FILL = 0.9 * CSR_STAGE_CAP;
+  if (max_len > 8.0 * mean_len + 64.0) return 0;
   int g = lo;
   while (g < 64 && (64.0 / g) * mean_len > FILL) g *= 2;
   const double fill = (64.0 / g) * mean_len;
@@ -290,15 +293,15 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
   a.x = static_cast<const AT *>(x);
   a.y = static_cast<AT *>(y);
   if (variant == BMFP_CSR_SCALAR) {
-    if (csr_stage_group(A.mean_row_blocks, 1) == 1) {
+    if (csr_stage_group(A.mean_row_blocks, A.max_row_blocks, 1) == 1) {
       launch_csr_stage<VT, AT, 1>(A, a, s);
     } else {
       Grid g = make_grid(A.n_rows);
       a.xcd_per = g.per;
       k_csr_scalar<VT, AT><<<g.groups, 256, 0, s>>>(a);
     }
-  } else if (variant == BMFP_CSR_VECTOR && csr_stage_group(A.mean_row_blocks, 2) > 0) {
-    switch (csr_stage_group(A.mean_row_blocks, 2)) {
+  } else if (variant == BMFP_CSR_VECTOR && csr_stage_group(A.mean_row_blocks, A.max_row_blocks, 2) > 0) {
+    switch (csr_stage_group(A.mean_row_blocks, A.max_row_blocks, 2)) {
     case 2: launch_csr_stage<VT, AT, 2>(A, a, s); break;
     case 4: launch_csr_stage<VT, AT, 4>(A, a, s); break;
     case 8: launch_csr_stage<VT, AT, 8>(A, a, s); break;
