@@ -71,7 +71,7 @@ BcsrArgs<VT, AT> bcsr_args(const bmfp_matrix_t &A, const void *x, void *y, int p
 }
 
 // Wavefronts per workgroup of k_bcsr_stage (LDS per wavefront: 6.9 KB at bs 1,
-// 10.1 KB at bs 2, 14.1 KB at bs 3 in FP64), the fastest of 1, 2 and 4 on the
+// 10.4 KB at bs 2, 14.4 KB at bs 3 in FP64), the fastest of 1, 2 and 4 on the
 // HBM-resident 9-point stencils.
 template <int BS> constexpr int stage_wpg() { return BS == 1 ? 4 : 2; }
 
@@ -97,7 +97,7 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
   }
   case BMFP_BCSR_GROUP_PER_ROW: {
     constexpr DenseShape D = dense_shape<VT, BS>();
-    if constexpr (BS <= 3 && (BS * BS * (int) sizeof(VT)) % 8 == 0) {
+    if constexpr (BS <= 3 && (BS * BS * (int) sizeof(VT)) % 4 == 0) {
       launch_stage<VT, AT, BS, LAYOUT, SCALED>(A, x, y, s);
     } else if constexpr (BS == 2 || BS == 3) {
       constexpr int RPW = 64 / BS;
@@ -217,15 +217,15 @@ int auto_variant(const bmfp_matrix_t &A)
   }
   // Fastest family per (layout, bs) measured on HBM-resident 9-point stencil
   // matrices (FP64, ~4.4e8 stored values): for bs 1-3 the wavefront's blocks
-  // staged in LDS and split over each block row's lanes (bs 1: only FP64 and
-  // only while rows average at most 9 blocks, the staging capacity; otherwise
+  // staged in LDS and split over each block row's lanes (bs 1: FP64 and FP32,
+  // while rows average at most 9 blocks, the staging capacity; otherwise
   // batched thread per row); 16-byte lane slices (row-major) or
   // shuffle-shared x segments (column-major) from bs = 4 up.
   // block sizes without a precompiled instantiation: a kernel specialised by
   // hipRTC (F64 / F32), the runtime-bs thread-per-row kernel for F16
   if (!precompiled_bs(A.bs)) return A.dtype == BMFP_F16 ? BMFP_BCSR_THREAD_PER_ROW : BMFP_BCSR_JIT;
-  if (A.bs == 1)  // staged only for FP64 (8-byte blocks)
-    return A.dtype == BMFP_F64 && A.mean_row_blocks <= 9.0 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_THREAD_PER_ROW;
+  if (A.bs == 1)  // staged for FP64 and FP32 (whole 4-byte words)
+    return A.dtype != BMFP_F16 && A.mean_row_blocks <= 9.0 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_THREAD_PER_ROW;
   if (A.bs <= 3) return BMFP_BCSR_GROUP_PER_ROW;
   return A.layout == BMFP_COL_MAJOR ? BMFP_BCSR_SHUFFLE_X : BMFP_BCSR_GROUP_PER_ROW;
 }

@@ -14,8 +14,8 @@
 //   k_bcsr_stage lane group per block row,   the same family for bs 1-3
 //                wavefront's blocks in LDS,
 //                blocks split over its lanes
-//   k_bcsr_split as k_bcsr_stage without     the same family (blocks of a
-//                the LDS copy                size that is not 8-byte aligned)
+//   k_bcsr_split as k_bcsr_stage without     the same family (FP16 bs 3:
+//                the LDS copy                blocks not of whole 4-byte words)
 //   k_bcsr_shfx  thread per row, x segment   K10/K11 :402-469 (there: shared memory,
 //                shared by lane shuffles      wrong for bs<32; here: per block row)
 //   k_bcsr_bgr   lane group per block row    K12-K16 :485-673 (there: wrong for
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_bcsr_split(BcsrArgs<VT, AT> a)
 // k_bcsr_split with the wavefront's blocks staged in LDS first.  The values
 // and column indices of the wavefront's RPW block rows are one contiguous
 // range; it is copied to LDS with 16-byte loads at consecutive addresses
-// across the lanes (one load instruction moves 1 KB), then each lane reads
+// across the lanes (one load instruction moves 1 KB; StagedRange), then each lane reads
 // its whole blocks from LDS and gathers x from global memory as in
 // k_bcsr_split.  A wavefront whose range exceeds CAP blocks (rows longer than
 // 9 blocks on average) reads global memory directly.  No workgroup barrier:
@@ -429,6 +429,50 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Copy of bytes [s, e) of a buffer whose base is 16-byte aligned into LDS,
+// in two halves so that several ranges' loads can be in flight together:
+// load() issues 16-byte loads at consecutive addresses across the lanes from
+// s rounded down to 16 bytes (up to 12 bytes before s come along), the
+// partial last chunk 4 bytes at a time (s and e are multiples of 4,
+// e - s <= CAPB); store() writes them to dst; shift is s's offset in dst.
+// LDS bytes that hold a staged range of up to capb bytes: the start rounded
+// down to 16 bytes adds up to 12, and the copy writes whole 16-byte chunks.
+constexpr int staged_bytes(int capb) { return (capb + 12 + 15) / 16 * 16; }
+
+template <int CAPB>
+struct StagedRange {
+  static constexpr int NCH = (staged_bytes(CAPB) / 16 + 63) / 64;
+  u32x4 t[NCH];
+  int n = 0, shift = 0;
+  __device__ __forceinline__ void load(const void *base_v, int64_t s, int64_t e, int lane)
+  {
+    const char *base = static_cast<const char *>(base_v);
+    const int64_t s_al = s & ~(int64_t) 15;
+    n = (int) (e - s_al);
+    shift = (int) (s - s_al);
+#pragma unroll
+    for (int m = 0; m < NCH; ++m) {
+      const int k = 16 * (lane + 64 * m);
+      if (k < n) {
+        if (k + 16 <= n) {
+          t[m] = *reinterpret_cast<const u32x4 *>(base + s_al + k);
+        } else {
+          const unsigned *w = reinterpret_cast<const unsigned *>(base + s_al + k);
+          t[m] = u32x4{w[0], n - k > 4 ? w[1] : 0u, n - k > 8 ? w[2] : 0u, 0u};
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void store(char *dst, int lane) const
+  {
+#pragma unroll
+    for (int m = 0; m < NCH; ++m) {
+      const int k = 16 * (lane + 64 * m);
+      if (k < n) *reinterpret_cast<u32x4 *>(dst + k) = t[m];
+    }
+  }
+};
 
 // Accumulates blocks b, b + BS, ... < b1 of one block row into part[] in
 // predicated batches of P: col_at(k) and block_at(k, v) fetch block k's column
@@ -476,14 +520,11 @@ template <class VT, class AT, int BS, int LAYOUT, bool SCALED, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_bcsr_stage(BcsrArgs<VT, AT> a)
 {
   constexpr int P = split_batch<BS>(), RPW = 64 / BS, CAP = stage_cap<BS>();
-  constexpr int BB = BS * BS * (int) sizeof(VT);       // bytes per block
-  static_assert(BB % 8 == 0, "staging copies 16-byte chunks with an 8-byte tail");
-  constexpr int SLACK = BB % 16 == 0 ? 0 : 16;         // range start rounded down to 16 bytes
-  constexpr int VBYTES = (CAP * BB + SLACK + 15) / 16 * 16;
-  constexpr int NCH = (VBYTES / 16 + 63) / 64, NCOL = (CAP + 63) / 64;
+  constexpr int BB = BS * BS * (int) sizeof(VT);  // bytes per block
+  static_assert(BB % 4 == 0, "staging copies whole 4-byte words");
   constexpr int VALIGN = lowbit(BB);
-  __shared__ __align__(16) char vals_s[WPG][VBYTES];
-  __shared__ int cols_s[WPG][CAP];
+  __shared__ __align__(16) char vals_s[WPG][staged_bytes(CAP * BB)];
+  __shared__ __align__(16) char cols_s[WPG][staged_bytes(CAP * 4)];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int64_t r0_64 = (tile_id(a.xcd_per) * WPG + wib) * RPW;
   if (r0_64 >= a.n_brows) return;  // the whole wavefront leaves; no barrier follows
@@ -501,41 +542,17 @@ __global__ __launch_bounds__(64 * WPG) void k_bcsr_stage(BcsrArgs<VT, AT> a)
 #pragma unroll
   for (int r = 0; r < BS; ++r) part[r] = 0;
   if (B1 - B0 <= CAP) {
-    const char *vbase = reinterpret_cast<const char *>(a.vals);
-    const int64_t s_byte = (int64_t) B0 * BB;
-    const int64_t s_al = s_byte & ~(int64_t) 15;           // values start 256-byte aligned
-    const int shift = SLACK ? (int) (s_byte - s_al) : 0;
-    const int e_rel = (int) ((int64_t) B1 * BB - s_al);    // staged bytes, a multiple of 8
-    const int nb = B1 - B0;
-    u32x4 tv[NCH];
-    int tc[NCOL];
-#pragma unroll
-    for (int m = 0; m < NCH; ++m) {
-      const int k = 16 * (lane + 64 * m);
-      if (k < e_rel) {
-        if (SLACK == 0 || k + 16 <= e_rel) {
-          tv[m] = *reinterpret_cast<const u32x4 *>(vbase + s_al + k);
-        } else {  // the last 8 bytes of the range
-          const u32x2 t = *reinterpret_cast<const u32x2 *>(vbase + s_al + k);
-          tv[m] = u32x4{t.x, t.y, 0u, 0u};
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < NCOL; ++m)
-      if (lane + 64 * m < nb) tc[m] = a.cols[B0 + lane + 64 * m];
-#pragma unroll
-    for (int m = 0; m < NCH; ++m) {
-      const int k = 16 * (lane + 64 * m);
-      if (k < e_rel) *reinterpret_cast<u32x4 *>(&vals_s[wib][k]) = tv[m];
-    }
-#pragma unroll
-    for (int m = 0; m < NCOL; ++m)
-      if (lane + 64 * m < nb) cols_s[wib][lane + 64 * m] = tc[m];
+    StagedRange<CAP * BB> sv;
+    StagedRange<CAP * 4> sc;
+    sv.load(a.vals, (int64_t) B0 * BB, (int64_t) B1 * BB, lane);
+    sc.load(a.cols, (int64_t) B0 * 4, (int64_t) B1 * 4, lane);
+    sv.store(vals_s[wib], lane);
+    sc.store(cols_s[wib], lane);
     wave_lds_sync();
-    const char *vs = &vals_s[wib][shift];
+    const char *vs = vals_s[wib] + sv.shift;
+    const int *cs = reinterpret_cast<const int *>(cols_s[wib] + sc.shift);
     split_accumulate<VT, AT, BS, LAYOUT, P, true>(
-        b, b1, a.x, part, [&](int k) { return cols_s[wib][k - B0]; },
+        b, b1, a.x, part, [&](int k) { return cs[k - B0]; },
         [&](int k, VT(&v)[BS * BS]) {
           load_n<VT, BS * BS, VALIGN>(reinterpret_cast<const VT *>(vs + (k - B0) * BB), v);
         });
@@ -770,44 +787,13 @@ __global__ __launch_bounds__(256) void k_csr_vector(CsrArgs<VT, AT> a)
 // ============================================================================
 constexpr int CSR_STAGE_CAP = 640;
 
-// Copies bytes [s, e) of the buffer at base (16-byte aligned) to LDS at dst:
-// the start rounded down to 16 bytes (up to 12 bytes before s come along),
-// 16 bytes per lane per step, the last partial chunk 4 bytes at a time (s and
-// e are multiples of 4; e - s <= CAPB).  Returns s's offset in dst.
-template <int CAPB>
-__device__ __forceinline__ int stage_range(const char *__restrict__ base, int64_t s, int64_t e, char *dst, int lane)
-{
-  constexpr int NCH = ((CAPB + 16) / 16 + 63) / 64;
-  const int64_t s_al = s & ~(int64_t) 15;
-  const int n = (int) (e - s_al);
-  u32x4 t[NCH];
-#pragma unroll
-  for (int m = 0; m < NCH; ++m) {
-    const int k = 16 * (lane + 64 * m);
-    if (k < n) {
-      if (k + 16 <= n) {
-        t[m] = *reinterpret_cast<const u32x4 *>(base + s_al + k);
-      } else {
-        const unsigned *w = reinterpret_cast<const unsigned *>(base + s_al + k);
-        t[m] = u32x4{w[0], n - k > 4 ? w[1] : 0u, n - k > 8 ? w[2] : 0u, 0u};
-      }
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < NCH; ++m) {
-    const int k = 16 * (lane + 64 * m);
-    if (k < n) *reinterpret_cast<u32x4 *>(dst + k) = t[m];
-  }
-  return (int) (s - s_al);
-}
-
 template <class VT, class AT, int G, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_csr_stage(CsrArgs<VT, AT> a)
 {
   constexpr int RPW = 64 / G, CAP = CSR_STAGE_CAP, U = G == 1 ? 8 : 4;
   constexpr int VCAP = CAP * (int) sizeof(VT), CCAP = CAP * 4;
-  __shared__ __align__(16) char vals_s[WPG][VCAP + 16];
-  __shared__ __align__(16) char cols_s[WPG][CCAP + 16];
+  __shared__ __align__(16) char vals_s[WPG][staged_bytes(VCAP)];
+  __shared__ __align__(16) char cols_s[WPG][staged_bytes(CCAP)];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int64_t r0_64 = (tile_id(a.xcd_per) * WPG + wib) * RPW;
   if (r0_64 >= a.n_rows) return;  // the whole wavefront leaves; no barrier follows
@@ -822,13 +808,15 @@ __global__ __launch_bounds__(64 * WPG) void k_csr_stage(CsrArgs<VT, AT> a)
   }
   AT acc = 0;
   if (J1 - J0 <= CAP) {
-    const int vsh = stage_range<VCAP>(reinterpret_cast<const char *>(a.vals), (int64_t) J0 * (int) sizeof(VT),
-                                      (int64_t) J1 * (int) sizeof(VT), vals_s[wib], lane);
-    const int csh = stage_range<CCAP>(reinterpret_cast<const char *>(a.cols), (int64_t) J0 * 4, (int64_t) J1 * 4,
-                                      cols_s[wib], lane);
+    StagedRange<VCAP> sv;
+    StagedRange<CCAP> sc;
+    sv.load(a.vals, (int64_t) J0 * (int) sizeof(VT), (int64_t) J1 * (int) sizeof(VT), lane);
+    sc.load(a.cols, (int64_t) J0 * 4, (int64_t) J1 * 4, lane);
+    sv.store(vals_s[wib], lane);
+    sc.store(cols_s[wib], lane);
     wave_lds_sync();
-    const VT *vs = reinterpret_cast<const VT *>(vals_s[wib] + vsh);
-    const int *cs = reinterpret_cast<const int *>(cols_s[wib] + csh);
+    const VT *vs = reinterpret_cast<const VT *>(vals_s[wib] + sv.shift);
+    const int *cs = reinterpret_cast<const int *>(cols_s[wib] + sc.shift);
     for (; e < e1; e += U * G) {
       int c[U];
       AT xv[U];

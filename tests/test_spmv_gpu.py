@@ -313,7 +313,9 @@ def test_bcsr_auto_choice_small_blocks():
         assert A.auto_variant() == want, (bs, layout)
     m, x = stencil_case(9, 7, 1, kind=0, seed=1)
     A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1, dtype=bm.F32)
-    assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW      # 4-byte blocks are not staged
+    assert A.auto_variant() == bm.BCSR_GROUP_PER_ROW       # 4-byte blocks are staged too
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1, dtype=bm.F16)
+    assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW      # 2-byte blocks are not
     m, x = ragged_case(200, 300, 1, max_len=40, seed=5, empty_every=0)   # mean ~20 blocks per row
     A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1)
     assert A.auto_variant() == bm.BCSR_THREAD_PER_ROW
