@@ -210,12 +210,14 @@ def serving_stream(n_sample, seed=2):
     return reqs, arrivals[:n_sample] - arrivals[0]
 
 
-def plain_one_at_a_time(reqs, max_iter):
+def plain_one_at_a_time(reqs, max_iter, use_jacobi=False):
     """Each request's right-hand sides solved one after another by the plain
-    FP64 BiCGStab (x0 = 0, 1e-10).  Timed: the solve calls only (matrix
-    generation, solver set-up and its first graph build are excluded).
-    Compulsory bytes per iteration of the fused solver: two FP64 SpMVs (matrix,
-    x read, y written) and 15 further FP64 vector passes."""
+    FP64 BiCGStab (x0 = 0, 1e-10), with point-Jacobi preconditioning when
+    use_jacobi (as gpu_bicgstab(A, true)).  Timed: the solve calls only
+    (matrix generation, solver set-up and its first graph build are
+    excluded).  Compulsory bytes per iteration of the fused solver: two FP64
+    SpMVs (matrix, x read, y written) and 15 further FP64 vector passes (two
+    more for the inverse diagonal with Jacobi)."""
     total, per_req, xs, iters, moved = 0.0, [], [], [], 0.0
     for r in reqs:
         A = bm.Matrix.stencil(r["side"], r["side"], r["bs"], 0, seed=r["seed"])
@@ -223,7 +225,7 @@ def plain_one_at_a_time(reqs, max_iter):
         B = bm.DeviceArray(r["nrhs"] * n)
         B.fill(1, seed=r["rhs_seed"], stream_id=6)
         x = bm.DeviceArray(n)
-        S = bm.BiCGStab(A)
+        S = bm.BiCGStab(A, use_jacobi=use_jacobi)
         S.solve(B.view(0, n), x, 1e-10, 2)            # graph build, untimed
         t_req, xr = 0.0, []
         for k in range(r["nrhs"]):
@@ -235,7 +237,7 @@ def plain_one_at_a_time(reqs, max_iter):
                 raise RuntimeError(f"plain FP64 BiCGStab failed on {r}: status {info.status}")
             iters.append(info.iterations)
             moved += info.iterations * (2.0 * (8.0 * A.nnzb * r["bs"] ** 2 + 4.0 * A.nnzb + 4.0 * (A.n_rows + 1)
-                                               + 16.0 * n) + 15 * 8.0 * n)
+                                               + 16.0 * n) + (17 if use_jacobi else 15) * 8.0 * n)
             xr.append(x.download())
         total += t_req
         per_req.append(t_req)
@@ -245,14 +247,15 @@ def plain_one_at_a_time(reqs, max_iter):
     return total, np.array(per_req), xs, iters, moved
 
 
-def request_bytes_moved(r, outer, inner):
+def request_bytes_moved(r, outer, inner, jacobi=False):
     """Compulsory HBM bytes of one served request (right-hand sides share every
     matrix read).  Refinement step 0 (x = 0): b read, x' and r32 written, no
     matrix.  Later steps: the FP64 matrix once, x, d, b read and x', r32
     written.  Inner iteration: the FP32 matrix twice and 68 bytes of FP32
     vector traffic per right-hand side and row (v = A p: ri, p, v, r32 read,
     p, v written; t = A s: ri, v read, s, t written; updates: s, p, d, t, r32
-    read, d, ri written)."""
+    read, d, ri written), plus the FP32 inverse diagonal read three times per
+    row with Jacobi."""
     side, bs, nr = r["side"], r["bs"], r["nrhs"]
     nodes = side * side
     nnzb = (3 * side - 2) ** 2
@@ -260,7 +263,7 @@ def request_bytes_moved(r, outer, inner):
     idx = 4.0 * nnzb + 4.0 * (nodes + 1)
     first = nr * n * (8 + 8 + 4)
     resid = 8.0 * nnzb * bs * bs + idx + nr * n * (8 + 4 + 8 + 8 + 4)
-    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 68.0
+    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 68.0 + (12.0 * n if jacobi else 0.0)
     return (first if outer > 0 else 0.0) + max(outer - 1, 0) * resid + inner * inner_it
 
 
@@ -301,7 +304,8 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
             worst_res = max(worst_res, res.rel_residual)
             outer.append(res.outer_iterations)
             inner.append(res.inner_iterations)
-            moved += request_bytes_moved(reqs[j], res.outer_iterations, res.inner_iterations)
+            moved += request_bytes_moved(reqs[j], res.outer_iterations, res.inner_iterations,
+                                         bool(cfg.get("use_jacobi", 0)))
             if ref_x is not None:
                 xs[j] = x
             done += 1
@@ -361,6 +365,30 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
         log(f"[bench] serve lambda={lam}: {r['solves_per_s']} solves/s ({r['speedup_vs_plain']}x plain), "
             f"p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, busy {r['gpu_busy_frac']}")
         runs.append(r)
+    jacobi = None
+    if plain:
+        # the same stream with point-Jacobi preconditioning on both sides (the
+        # reference's driver solves with gpu_bicgstab(A, true)): plain FP64
+        # Jacobi BiCGStab one at a time against the served loop with Jacobi inner
+        # solves, saturated (the last lambda)
+        pj_s, pj_per, pj_x, pj_it, pj_bytes = plain_one_at_a_time(reqs, max_iter, use_jacobi=True)
+        log(f"[bench] plain FP64 + Jacobi one at a time: {pj_s:.3f} s of solves ({n_sample / pj_s:.1f} solves/s)")
+        cfg_j = dict(cfg, use_jacobi=1)
+        acc_j = serve_replay(reqs, arrivals, float("inf"), cfg_j, pj_x)
+        rj = serve_replay(reqs, arrivals, lambdas[-1], cfg_j)
+        rj["speedup_vs_plain_jacobi"] = round(rj["solves_per_s"] / (n_sample / pj_s), 3)
+        rj["speedup_vs_plain"] = round(rj["solves_per_s"] / (n_sample / plain_s), 3)
+        log(f"[bench] serve + Jacobi lambda={lambdas[-1]}: {rj['solves_per_s']} solves/s "
+            f"({rj['speedup_vs_plain_jacobi']}x plain + Jacobi, {rj['speedup_vs_plain']}x plain), "
+            f"p50 {rj['p50_ms']} ms, p99 {rj['p99_ms']} ms")
+        jacobi = {"plain_fp64_jacobi": {"solves_per_s": round(n_sample / pj_s, 2), "solve_seconds": round(pj_s, 4),
+                                        "GBs": round(pj_bytes / pj_s / 1e9, 1),
+                                        "p50_ms": round(float(np.percentile(pj_per, 50)) * 1e3, 3),
+                                        "p99_ms": round(float(np.percentile(pj_per, 99)) * 1e3, 3),
+                                        "mean_iterations": round(float(np.mean(pj_it)), 1)},
+                  "served_jacobi": rj,
+                  "accuracy": {"max_err_vs_plain_jacobi": acc_j["max_err_vs_fp64"],
+                               "max_rel_residual": acc_j["max_rel_residual"]}}
     return {
         "workload": f"first {n_sample} of a 20,000-request stream (seed 2): side U{{16..512}}, "
                     f"bs in {{2,3,4,6,8}}, 1-4 RHS uniform[-1,1], tol 1e-10, diagonally dominant "
@@ -374,6 +402,7 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
                        "timing": "solve calls only, one request after another"} if plain else None,
         "served": runs,
         "accuracy": accuracy,
+        "jacobi": jacobi,
     }
 
 

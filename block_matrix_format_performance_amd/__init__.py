@@ -551,7 +551,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget_bytes", ctypes.c_size_t), ("max_systems", ctypes.c_int),
                 ("iterations_per_round", ctypes.c_int), ("slot_groups", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int),
-                ("chunks_per_tile", ctypes.c_int)]
+                ("chunks_per_tile", ctypes.c_int), ("use_jacobi", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

@@ -62,6 +62,7 @@ struct ReqDesc {
   const int *cols;
   const double *v64;
   const float *v32;
+  const float *dinv;  // Jacobi: 1 / diagonal entry per scalar row (FP32); null without
   const double *b;
   double *x[2];
   float *r32, *d, *ri, *p[2], *v[2], *s, *t;
@@ -358,7 +359,9 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
   const gptr<const float> pp = glb((const float *) D.p[(it - 1) & 1]), vp = glb((const float *) D.v[(it - 1) & 1]);
   const gptr<float> pc = glb(D.p[it & 1]), vc = glb(D.v[it & 1]);
   const gptr<const float> ri = glb((const float *) D.ri), rh = glb((const float *) D.r32);
-  auto gather = [&](uint32_t o, float (&x)[NR]) {
+  const gptr<const float> dinv = glb(D.dinv);
+  const bool jac = D.dinv != nullptr;
+  auto form_p = [&](uint32_t o, float (&x)[NR]) {
     if (it == 0) {
       ldv<float, NR>(rh, o, x);
       return;
@@ -370,8 +373,17 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
 #pragma unroll
     for (int k = 0; k < NR; ++k) x[k] = p_update(rr[k], p0[k], v0[k], beta[k], omega[k]);
   };
+  // the SpMV operand: p, or D^-1 p with Jacobi (right preconditioning)
+  auto gather = [&](uint32_t o, float (&x)[NR]) {
+    form_p(o, x);
+    if (jac) {
+      const float m = ld(dinv, o / NR);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) x[k] *= m;
+    }
+  };
   auto own = [&](uint32_t o, AOwn<NR> &m) {
-    gather(o, m.p);
+    form_p(o, m.p);
     if (it == 0) {
 #pragma unroll
       for (int k = 0; k < NR; ++k) m.rh[k] = m.p[k];
@@ -404,15 +416,26 @@ __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &
   const gptr<const float> vcur = glb((const float *) D.v[it & 1]);
   const gptr<const float> ri = glb((const float *) (it == 0 ? D.r32 : D.ri));
   const gptr<float> s_out = glb(D.s), t_out = glb(D.t);
-  auto gather = [&](uint32_t o, float (&x)[NR]) {
+  const gptr<const float> dinv = glb(D.dinv);
+  const bool jac = D.dinv != nullptr;
+  auto form_s = [&](uint32_t o, float (&x)[NR]) {
     float rr[NR], vv[NR];
     ldv<float, NR>(ri, o, rr);
     ldv<float, NR>(vcur, o, vv);
 #pragma unroll
     for (int k = 0; k < NR; ++k) x[k] = s_update(rr[k], vv[k], alpha[k]);
   };
+  // the SpMV operand: s, or D^-1 s with Jacobi
+  auto gather = [&](uint32_t o, float (&x)[NR]) {
+    form_s(o, x);
+    if (jac) {
+      const float m = ld(dinv, o / NR);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) x[k] *= m;
+    }
+  };
   chunk_spmv<float, float, BS, NR, 2, BOwn<NR>>(D, glb(D.v32), br0, true, gather,
-                                                [&](uint32_t o, BOwn<NR> &m) { gather(o, m.s); },
+                                                [&](uint32_t o, BOwn<NR> &m) { form_s(o, m.s); },
                                                 [&](uint32_t o, const float (&y)[NR], const BOwn<NR> &m) {
                                                   stv<float, NR>(s_out, o, m.s);
                                                   stv<float, NR>(t_out, o, y);
@@ -424,9 +447,10 @@ __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &
                                                 });
 }
 
-// ri = s - omega t, d += alpha p + omega s (d = 0 before iteration 0; a live
-// right-hand side outside the inner solve gets d = 0 at iteration 0 and keeps
-// d afterwards); partials (r32, ri), (ri, ri).
+// ri = s - omega t, d += alpha p + omega s, or d += D^-1 (alpha p + omega s)
+// with Jacobi (d = 0 before iteration 0; a live right-hand side outside the
+// inner solve gets d = 0 at iteration 0 and keeps d afterwards); partials
+// (r32, ri), (ri, ri).
 template <int BS, int NR>
 __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
@@ -441,8 +465,11 @@ __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &
   const gptr<const float> pcur = glb((const float *) D.p[it & 1]), s_in = glb((const float *) D.s),
                           t_in = glb((const float *) D.t), rh = glb((const float *) D.r32);
   const gptr<float> d = glb(D.d), ri = glb(D.ri);
+  const gptr<const float> dinv = glb(D.dinv);
+  const bool jac = D.dinv != nullptr;
   chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
     float s[NR], p[NR], dd[NR], t[NR], h[NR], rn[NR];
+    const float jm = jac ? ld(dinv, o / NR) : 1.f;
     ldv<float, NR>(s_in, o, s);
     ldv<float, NR>(pcur, o, p);
     if (it == 0) {
@@ -456,7 +483,9 @@ __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       rn[k] = fmadd(-omega[k], t[k], s[k]);
-      if ((act >> k) & 1u) dd[k] = fmadd(omega[k], s[k], fmadd(alpha[k], p[k], dd[k]));
+      if ((act >> k) & 1u)
+        dd[k] = jac ? fmadd(jm, fmadd(omega[k], s[k], alpha[k] * p[k]), dd[k])
+                    : fmadd(omega[k], s[k], fmadd(alpha[k], p[k], dd[k]));
       acc[k][0] += (double) h[k] * (double) rn[k];
       acc[k][1] += (double) rn[k] * (double) rn[k];
     }
@@ -710,6 +739,28 @@ __global__ __launch_bounds__(256) void k_stencil_rp(int nx, int ny, int *__restr
 __global__ __launch_bounds__(256) void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n)
 {
   for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) b[i] = (float) a[i];
+}
+
+// Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64
+// row-major blocks, 1 where the diagonal block is absent or |a_ii| < 1e-20
+// (the rule of the BiCGStab solver's k_jacobi), stored in FP32.
+__global__ __launch_bounds__(256) void k_serve_dinv(int n_brows, int bs, const int *__restrict__ rp,
+                                                    const int *__restrict__ cols, const double *__restrict__ v64,
+                                                    float *__restrict__ dinv)
+{
+  const int64_t n = (int64_t) n_brows * bs;
+  for (int64_t row = (int64_t) blockIdx.x * 256 + threadIdx.x; row < n; row += (int64_t) gridDim.x * 256) {
+    const int br = (int) (row / bs), r = (int) (row - (int64_t) br * bs);
+    double a = 0.0;
+    bool found = false;
+    for (int b = rp[br]; b < rp[br + 1]; ++b)
+      if (cols[b] == br) {
+        a = v64[((int64_t) b * bs + r) * bs + r];
+        found = true;
+        break;
+      }
+    dinv[row] = (float) ((!found || fabs(a) < 1e-20) ? 1.0 : 1.0 / a);
+  }
 }
 
 // b of a request with nr right-hand sides, interleaved (element i of k at

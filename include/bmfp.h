@@ -304,6 +304,8 @@ typedef struct {
   int max_outer;              /* refinement steps per right-hand side                  */
   int max_inner;              /* inner iterations per refinement step                  */
   int chunks_per_tile;        /* tile height in 256-row chunks; 0: tuned per bs        */
+  int use_jacobi;             /* 1: point-Jacobi (1/diagonal, right) preconditioning of
+                                 the inner solves, as gpu_bicgstab(A, true); 0: none   */
 } bmfp_server_config;
 
 typedef struct {

@@ -117,7 +117,7 @@ def test_stencil_create_argument_errors():
 
 @pytest.mark.parametrize("bad", [dict(max_systems=0), dict(iterations_per_round=0), dict(slot_groups=-2),
                                  dict(chunks_per_tile=-2), dict(inner_tol=0.0), dict(max_outer=0),
-                                 dict(max_inner=0)])
+                                 dict(max_inner=0), dict(use_jacobi=2)])
 def test_server_config_validation(bad):
     with pytest.raises(bm.BmfpError) as e:
         bm.Server(**bad)
@@ -125,12 +125,13 @@ def test_server_config_validation(bad):
 
 
 def test_server_request_footprint():
-    # FP64 + FP32 matrix, 3 FP64 and 9 FP32 vectors over all right-hand sides, 256-B aligned pieces
+    # FP64 + FP32 matrix, the FP32 Jacobi inverse diagonal, 3 FP64 and 9 FP32 vectors over all
+    # right-hand sides, 256-B aligned pieces
     def a256(b):
         return (b + 255) // 256 * 256
     for nx, ny, bs, nrhs in [(16, 16, 2, 1), (100, 37, 6, 3), (512, 512, 8, 4)]:
         nb, nnzb = nx * ny, (3 * nx - 2) * (3 * ny - 2)
         n, nv = nb * bs, nnzb * bs * bs
-        want = (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + a256(4 * nv) + 3 * a256(8 * n * nrhs)
-                + 9 * a256(4 * n * nrhs))
+        want = (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + a256(4 * nv) + a256(4 * n)
+                + 3 * a256(8 * n * nrhs) + 9 * a256(4 * n * nrhs))
         assert bm.Server.request_bytes(bm.Server.stencil_request(nx, ny, bs, nrhs=nrhs)) == want

@@ -90,6 +90,30 @@ def test_results_identical_alone_and_in_mix():
         check_against_oracle(s["nx"], s["ny"], s["bs"], 0, s["seed"], s["rhs_seed"], s["nrhs"], res, x)
 
 
+def test_jacobi_preconditioned_requests():
+    # point-Jacobi inner solves (use_jacobi=1, the preconditioner of
+    # gpu_bicgstab(A, true)): every request still meets the oracle bar, the
+    # inner solves take fewer iterations, and a request alone gives the same
+    # bits as in the mix
+    specs = mix_requests()
+    plain = bm.Server(memory_budget_bytes=1 << 30, max_systems=64)
+    jac = bm.Server(memory_budget_bytes=1 << 30, max_systems=64, use_jacobi=1)
+    ip = [plain.submit(plain.stencil_request(**s)) for s in specs]
+    ij = [jac.submit(jac.stencil_request(**s)) for s in specs]
+    op, oj = plain.drain(), jac.drain()
+    for i, s in zip(ij, specs):
+        res, x = oj[i]
+        check_against_oracle(s["nx"], s["ny"], s["bs"], 0, s["seed"], s["rhs_seed"], s["nrhs"], res, x)
+    inner_plain = sum(op[i][0].inner_iterations for i in ip)
+    inner_jac = sum(oj[i][0].inner_iterations for i in ij)
+    assert inner_jac <= 0.8 * inner_plain, (inner_jac, inner_plain)
+    alone = bm.Server(memory_budget_bytes=1 << 30, max_systems=64, use_jacobi=1)
+    alone.submit(alone.stencil_request(**specs[2]))
+    ra, xa = next(iter(alone.drain().values()))
+    assert ra.inner_iterations == oj[ij[2]][0].inner_iterations
+    assert np.array_equal(xa, oj[ij[2]][1])
+
+
 def test_device_generator_matches_host_generator():
     # explicit arrays from the host generator vs the on-device generator:
     # identical matrices and right-hand sides give bitwise identical solutions
