@@ -362,7 +362,8 @@ def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=Tr
     for k, lam in enumerate(lambdas):
         r = serve_replay(reqs, arrivals, lam, cfg)
         r["speedup_vs_plain"] = round(r["solves_per_s"] / (n_sample / plain_s), 3) if plain else None
-        log(f"[bench] serve lambda={lam}: {r['solves_per_s']} solves/s ({r['speedup_vs_plain']}x plain), "
+        vs = f"{r['speedup_vs_plain']}x plain" if plain else "no plain comparator"
+        log(f"[bench] serve lambda={lam}: {r['solves_per_s']} solves/s ({vs}), "
             f"p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, busy {r['gpu_busy_frac']}")
         runs.append(r)
     jacobi = None
