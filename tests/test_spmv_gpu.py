@@ -92,9 +92,10 @@ def test_bcsr_f64_ragged_empty_rows(bs, layout):
         check_bcsr(m, x, layout, v)
 
 
+@pytest.mark.parametrize("dtype", [bm.F64, bm.F32])
 @pytest.mark.parametrize("bs", [1, 2, 3])
 @pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
-def test_bcsr_small_blocks_short_and_long_row_runs(bs, layout):
+def test_bcsr_small_blocks_short_and_long_row_runs(bs, layout, dtype):
     # wavefronts of 64/bs block rows: runs of short rows (at most 9 blocks
     # each) alternate with runs holding a few long rows, so some wavefronts
     # stage their blocks in LDS and others read them directly
@@ -109,8 +110,8 @@ def test_bcsr_small_blocks_short_and_long_row_runs(bs, layout):
     m = oracle.BCSR(n_rows, n_cols, bs, np.array(rp), np.array(cols, dtype=np.int32),
                     rng.uniform(-1, 1, len(cols) * bs * bs))
     x = rng.uniform(0, 1, n_cols * bs)
-    check_bcsr(m, x, layout, bm.BCSR_GROUP_PER_ROW)
-    check_bcsr(m, x, layout, bm.SPMV_AUTO)
+    check_bcsr(m, x, layout, bm.BCSR_GROUP_PER_ROW, dtype=dtype)
+    check_bcsr(m, x, layout, bm.SPMV_AUTO, dtype=dtype)
 
 
 @pytest.mark.parametrize("bs", [1, 4, 16])
@@ -134,7 +135,7 @@ def test_bcsr_runtime_block_size(bs):
         assert e.value.status == bm.ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("bs", [2, 4, 8, 12, 32])
+@pytest.mark.parametrize("bs", [1, 2, 3, 4, 8, 12, 32])
 @pytest.mark.parametrize("dtype", [bm.F32, bm.F16])
 def test_bcsr_low_precision_storage(bs, dtype):
     m, x = stencil_case(9, 8, bs, kind=0, seed=2)
