@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -150,7 +152,10 @@ int parse_mm(const std::string &text, const char *path, Parsed *out)
 
   std::vector<Entry> ent;
   if (coord) {
-    ent.reserve((size_t) ((symmetric || skew) ? 2 * NZ : NZ));
+    // an entry line takes at least 4 bytes ("i j\n"), so the file bounds the
+    // entries it can hold whatever its size line declares
+    const size_t fit = std::min<size_t>((size_t) NZ, text.size() / 4 + 1);
+    ent.reserve((symmetric || skew) ? 2 * fit : fit);
     for (long long k = 0; k < NZ; k++) {
       if (!L.next(&b, &e)) return fail(BMFP_ERR_INVALID, "%s: %lld of %lld entries present", path, k, NZ);
       const char *p = b;
@@ -198,10 +203,7 @@ int parse_mm(const std::string &text, const char *path, Parsed *out)
   return BMFP_OK;
 }
 
-}  // namespace
-
-extern "C" int bmfp_mm_read(const char *path, int *n_rows, int *n_cols, int *nnz, int *row_ptr, int *columns,
-                            double *values)
+int mm_read(const char *path, int *n_rows, int *n_cols, int *nnz, int *row_ptr, int *columns, double *values)
 {
   if (!path || !n_rows || !n_cols || !nnz) return fail(BMFP_ERR_INVALID, "NULL argument");
   std::string text;
@@ -224,8 +226,7 @@ extern "C" int bmfp_mm_read(const char *path, int *n_rows, int *n_cols, int *nnz
   return BMFP_OK;
 }
 
-extern "C" int bmfp_mm_write(const char *path, int n_rows, int n_cols, const int *row_ptr, const int *columns,
-                             const double *values)
+int mm_write(const char *path, int n_rows, int n_cols, const int *row_ptr, const int *columns, const double *values)
 {
   if (!path || !row_ptr || n_rows < 0 || n_cols < 0) return fail(BMFP_ERR_INVALID, "bad argument");
   const int nnz = row_ptr[n_rows];
@@ -240,9 +241,8 @@ extern "C" int bmfp_mm_write(const char *path, int n_rows, int n_cols, const int
   return BMFP_OK;
 }
 
-extern "C" int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr, const int *columns, int value_bytes,
-                                      const int *candidates, int n_candidates, int *bs_out, int *nnzb_out,
-                                      double *fill_out)
+int block_size_detect(int n_rows, int n_cols, const int *row_ptr, const int *columns, int value_bytes,
+                      const int *candidates, int n_candidates, int *bs_out, int *nnzb_out, double *fill_out)
 {
   if (n_rows < 0 || n_cols < 0 || !row_ptr || !candidates || n_candidates < 1 || !bs_out || value_bytes < 1)
     return fail(BMFP_ERR_INVALID, "bad argument");
@@ -278,4 +278,42 @@ extern "C" int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr
   if (nnzb_out) *nnzb_out = (int) best_nnzb;
   if (fill_out) *fill_out = nnz ? (double) best_nnzb * best * best / nnz : 1.0;
   return BMFP_OK;
+}
+
+// A host allocation that fails (a file or arguments declaring more than the
+// host can hold) returns BMFP_ERR_NO_MEMORY instead of throwing across the C
+// ABI.
+template <class F> int guarded(const char *what, F &&f)
+{
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
+  } catch (const std::length_error &) {
+    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
+  }
+}
+
+}  // namespace
+
+extern "C" int bmfp_mm_read(const char *path, int *n_rows, int *n_cols, int *nnz, int *row_ptr, int *columns,
+                            double *values)
+{
+  return guarded("bmfp_mm_read", [&] { return mm_read(path, n_rows, n_cols, nnz, row_ptr, columns, values); });
+}
+
+extern "C" int bmfp_mm_write(const char *path, int n_rows, int n_cols, const int *row_ptr, const int *columns,
+                             const double *values)
+{
+  return guarded("bmfp_mm_write", [&] { return mm_write(path, n_rows, n_cols, row_ptr, columns, values); });
+}
+
+extern "C" int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr, const int *columns, int value_bytes,
+                                      const int *candidates, int n_candidates, int *bs_out, int *nnzb_out,
+                                      double *fill_out)
+{
+  return guarded("bmfp_block_size_detect", [&] {
+    return block_size_detect(n_rows, n_cols, row_ptr, columns, value_bytes, candidates, n_candidates, bs_out,
+                             nnzb_out, fill_out);
+  });
 }

@@ -32,7 +32,7 @@ enum {
   BMFP_OK = 0,
   BMFP_ERR_INVALID = -1,        /* bad argument / shape                      */
   BMFP_ERR_UNSUPPORTED = -2,    /* e.g. a block size with no kernel          */
-  BMFP_ERR_NO_MEMORY = -3,      /* device allocation failed                  */
+  BMFP_ERR_NO_MEMORY = -3,      /* device (or host) allocation failed        */
   BMFP_ERR_BREAKDOWN = -4,      /* BiCGStab breakdown ((r^,v)=0, rho=0, ...) */
   BMFP_ERR_NOT_CONVERGED = -5,  /* max_iterations reached                    */
   BMFP_ERR_HIP = -6,            /* HIP runtime error                         */

@@ -102,6 +102,10 @@ def test_empty_and_all_zero_matrices(tmp_path):
     ("%%MatrixMarket matrix coordinate real general\n3 3 1\n1 1 x\n", bm.ERR_INVALID),
     ("%%MatrixMarket matrix coordinate real symmetric\n3 4 1\n1 1 1.0\n", bm.ERR_INVALID),
     ("%%MatrixMarket matrix array real symmetric\n2 2\n1\n2\n3\n", bm.ERR_UNSUPPORTED),
+    # size lines declaring far more entries than the file holds: rejected
+    # without reserving memory for the declared count
+    ("%%MatrixMarket matrix coordinate real general\n3 3 2000000000\n1 1 1.0\n", bm.ERR_INVALID),
+    ("%%MatrixMarket matrix coordinate real symmetric\n3 3 9000000000000\n1 1 1.0\n", bm.ERR_INVALID),
 ])
 def test_reader_errors(tmp_path, text, status):
     path = tmp_path / "bad.mtx"
