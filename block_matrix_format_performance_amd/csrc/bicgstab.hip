@@ -289,34 +289,42 @@ bmfp_solver_t::~bmfp_solver_t()
 
 extern "C" int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes)
 {
-  if (!A || !bytes) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *bytes = solver_workspace_bytes(*A, use_jacobi);
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_bicgstab_workspace_bytes", [&]() -> int {
+    if (!A || !bytes) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *bytes = solver_workspace_bytes(*A, use_jacobi);
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_bicgstab_create(bmfp_solver *out, bmfp_matrix A, int use_jacobi)
 {
-  if (!out || !A) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
-  BMFP_TRY(solver_create(*S, A, use_jacobi));
-  *out = S.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_bicgstab_create", [&]() -> int {
+    if (!out || !A) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
+    BMFP_TRY(solver_create(*S, A, use_jacobi));
+    *out = S.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_bicgstab_solve(bmfp_solver s, const void *b, void *x, double eps, int max_iterations,
                                    bmfp_stream stream, bmfp_solve_info *info)
 {
-  if (!s || s->refine) return fail(BMFP_ERR_INVALID, "not a BiCGStab handle");
-  const int st = solver_run(*s, b, x, eps, max_iterations, static_cast<hipStream_t>(stream), info);
-  if (st == BMFP_ERR_NOT_CONVERGED) set_error("BiCGStab: max_iterations (%d) reached", max_iterations);
-  else if (st == BMFP_ERR_BREAKDOWN) set_error("BiCGStab breakdown");
-  else if (st == BMFP_ERR_NONFINITE) set_error("BiCGStab: non-finite value in the iteration");
-  return st;
+  return bmfp::guarded("bmfp_bicgstab_solve", [&]() -> int {
+    if (!s || s->refine) return fail(BMFP_ERR_INVALID, "not a BiCGStab handle");
+    const int st = solver_run(*s, b, x, eps, max_iterations, static_cast<hipStream_t>(stream), info);
+    if (st == BMFP_ERR_NOT_CONVERGED) set_error("BiCGStab: max_iterations (%d) reached", max_iterations);
+    else if (st == BMFP_ERR_BREAKDOWN) set_error("BiCGStab breakdown");
+    else if (st == BMFP_ERR_NONFINITE) set_error("BiCGStab: non-finite value in the iteration");
+    return st;
+  });
 }
 
 extern "C" int bmfp_bicgstab_destroy(bmfp_solver s)
 {
-  delete s;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_bicgstab_destroy", [&]() -> int {
+    delete s;
+    return BMFP_OK;
+  });
 }

@@ -9,6 +9,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "bmfp.h"
@@ -21,6 +23,20 @@ const char *get_error();
 
 // Records a formatted error and returns `code`.
 int fail(int code, const char *fmt, ...);
+
+// Runs the body of a C entry point: a host allocation that fails inside it
+// (arguments or files declaring more than the host can hold) returns
+// BMFP_ERR_NO_MEMORY instead of throwing across the C ABI.
+template <class F> int guarded(const char *what, F &&body)
+{
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
+  } catch (const std::length_error &) {
+    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
+  }
+}
 
 inline int hip_fail(hipError_t e, const char *what, const char *file, int line)
 {

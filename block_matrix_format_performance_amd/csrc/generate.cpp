@@ -14,26 +14,28 @@ using namespace bmfp;
 extern "C" int bmfp_gen_n_diag_bcsr(int n_rows, int blocks_per_row, int bs, int *row_ptr, int *columns,
                                     double *values)
 {
-  // matrix_converters.h:157-196
-  if (n_rows < 1 || bs < 1 || blocks_per_row < 1 || blocks_per_row > n_rows)
-    return fail(BMFP_ERR_INVALID, "gen_n_diag_bcsr: bad shape");
-  if (!row_ptr || !columns || !values) return fail(BMFP_ERR_INVALID, "gen_n_diag_bcsr: NULL array");
-  const size_t bs2 = (size_t) bs * bs;
-  size_t block = 0;
-  for (int row = 0; row < n_rows; row++) {
-    row_ptr[row] = row * blocks_per_row;
-    int first = row - blocks_per_row / 2;
-    if (row < blocks_per_row / 2) first = 0;
-    else if (row > n_rows - blocks_per_row) first = n_rows - blocks_per_row;
-    for (int k = 0; k < blocks_per_row; k++, block++) {
-      const int col = first + k;
-      columns[block] = col;
-      double *blk = values + block * bs2;
-      for (size_t i = 0; i < bs2; i++) blk[i] = ((double) col + (double) i) / (double) n_rows;
+  return bmfp::guarded("bmfp_gen_n_diag_bcsr", [&]() -> int {
+    // matrix_converters.h:157-196
+    if (n_rows < 1 || bs < 1 || blocks_per_row < 1 || blocks_per_row > n_rows)
+      return fail(BMFP_ERR_INVALID, "gen_n_diag_bcsr: bad shape");
+    if (!row_ptr || !columns || !values) return fail(BMFP_ERR_INVALID, "gen_n_diag_bcsr: NULL array");
+    const size_t bs2 = (size_t) bs * bs;
+    size_t block = 0;
+    for (int row = 0; row < n_rows; row++) {
+      row_ptr[row] = row * blocks_per_row;
+      int first = row - blocks_per_row / 2;
+      if (row < blocks_per_row / 2) first = 0;
+      else if (row > n_rows - blocks_per_row) first = n_rows - blocks_per_row;
+      for (int k = 0; k < blocks_per_row; k++, block++) {
+        const int col = first + k;
+        columns[block] = col;
+        double *blk = values + block * bs2;
+        for (size_t i = 0; i < bs2; i++) blk[i] = ((double) col + (double) i) / (double) n_rows;
+      }
     }
-  }
-  row_ptr[n_rows] = n_rows * blocks_per_row;
-  return BMFP_OK;
+    row_ptr[n_rows] = n_rows * blocks_per_row;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int64_t bmfp_stencil_nnzb(int nx, int ny)
@@ -112,30 +114,32 @@ void stencil_rows(int nx, int ny, int bs, int kind, uint64_t seed, int iy0, int 
 extern "C" int bmfp_gen_stencil9(int nx, int ny, int bs, int kind, uint64_t seed, int *row_ptr, int *columns,
                                  double *values)
 {
-  if (nx < 1 || ny < 1 || bs < 1 || (kind != 0 && kind != 1))
-    return fail(BMFP_ERR_INVALID, "gen_stencil9: bad arguments (nx %d ny %d bs %d kind %d)", nx, ny, bs, kind);
-  if (!row_ptr || !columns || !values) return fail(BMFP_ERR_INVALID, "gen_stencil9: NULL array");
-  if ((int64_t) nx * ny > INT32_MAX || bmfp_stencil_nnzb(nx, ny) * bs * bs > INT32_MAX)
-    return fail(BMFP_ERR_INVALID, "gen_stencil9: matrix too large for int32 indices");
-  // row pointers: count of in-grid neighbours (3x3 window clipped at edges)
-  int64_t nb = 0;
-  row_ptr[0] = 0;
-  for (int iy = 0; iy < ny; iy++) {
-    const int wy = 3 - (iy == 0) - (iy == ny - 1);
-    for (int ix = 0; ix < nx; ix++) {
-      const int wx = 3 - (ix == 0) - (ix == nx - 1);
-      nb += wy * wx;
-      row_ptr[(int64_t) iy * nx + ix + 1] = (int) nb;
+  return bmfp::guarded("bmfp_gen_stencil9", [&]() -> int {
+    if (nx < 1 || ny < 1 || bs < 1 || (kind != 0 && kind != 1))
+      return fail(BMFP_ERR_INVALID, "gen_stencil9: bad arguments (nx %d ny %d bs %d kind %d)", nx, ny, bs, kind);
+    if (!row_ptr || !columns || !values) return fail(BMFP_ERR_INVALID, "gen_stencil9: NULL array");
+    if ((int64_t) nx * ny > INT32_MAX || bmfp_stencil_nnzb(nx, ny) * bs * bs > INT32_MAX)
+      return fail(BMFP_ERR_INVALID, "gen_stencil9: matrix too large for int32 indices");
+    // row pointers: count of in-grid neighbours (3x3 window clipped at edges)
+    int64_t nb = 0;
+    row_ptr[0] = 0;
+    for (int iy = 0; iy < ny; iy++) {
+      const int wy = 3 - (iy == 0) - (iy == ny - 1);
+      for (int ix = 0; ix < nx; ix++) {
+        const int wx = 3 - (ix == 0) - (ix == nx - 1);
+        nb += wy * wx;
+        row_ptr[(int64_t) iy * nx + ix + 1] = (int) nb;
+      }
     }
-  }
-  unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
-  if ((int64_t) nb * bs * bs < (1 << 20)) nt = 1;
-  nt = std::min<unsigned>(nt, (unsigned) ny);
-  std::vector<std::thread> pool;
-  for (unsigned t = 0; t < nt; t++) {
-    const int y0 = (int) ((int64_t) ny * t / nt), y1 = (int) ((int64_t) ny * (t + 1) / nt);
-    pool.emplace_back(stencil_rows, nx, ny, bs, kind, seed, y0, y1, row_ptr, columns, values);
-  }
-  for (auto &th : pool) th.join();
-  return BMFP_OK;
+    unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    if ((int64_t) nb * bs * bs < (1 << 20)) nt = 1;
+    nt = std::min<unsigned>(nt, (unsigned) ny);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++) {
+      const int y0 = (int) ((int64_t) ny * t / nt), y1 = (int) ((int64_t) ny * (t + 1) / nt);
+      pool.emplace_back(stencil_rows, nx, ny, bs, kind, seed, y0, y1, row_ptr, columns, values);
+    }
+    for (auto &th : pool) th.join();
+    return BMFP_OK;
+  });
 }

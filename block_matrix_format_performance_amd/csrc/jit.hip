@@ -153,8 +153,10 @@ using namespace bmfp;
 
 extern "C" int bmfp_jit_build(int bs, int dtype, int layout, size_t *code_bytes, double *seconds)
 {
-  std::vector<char> code;
-  BMFP_TRY(jit_compile(bs, dtype, layout, "gfx950", &code, seconds));
-  if (code_bytes) *code_bytes = code.size();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_jit_build", [&]() -> int {
+    std::vector<char> code;
+    BMFP_TRY(jit_compile(bs, dtype, layout, "gfx950", &code, seconds));
+    if (code_bytes) *code_bytes = code.size();
+    return BMFP_OK;
+  });
 }

@@ -16,8 +16,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <new>
-#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -280,39 +278,25 @@ int block_size_detect(int n_rows, int n_cols, const int *row_ptr, const int *col
   return BMFP_OK;
 }
 
-// A host allocation that fails (a file or arguments declaring more than the
-// host can hold) returns BMFP_ERR_NO_MEMORY instead of throwing across the C
-// ABI.
-template <class F> int guarded(const char *what, F &&f)
-{
-  try {
-    return f();
-  } catch (const std::bad_alloc &) {
-    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
-  } catch (const std::length_error &) {
-    return fail(BMFP_ERR_NO_MEMORY, "%s: host allocation failed", what);
-  }
-}
-
 }  // namespace
 
 extern "C" int bmfp_mm_read(const char *path, int *n_rows, int *n_cols, int *nnz, int *row_ptr, int *columns,
                             double *values)
 {
-  return guarded("bmfp_mm_read", [&] { return mm_read(path, n_rows, n_cols, nnz, row_ptr, columns, values); });
+  return bmfp::guarded("bmfp_mm_read", [&] { return mm_read(path, n_rows, n_cols, nnz, row_ptr, columns, values); });
 }
 
 extern "C" int bmfp_mm_write(const char *path, int n_rows, int n_cols, const int *row_ptr, const int *columns,
                              const double *values)
 {
-  return guarded("bmfp_mm_write", [&] { return mm_write(path, n_rows, n_cols, row_ptr, columns, values); });
+  return bmfp::guarded("bmfp_mm_write", [&] { return mm_write(path, n_rows, n_cols, row_ptr, columns, values); });
 }
 
 extern "C" int bmfp_block_size_detect(int n_rows, int n_cols, const int *row_ptr, const int *columns, int value_bytes,
                                       const int *candidates, int n_candidates, int *bs_out, int *nnzb_out,
                                       double *fill_out)
 {
-  return guarded("bmfp_block_size_detect", [&] {
+  return bmfp::guarded("bmfp_block_size_detect", [&] {
     return block_size_detect(n_rows, n_cols, row_ptr, columns, value_bytes, candidates, n_candidates, bs_out,
                              nnzb_out, fill_out);
   });

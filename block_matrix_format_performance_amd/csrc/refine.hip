@@ -221,26 +221,32 @@ using namespace bmfp;
 
 extern "C" int bmfp_refine_create(bmfp_solver *out, bmfp_matrix A64, int inner_dtype, int use_jacobi)
 {
-  if (!out || !A64) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  std::unique_ptr<bmfp_solver_t> H(new bmfp_solver_t);
-  BMFP_TRY(refine_create(*H, A64, inner_dtype, use_jacobi));
-  *out = H.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_refine_create", [&]() -> int {
+    if (!out || !A64) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    std::unique_ptr<bmfp_solver_t> H(new bmfp_solver_t);
+    BMFP_TRY(refine_create(*H, A64, inner_dtype, use_jacobi));
+    *out = H.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps, double inner_tol,
                                  int max_outer, int max_inner, bmfp_stream stream, bmfp_solve_info *info)
 {
-  if (!s || !s->refine) return fail(BMFP_ERR_INVALID, "not a refinement handle");
-  const int st = refine_run(*s->refine, b, x, eps, inner_tol, max_outer, max_inner,
-                            static_cast<hipStream_t>(stream), info);
-  if (st == BMFP_ERR_NOT_CONVERGED) set_error("refinement: max_outer (%d) reached", max_outer);
-  return st;
+  return bmfp::guarded("bmfp_refine_solve", [&]() -> int {
+    if (!s || !s->refine) return fail(BMFP_ERR_INVALID, "not a refinement handle");
+    const int st = refine_run(*s->refine, b, x, eps, inner_tol, max_outer, max_inner,
+                              static_cast<hipStream_t>(stream), info);
+    if (st == BMFP_ERR_NOT_CONVERGED) set_error("refinement: max_outer (%d) reached", max_outer);
+    return st;
+  });
 }
 
 extern "C" int bmfp_refine_destroy(bmfp_solver s)
 {
-  delete s;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_refine_destroy", [&]() -> int {
+    delete s;
+    return BMFP_OK;
+  });
 }

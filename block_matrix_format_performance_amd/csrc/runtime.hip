@@ -154,199 +154,243 @@ extern "C" const char *bmfp_status_string(int status)
   }
 }
 
-extern "C" int bmfp_version(void) { return BMFP_VERSION; }
+extern "C" int bmfp_version(void) { return bmfp::guarded("bmfp_version", [&]() -> int { return BMFP_VERSION; }); }
 
 // ---- device / memory / streams ---------------------------------------------------
 extern "C" int bmfp_device_count(int *count)
 {
-  if (!count) return fail(BMFP_ERR_INVALID, "count is NULL");
-  int n = 0;
-  hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess) n = 0;
-  *count = n;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_device_count", [&]() -> int {
+    if (!count) return fail(BMFP_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_init(int device)
 {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
-    return fail(BMFP_ERR_NO_DEVICE, "no HIP device visible");
-  if (device < 0 || device >= n) return fail(BMFP_ERR_INVALID, "device %d out of range [0,%d)", device, n);
-  BMFP_HIP(hipSetDevice(device));
-  hipDeviceProp_t p;
-  BMFP_HIP(hipGetDeviceProperties(&p, device));
-  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
-    return fail(BMFP_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, p.gcnArchName);
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_init", [&]() -> int {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      return fail(BMFP_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(BMFP_ERR_INVALID, "device %d out of range [0,%d)", device, n);
+    BMFP_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    BMFP_HIP(hipGetDeviceProperties(&p, device));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+      return fail(BMFP_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, p.gcnArchName);
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_device_info(char *name, size_t name_len, size_t *total_mem, int *cus, int *arch)
 {
-  int dev = 0;
-  BMFP_HIP(hipGetDevice(&dev));
-  hipDeviceProp_t p;
-  BMFP_HIP(hipGetDeviceProperties(&p, dev));
-  if (name && name_len) {
-    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
-  }
-  if (total_mem) *total_mem = p.totalGlobalMem;
-  if (cus) *cus = p.multiProcessorCount;
-  if (arch) *arch = p.major * 100 + p.minor;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_device_info", [&]() -> int {
+    int dev = 0;
+    BMFP_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    BMFP_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len) {
+      std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (cus) *cus = p.multiProcessorCount;
+    if (arch) *arch = p.major * 100 + p.minor;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_malloc(void **ptr, size_t bytes)
 {
-  if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
-  BMFP_HIP(hipMalloc(ptr, bytes ? bytes : 256));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_malloc", [&]() -> int {
+    if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
+    BMFP_HIP(hipMalloc(ptr, bytes ? bytes : 256));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_free(void *ptr)
 {
-  if (ptr) BMFP_HIP(hipFree(ptr));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_free", [&]() -> int {
+    if (ptr) BMFP_HIP(hipFree(ptr));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_malloc_host(void **ptr, size_t bytes)
 {
-  if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
-  BMFP_HIP(hipHostMalloc(ptr, bytes ? bytes : 64, hipHostMallocDefault));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_malloc_host", [&]() -> int {
+    if (!ptr) return fail(BMFP_ERR_INVALID, "ptr is NULL");
+    BMFP_HIP(hipHostMalloc(ptr, bytes ? bytes : 64, hipHostMallocDefault));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_free_host(void *ptr)
 {
-  if (ptr) BMFP_HIP(hipHostFree(ptr));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_free_host", [&]() -> int {
+    if (ptr) BMFP_HIP(hipHostFree(ptr));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_memcpy_h2d", [&]() -> int {
+    if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_memcpy_d2h", [&]() -> int {
+    if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
-  if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_memcpy_d2d", [&]() -> int {
+    if (bytes) BMFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_memset(void *dst, int value, size_t bytes)
 {
-  if (bytes) BMFP_HIP(hipMemset(dst, value, bytes));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_memset", [&]() -> int {
+    if (bytes) BMFP_HIP(hipMemset(dst, value, bytes));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stream_create(bmfp_stream *stream)
 {
-  if (!stream) return fail(BMFP_ERR_INVALID, "stream is NULL");
-  hipStream_t s;
-  BMFP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  *stream = s;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stream_create", [&]() -> int {
+    if (!stream) return fail(BMFP_ERR_INVALID, "stream is NULL");
+    hipStream_t s;
+    BMFP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stream_destroy(bmfp_stream stream)
 {
-  if (stream) BMFP_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stream_destroy", [&]() -> int {
+    if (stream) BMFP_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stream_synchronize(bmfp_stream stream)
 {
-  BMFP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stream_synchronize", [&]() -> int {
+    BMFP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_device_synchronize(void)
 {
-  BMFP_HIP(hipDeviceSynchronize());
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_device_synchronize", [&]() -> int {
+    BMFP_HIP(hipDeviceSynchronize());
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_event_create(bmfp_event *event)
 {
-  if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
-  hipEvent_t e;
-  BMFP_HIP(hipEventCreate(&e));
-  *event = e;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_event_create", [&]() -> int {
+    if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
+    hipEvent_t e;
+    BMFP_HIP(hipEventCreate(&e));
+    *event = e;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_event_destroy(bmfp_event event)
 {
-  if (event) BMFP_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_event_destroy", [&]() -> int {
+    if (event) BMFP_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_event_record(bmfp_event event, bmfp_stream stream)
 {
-  if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
-  BMFP_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_event_record", [&]() -> int {
+    if (!event) return fail(BMFP_ERR_INVALID, "event is NULL");
+    BMFP_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_event_elapsed(bmfp_event start, bmfp_event stop, double *seconds)
 {
-  if (!start || !stop || !seconds) return fail(BMFP_ERR_INVALID, "NULL argument");
-  BMFP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
-  float ms = 0.f;
-  BMFP_HIP(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
-  *seconds = ms * 1e-3;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_event_elapsed", [&]() -> int {
+    if (!start || !stop || !seconds) return fail(BMFP_ERR_INVALID, "NULL argument");
+    BMFP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    float ms = 0.f;
+    BMFP_HIP(hipEventElapsedTime(&ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    *seconds = ms * 1e-3;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stream_copy(void *dst, const void *src, size_t bytes, bmfp_stream stream)
 {
-  if (bytes == 0) return BMFP_OK;
-  if (!dst || !src || bytes % 16 || ((uintptr_t) dst | (uintptr_t) src) % 16)
-    return fail(BMFP_ERR_INVALID, "bmfp_stream_copy: needs 16-byte aligned pointers and size");
-  const int64_t n16 = (int64_t) (bytes / 16);
-  const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
-  k_stream_copy<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<u32x4c *>(dst),
-                                                                        static_cast<const u32x4c *>(src), n16);
-  BMFP_LAUNCHED();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stream_copy", [&]() -> int {
+    if (bytes == 0) return BMFP_OK;
+    if (!dst || !src || bytes % 16 || ((uintptr_t) dst | (uintptr_t) src) % 16)
+      return fail(BMFP_ERR_INVALID, "bmfp_stream_copy: needs 16-byte aligned pointers and size");
+    const int64_t n16 = (int64_t) (bytes / 16);
+    const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
+    k_stream_copy<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<u32x4c *>(dst),
+                                                                          static_cast<const u32x4c *>(src), n16);
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stream_read(const void *src, size_t bytes, void *sink, bmfp_stream stream)
 {
-  if (bytes == 0) return BMFP_OK;
-  if (!src || !sink || bytes % 16 || (uintptr_t) src % 16 || (uintptr_t) sink % 4)
-    return fail(BMFP_ERR_INVALID, "bmfp_stream_read: needs a 16-byte aligned source and size and a sink");
-  const int64_t n16 = (int64_t) (bytes / 16);
-  const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
-  k_stream_read<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const u32x4c *>(src), n16,
-                                                                        static_cast<unsigned *>(sink));
-  BMFP_LAUNCHED();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stream_read", [&]() -> int {
+    if (bytes == 0) return BMFP_OK;
+    if (!src || !sink || bytes % 16 || (uintptr_t) src % 16 || (uintptr_t) sink % 4)
+      return fail(BMFP_ERR_INVALID, "bmfp_stream_read: needs a 16-byte aligned source and size and a sink");
+    const int64_t n16 = (int64_t) (bytes / 16);
+    const unsigned groups = (unsigned) std::min<int64_t>((n16 + 255) / 256, 256 * 32);
+    k_stream_read<<<groups, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const u32x4c *>(src), n16,
+                                                                          static_cast<unsigned *>(sink));
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_fill(void *dst, int dtype, size_t n, int kind, uint64_t seed, uint64_t stream_id,
                          double value, bmfp_stream stream)
 {
-  if (n == 0) return BMFP_OK;
-  if (!dst || kind < 0 || kind > 2) return fail(BMFP_ERR_INVALID, "bmfp_fill: bad arguments");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const uint64_t key = rng_key(seed, stream_id);
-  const unsigned groups = (unsigned) std::min<size_t>((n + 255) / 256, 8192);
-  if (dtype == BMFP_F64)
-    k_fill<double><<<groups, 256, 0, s>>>(static_cast<double *>(dst), (int64_t) n, kind, key, value);
-  else if (dtype == BMFP_F32)
-    k_fill<float><<<groups, 256, 0, s>>>(static_cast<float *>(dst), (int64_t) n, kind, key, value);
-  else
-    return fail(BMFP_ERR_INVALID, "bmfp_fill: dtype must be F64 or F32");
-  BMFP_LAUNCHED();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_fill", [&]() -> int {
+    if (n == 0) return BMFP_OK;
+    if (!dst || kind < 0 || kind > 2) return fail(BMFP_ERR_INVALID, "bmfp_fill: bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t key = rng_key(seed, stream_id);
+    const unsigned groups = (unsigned) std::min<size_t>((n + 255) / 256, 8192);
+    if (dtype == BMFP_F64)
+      k_fill<double><<<groups, 256, 0, s>>>(static_cast<double *>(dst), (int64_t) n, kind, key, value);
+    else if (dtype == BMFP_F32)
+      k_fill<float><<<groups, 256, 0, s>>>(static_cast<float *>(dst), (int64_t) n, kind, key, value);
+    else
+      return fail(BMFP_ERR_INVALID, "bmfp_fill: dtype must be F64 or F32");
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  });
 }
 
 // ---- matrices -------------------------------------------------------------------
@@ -453,255 +497,275 @@ int convert_values(bmfp_matrix_t &m, const double *src, int src_layout, hipStrea
 extern "C" int bmfp_bcsr_create(bmfp_matrix *out, int dtype, int layout, int n_rows, int n_cols, int bs, int nnzb,
                                 const int *row_ptr, const int *columns, const double *values)
 {
-  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
-  *out = nullptr;
-  if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
-  if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
-  int mx = 0;
-  BMFP_TRY(check_structure(n_rows, n_cols, bs, nnzb, row_ptr, columns, &mx));
-  if (nnzb && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
-  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
-  m->is_bcsr = 1;
-  m->dtype = dtype;
-  m->layout = layout;
-  m->n_rows = n_rows;
-  m->n_cols = n_cols;
-  m->bs = bs;
-  m->nnzb = nnzb;
-  m->max_row_blocks = mx;
-  m->mean_row_blocks = n_rows ? (double) nnzb / n_rows : 0.0;
-  BMFP_HIP(hipGetDevice(&m->device));
-  BMFP_TRY(upload_indices(*m, row_ptr, columns));
-  const size_t nv = (size_t) m->n_vals();
-  if (dtype == BMFP_F64) {
-    BMFP_TRY(m->values.alloc(nv * sizeof(double)));
-    if (nv) BMFP_HIP(hipMemcpy(m->values.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
-  } else {
-    DeviceBuffer staging;
-    BMFP_TRY(staging.alloc(nv * sizeof(double)));
-    if (nv) BMFP_HIP(hipMemcpy(staging.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
-    BMFP_TRY(convert_values(*m, staging.as<double>(), layout, nullptr));
-    BMFP_HIP(hipDeviceSynchronize());
-  }
-  *out = m.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_bcsr_create", [&]() -> int {
+    if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
+    if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
+    int mx = 0;
+    BMFP_TRY(check_structure(n_rows, n_cols, bs, nnzb, row_ptr, columns, &mx));
+    if (nnzb && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 1;
+    m->dtype = dtype;
+    m->layout = layout;
+    m->n_rows = n_rows;
+    m->n_cols = n_cols;
+    m->bs = bs;
+    m->nnzb = nnzb;
+    m->max_row_blocks = mx;
+    m->mean_row_blocks = n_rows ? (double) nnzb / n_rows : 0.0;
+    BMFP_HIP(hipGetDevice(&m->device));
+    BMFP_TRY(upload_indices(*m, row_ptr, columns));
+    const size_t nv = (size_t) m->n_vals();
+    if (dtype == BMFP_F64) {
+      BMFP_TRY(m->values.alloc(nv * sizeof(double)));
+      if (nv) BMFP_HIP(hipMemcpy(m->values.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      DeviceBuffer staging;
+      BMFP_TRY(staging.alloc(nv * sizeof(double)));
+      if (nv) BMFP_HIP(hipMemcpy(staging.ptr, values, nv * sizeof(double), hipMemcpyHostToDevice));
+      BMFP_TRY(convert_values(*m, staging.as<double>(), layout, nullptr));
+      BMFP_HIP(hipDeviceSynchronize());
+    }
+    *out = m.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_cols, int nnz, const int *row_ptr,
                                const int *columns, const double *values)
 {
-  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
-  *out = nullptr;
-  if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
-  int mx = 0;
-  BMFP_TRY(check_structure(n_rows, n_cols, 1, nnz, row_ptr, columns, &mx));
-  if (nnz && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
-  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
-  m->is_bcsr = 0;
-  m->dtype = dtype;
-  m->n_rows = n_rows;
-  m->n_cols = n_cols;
-  m->bs = 1;
-  m->nnzb = nnz;
-  m->max_row_blocks = mx;
-  m->mean_row_blocks = n_rows ? (double) nnz / n_rows : 0.0;
-  m->csr_group = csr_group_for(m->mean_row_blocks);
-  BMFP_HIP(hipGetDevice(&m->device));
-  BMFP_TRY(upload_indices(*m, row_ptr, columns));
-  BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(n_rows, nnz)));
-  if (dtype == BMFP_F64) {
-    BMFP_TRY(m->values.alloc(sizeof(double) * (size_t) nnz));
-    if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, values, sizeof(double) * (size_t) nnz, hipMemcpyHostToDevice));
-  } else {
-    std::vector<float> f(values, values + nnz);
-    BMFP_TRY(m->values.alloc(sizeof(float) * (size_t) nnz));
-    if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, f.data(), sizeof(float) * (size_t) nnz, hipMemcpyHostToDevice));
-  }
-  *out = m.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_csr_create", [&]() -> int {
+    if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
+    int mx = 0;
+    BMFP_TRY(check_structure(n_rows, n_cols, 1, nnz, row_ptr, columns, &mx));
+    if (nnz && !values) return fail(BMFP_ERR_INVALID, "values is NULL");
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 0;
+    m->dtype = dtype;
+    m->n_rows = n_rows;
+    m->n_cols = n_cols;
+    m->bs = 1;
+    m->nnzb = nnz;
+    m->max_row_blocks = mx;
+    m->mean_row_blocks = n_rows ? (double) nnz / n_rows : 0.0;
+    m->csr_group = csr_group_for(m->mean_row_blocks);
+    BMFP_HIP(hipGetDevice(&m->device));
+    BMFP_TRY(upload_indices(*m, row_ptr, columns));
+    BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(n_rows, nnz)));
+    if (dtype == BMFP_F64) {
+      BMFP_TRY(m->values.alloc(sizeof(double) * (size_t) nnz));
+      if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, values, sizeof(double) * (size_t) nnz, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(values, values + nnz);
+      BMFP_TRY(m->values.alloc(sizeof(float) * (size_t) nnz));
+      if (nnz) BMFP_HIP(hipMemcpy(m->values.ptr, f.data(), sizeof(float) * (size_t) nnz, hipMemcpyHostToDevice));
+    }
+    *out = m.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int kind, uint64_t seed, int dtype,
                                    int layout)
 {
-  if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
-  *out = nullptr;
-  if (nx < 1 || ny < 1 || bs < 1 || (kind != 0 && kind != 1))
-    return fail(BMFP_ERR_INVALID, "bad stencil (nx %d, ny %d, bs %d, kind %d)", nx, ny, bs, kind);
-  if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
-  if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
-  const int64_t nodes = (int64_t) nx * ny, nnzb = stencil_block_count(nx, ny);
-  if (nnzb * bs * bs > INT_MAX || nodes * bs > INT_MAX) return fail(BMFP_ERR_INVALID, "stencil too large for int32 indices");
-  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
-  m->is_bcsr = 1;
-  m->dtype = dtype;
-  m->layout = layout;
-  m->n_rows = m->n_cols = (int) nodes;
-  m->bs = bs;
-  m->nnzb = (int) nnzb;
-  m->max_row_blocks = std::min(nx, 3) * std::min(ny, 3);
-  m->mean_row_blocks = (double) nnzb / (double) nodes;
-  BMFP_HIP(hipGetDevice(&m->device));
-  std::vector<int> rp;
-  stencil_row_ptr(nx, ny, rp);
-  m->row_ptr = std::make_shared<DeviceBuffer>();
-  m->columns = std::make_shared<DeviceBuffer>();
-  BMFP_TRY(m->row_ptr->alloc(sizeof(int) * rp.size()));
-  BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnzb));
-  BMFP_HIP(hipMemcpy(m->row_ptr->ptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice));
-  const size_t nv = (size_t) m->n_vals();
-  const bool direct = dtype == BMFP_F64 && layout == BMFP_ROW_MAJOR;
-  DeviceBuffer staging;
-  double *gen = nullptr;
-  if (direct) {
-    BMFP_TRY(m->values.alloc(nv * sizeof(double)));
-    gen = m->values.as<double>();
-  } else {
-    BMFP_TRY(staging.alloc(nv * sizeof(double)));
-    gen = staging.as<double>();
-  }
-  const int64_t rows = nodes * bs;
-  k_gen_stencil_rows<<<(unsigned) ((rows + 255) / 256), 256, 0, nullptr>>>(nx, ny, bs, kind, seed,
-                                                                          m->row_ptr->as<int>(),
-                                                                          m->columns->as<int>(), gen, nullptr);
-  BMFP_LAUNCHED();
-  if (!direct) BMFP_TRY(convert_values(*m, gen, BMFP_ROW_MAJOR, nullptr));
-  BMFP_HIP(hipDeviceSynchronize());
-  *out = m.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_stencil_create", [&]() -> int {
+    if (!out) return fail(BMFP_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || bs < 1 || (kind != 0 && kind != 1))
+      return fail(BMFP_ERR_INVALID, "bad stencil (nx %d, ny %d, bs %d, kind %d)", nx, ny, bs, kind);
+    if (dtype < BMFP_F64 || dtype > BMFP_F16) return fail(BMFP_ERR_INVALID, "bad dtype %d", dtype);
+    if (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR) return fail(BMFP_ERR_INVALID, "bad layout %d", layout);
+    const int64_t nodes = (int64_t) nx * ny, nnzb = stencil_block_count(nx, ny);
+    if (nnzb * bs * bs > INT_MAX || nodes * bs > INT_MAX) return fail(BMFP_ERR_INVALID, "stencil too large for int32 indices");
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 1;
+    m->dtype = dtype;
+    m->layout = layout;
+    m->n_rows = m->n_cols = (int) nodes;
+    m->bs = bs;
+    m->nnzb = (int) nnzb;
+    m->max_row_blocks = std::min(nx, 3) * std::min(ny, 3);
+    m->mean_row_blocks = (double) nnzb / (double) nodes;
+    BMFP_HIP(hipGetDevice(&m->device));
+    std::vector<int> rp;
+    stencil_row_ptr(nx, ny, rp);
+    m->row_ptr = std::make_shared<DeviceBuffer>();
+    m->columns = std::make_shared<DeviceBuffer>();
+    BMFP_TRY(m->row_ptr->alloc(sizeof(int) * rp.size()));
+    BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnzb));
+    BMFP_HIP(hipMemcpy(m->row_ptr->ptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice));
+    const size_t nv = (size_t) m->n_vals();
+    const bool direct = dtype == BMFP_F64 && layout == BMFP_ROW_MAJOR;
+    DeviceBuffer staging;
+    double *gen = nullptr;
+    if (direct) {
+      BMFP_TRY(m->values.alloc(nv * sizeof(double)));
+      gen = m->values.as<double>();
+    } else {
+      BMFP_TRY(staging.alloc(nv * sizeof(double)));
+      gen = staging.as<double>();
+    }
+    const int64_t rows = nodes * bs;
+    k_gen_stencil_rows<<<(unsigned) ((rows + 255) / 256), 256, 0, nullptr>>>(nx, ny, bs, kind, seed,
+                                                                            m->row_ptr->as<int>(),
+                                                                            m->columns->as<int>(), gen, nullptr);
+    BMFP_LAUNCHED();
+    if (!direct) BMFP_TRY(convert_values(*m, gen, BMFP_ROW_MAJOR, nullptr));
+    BMFP_HIP(hipDeviceSynchronize());
+    *out = m.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_convert(bmfp_matrix src, int dtype, int layout, bmfp_matrix *out, bmfp_stream stream)
 {
-  if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (!src->is_bcsr || src->dtype != BMFP_F64)
-    return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_convert needs an F64 BCSR source");
-  if (dtype < BMFP_F64 || dtype > BMFP_F16 || (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR))
-    return fail(BMFP_ERR_INVALID, "bad dtype/layout");
-  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
-  m->is_bcsr = 1;
-  m->dtype = dtype;
-  m->layout = layout;
-  m->n_rows = src->n_rows;
-  m->n_cols = src->n_cols;
-  m->bs = src->bs;
-  m->nnzb = src->nnzb;
-  m->max_row_blocks = src->max_row_blocks;
-  m->mean_row_blocks = src->mean_row_blocks;
-  m->device = src->device;
-  m->row_ptr = src->row_ptr;
-  m->columns = src->columns;
-  BMFP_TRY(convert_values(*m, src->values.as<double>(), src->layout, static_cast<hipStream_t>(stream)));
-  *out = m.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_convert", [&]() -> int {
+    if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (!src->is_bcsr || src->dtype != BMFP_F64)
+      return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_convert needs an F64 BCSR source");
+    if (dtype < BMFP_F64 || dtype > BMFP_F16 || (layout != BMFP_ROW_MAJOR && layout != BMFP_COL_MAJOR))
+      return fail(BMFP_ERR_INVALID, "bad dtype/layout");
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 1;
+    m->dtype = dtype;
+    m->layout = layout;
+    m->n_rows = src->n_rows;
+    m->n_cols = src->n_cols;
+    m->bs = src->bs;
+    m->nnzb = src->nnzb;
+    m->max_row_blocks = src->max_row_blocks;
+    m->mean_row_blocks = src->mean_row_blocks;
+    m->device = src->device;
+    m->row_ptr = src->row_ptr;
+    m->columns = src->columns;
+    BMFP_TRY(convert_values(*m, src->values.as<double>(), src->layout, static_cast<hipStream_t>(stream)));
+    *out = m.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_destroy(bmfp_matrix m)
 {
-  delete m;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_destroy", [&]() -> int {
+    delete m;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_shape(bmfp_matrix m, int *is_bcsr, int *dtype, int *layout, int *n_rows, int *n_cols,
                                  int *bs, int *nnzb)
 {
-  if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
-  if (is_bcsr) *is_bcsr = m->is_bcsr;
-  if (dtype) *dtype = m->dtype;
-  if (layout) *layout = m->layout;
-  if (n_rows) *n_rows = m->n_rows;
-  if (n_cols) *n_cols = m->n_cols;
-  if (bs) *bs = m->bs;
-  if (nnzb) *nnzb = m->nnzb;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_shape", [&]() -> int {
+    if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
+    if (is_bcsr) *is_bcsr = m->is_bcsr;
+    if (dtype) *dtype = m->dtype;
+    if (layout) *layout = m->layout;
+    if (n_rows) *n_rows = m->n_rows;
+    if (n_cols) *n_cols = m->n_cols;
+    if (bs) *bs = m->bs;
+    if (nnzb) *nnzb = m->nnzb;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_spmv_traffic(bmfp_matrix m, double *bytes, double *flops)
 {
-  if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
-  const double nv = (double) m->n_vals();
-  const double vb = (double) dtype_size(m->dtype), xb = (double) vec_size(m->dtype);
-  double b = nv * vb + 4.0 * m->nnzb + 4.0 * (m->n_rows + 1.0) + xb * (double) m->x_len() + xb * (double) m->y_len();
-  if (m->dtype == BMFP_F16) b += 4.0 * m->n_rows;
-  if (bytes) *bytes = b;
-  if (flops) *flops = 2.0 * nv;
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_spmv_traffic", [&]() -> int {
+    if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
+    const double nv = (double) m->n_vals();
+    const double vb = (double) dtype_size(m->dtype), xb = (double) vec_size(m->dtype);
+    double b = nv * vb + 4.0 * m->nnzb + 4.0 * (m->n_rows + 1.0) + xb * (double) m->x_len() + xb * (double) m->y_len();
+    if (m->dtype == BMFP_F16) b += 4.0 * m->n_rows;
+    if (bytes) *bytes = b;
+    if (flops) *flops = 2.0 * nv;
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_to_csr(bmfp_matrix src, int dtype, bmfp_matrix *out, bmfp_stream stream)
 {
-  if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (!src->is_bcsr || src->dtype != BMFP_F64)
-    return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_to_csr needs an F64 BCSR source");
-  if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
-  const int64_t nnz = (int64_t) src->nnzb * src->bs * src->bs, rows = src->y_len();
-  if (nnz > INT_MAX || rows > INT_MAX) return fail(BMFP_ERR_INVALID, "expanded matrix too large for int32");
-  std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
-  m->is_bcsr = 0;
-  m->dtype = dtype;
-  m->n_rows = (int) rows;
-  m->n_cols = (int) src->x_len();
-  m->bs = 1;
-  m->nnzb = (int) nnz;
-  m->max_row_blocks = src->max_row_blocks * src->bs;
-  m->mean_row_blocks = src->mean_row_blocks * src->bs;
-  m->csr_group = csr_group_for(m->mean_row_blocks);
-  m->device = src->device;
-  m->row_ptr = std::make_shared<DeviceBuffer>();
-  m->columns = std::make_shared<DeviceBuffer>();
-  BMFP_TRY(m->row_ptr->alloc(sizeof(int) * (size_t) (rows + 1)));
-  BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnz));
-  BMFP_TRY(m->values.alloc((dtype == BMFP_F64 ? sizeof(double) : sizeof(float)) * (size_t) nnz));
-  BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(rows, nnz)));
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const unsigned groups = (unsigned) ((rows + 1 + 255) / 256);
-  const int *rp = src->row_ptr->as<int>(), *cols = src->columns->as<int>();
-  if (dtype == BMFP_F64)
-    k_bcsr_to_csr<double><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
-                                                 m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<double>());
-  else
-    k_bcsr_to_csr<float><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
-                                                m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<float>());
-  BMFP_LAUNCHED();
-  BMFP_HIP(hipStreamSynchronize(s));
-  *out = m.release();
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_to_csr", [&]() -> int {
+    if (!src || !out) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (!src->is_bcsr || src->dtype != BMFP_F64)
+      return fail(BMFP_ERR_UNSUPPORTED, "bmfp_matrix_to_csr needs an F64 BCSR source");
+    if (dtype != BMFP_F64 && dtype != BMFP_F32) return fail(BMFP_ERR_INVALID, "CSR dtype must be F64 or F32");
+    const int64_t nnz = (int64_t) src->nnzb * src->bs * src->bs, rows = src->y_len();
+    if (nnz > INT_MAX || rows > INT_MAX) return fail(BMFP_ERR_INVALID, "expanded matrix too large for int32");
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 0;
+    m->dtype = dtype;
+    m->n_rows = (int) rows;
+    m->n_cols = (int) src->x_len();
+    m->bs = 1;
+    m->nnzb = (int) nnz;
+    m->max_row_blocks = src->max_row_blocks * src->bs;
+    m->mean_row_blocks = src->mean_row_blocks * src->bs;
+    m->csr_group = csr_group_for(m->mean_row_blocks);
+    m->device = src->device;
+    m->row_ptr = std::make_shared<DeviceBuffer>();
+    m->columns = std::make_shared<DeviceBuffer>();
+    BMFP_TRY(m->row_ptr->alloc(sizeof(int) * (size_t) (rows + 1)));
+    BMFP_TRY(m->columns->alloc(sizeof(int) * (size_t) nnz));
+    BMFP_TRY(m->values.alloc((dtype == BMFP_F64 ? sizeof(double) : sizeof(float)) * (size_t) nnz));
+    BMFP_TRY(m->merge_carry.alloc(merge_carry_bytes(rows, nnz)));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned groups = (unsigned) ((rows + 1 + 255) / 256);
+    const int *rp = src->row_ptr->as<int>(), *cols = src->columns->as<int>();
+    if (dtype == BMFP_F64)
+      k_bcsr_to_csr<double><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
+                                                   m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<double>());
+    else
+      k_bcsr_to_csr<float><<<groups, 256, 0, s>>>(src->n_rows, src->bs, src->layout, rp, cols, src->values.as<double>(),
+                                                  m->row_ptr->as<int>(), m->columns->as<int>(), m->values.as<float>());
+    BMFP_LAUNCHED();
+    BMFP_HIP(hipStreamSynchronize(s));
+    *out = m.release();
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_download_structure(bmfp_matrix m, int *row_ptr, int *columns)
 {
-  if (!m || !row_ptr || (m->nnzb && !columns)) return fail(BMFP_ERR_INVALID, "NULL argument");
-  BMFP_HIP(hipMemcpy(row_ptr, m->row_ptr->ptr, sizeof(int) * ((size_t) m->n_rows + 1), hipMemcpyDeviceToHost));
-  if (m->nnzb) BMFP_HIP(hipMemcpy(columns, m->columns->ptr, sizeof(int) * (size_t) m->nnzb, hipMemcpyDeviceToHost));
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_download_structure", [&]() -> int {
+    if (!m || !row_ptr || (m->nnzb && !columns)) return fail(BMFP_ERR_INVALID, "NULL argument");
+    BMFP_HIP(hipMemcpy(row_ptr, m->row_ptr->ptr, sizeof(int) * ((size_t) m->n_rows + 1), hipMemcpyDeviceToHost));
+    if (m->nnzb) BMFP_HIP(hipMemcpy(columns, m->columns->ptr, sizeof(int) * (size_t) m->nnzb, hipMemcpyDeviceToHost));
+    return BMFP_OK;
+  });
 }
 
 extern "C" int bmfp_matrix_download_values(bmfp_matrix m, double *values)
 {
-  if (!m || !values) return fail(BMFP_ERR_INVALID, "NULL argument");
-  const size_t nv = (size_t) m->n_vals();
-  if (!nv) return BMFP_OK;
-  if (m->dtype == BMFP_F64) {
-    BMFP_HIP(hipMemcpy(values, m->values.ptr, nv * sizeof(double), hipMemcpyDeviceToHost));
-  } else if (m->dtype == BMFP_F32) {
-    std::vector<float> f(nv);
-    BMFP_HIP(hipMemcpy(f.data(), m->values.ptr, nv * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < nv; i++) values[i] = f[i];
-  } else {
-    std::vector<__half> h(nv);
-    std::vector<float> sc((size_t) m->n_rows);
-    std::vector<int> rp((size_t) m->n_rows + 1);
-    BMFP_HIP(hipMemcpy(h.data(), m->values.ptr, nv * sizeof(__half), hipMemcpyDeviceToHost));
-    BMFP_HIP(hipMemcpy(sc.data(), m->scale.ptr, sc.size() * sizeof(float), hipMemcpyDeviceToHost));
-    BMFP_HIP(hipMemcpy(rp.data(), m->row_ptr->ptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
-    const size_t bs2 = (size_t) m->bs * m->bs;
-    for (int br = 0; br < m->n_rows; br++)
-      for (size_t e = (size_t) rp[br] * bs2; e < (size_t) rp[br + 1] * bs2; e++)
-        values[e] = (double) __half2float(h[e]) * (double) sc[br];
-  }
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_matrix_download_values", [&]() -> int {
+    if (!m || !values) return fail(BMFP_ERR_INVALID, "NULL argument");
+    const size_t nv = (size_t) m->n_vals();
+    if (!nv) return BMFP_OK;
+    if (m->dtype == BMFP_F64) {
+      BMFP_HIP(hipMemcpy(values, m->values.ptr, nv * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (m->dtype == BMFP_F32) {
+      std::vector<float> f(nv);
+      BMFP_HIP(hipMemcpy(f.data(), m->values.ptr, nv * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nv; i++) values[i] = f[i];
+    } else {
+      std::vector<__half> h(nv);
+      std::vector<float> sc((size_t) m->n_rows);
+      std::vector<int> rp((size_t) m->n_rows + 1);
+      BMFP_HIP(hipMemcpy(h.data(), m->values.ptr, nv * sizeof(__half), hipMemcpyDeviceToHost));
+      BMFP_HIP(hipMemcpy(sc.data(), m->scale.ptr, sc.size() * sizeof(float), hipMemcpyDeviceToHost));
+      BMFP_HIP(hipMemcpy(rp.data(), m->row_ptr->ptr, rp.size() * sizeof(int), hipMemcpyDeviceToHost));
+      const size_t bs2 = (size_t) m->bs * m->bs;
+      for (int br = 0; br < m->n_rows; br++)
+        for (size_t e = (size_t) rp[br] * bs2; e < (size_t) rp[br + 1] * bs2; e++)
+          values[e] = (double) __half2float(h[e]) * (double) sc[br];
+    }
+    return BMFP_OK;
+  });
 }
 
 // ---- SpMV ---------------------------------------------------------------------------
@@ -720,38 +784,44 @@ static int spmv_dispatch(bmfp_matrix A, int variant, const void *x, void *y, hip
 
 extern "C" int bmfp_spmv(bmfp_matrix A, int variant, const void *x, void *y, bmfp_stream stream)
 {
-  if (!A || (A->n_rows && (!x || !y))) return fail(BMFP_ERR_INVALID, "NULL argument");
-  return spmv_dispatch(A, variant, x, y, static_cast<hipStream_t>(stream));
+  return bmfp::guarded("bmfp_spmv", [&]() -> int {
+    if (!A || (A->n_rows && (!x || !y))) return fail(BMFP_ERR_INVALID, "NULL argument");
+    return spmv_dispatch(A, variant, x, y, static_cast<hipStream_t>(stream));
+  });
 }
 
 extern "C" int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y, int warmup, int reps,
                                bmfp_stream stream, double *avg_seconds)
 {
-  if (!A || !avg_seconds || reps <= 0 || warmup < 0) return fail(BMFP_ERR_INVALID, "bad arguments");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  for (int i = 0; i < warmup; i++) BMFP_TRY(spmv_dispatch(A, variant, x, y, s));
-  hipEvent_t e0, e1;
-  BMFP_HIP(hipEventCreate(&e0));
-  BMFP_HIP(hipEventCreate(&e1));
-  int st = BMFP_OK;
-  if (hipEventRecord(e0, s) != hipSuccess) st = fail(BMFP_ERR_HIP, "hipEventRecord failed");
-  for (int i = 0; i < reps && st == BMFP_OK; i++) st = spmv_dispatch(A, variant, x, y, s);
-  float ms = 0.f;
-  if (st == BMFP_OK) {
-    hipError_t e = hipEventRecord(e1, s);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) st = hip_fail(e, "event timing", __FILE__, __LINE__);
-  }
-  (void) hipEventDestroy(e0);
-  (void) hipEventDestroy(e1);
-  if (st == BMFP_OK) *avg_seconds = ms * 1e-3 / reps;
-  return st;
+  return bmfp::guarded("bmfp_spmv_timed", [&]() -> int {
+    if (!A || !avg_seconds || reps <= 0 || warmup < 0) return fail(BMFP_ERR_INVALID, "bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < warmup; i++) BMFP_TRY(spmv_dispatch(A, variant, x, y, s));
+    hipEvent_t e0, e1;
+    BMFP_HIP(hipEventCreate(&e0));
+    BMFP_HIP(hipEventCreate(&e1));
+    int st = BMFP_OK;
+    if (hipEventRecord(e0, s) != hipSuccess) st = fail(BMFP_ERR_HIP, "hipEventRecord failed");
+    for (int i = 0; i < reps && st == BMFP_OK; i++) st = spmv_dispatch(A, variant, x, y, s);
+    float ms = 0.f;
+    if (st == BMFP_OK) {
+      hipError_t e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) st = hip_fail(e, "event timing", __FILE__, __LINE__);
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    if (st == BMFP_OK) *avg_seconds = ms * 1e-3 / reps;
+    return st;
+  });
 }
 
 extern "C" int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant)
 {
-  if (!A || !variant) return fail(BMFP_ERR_INVALID, "NULL argument");
-  *variant = auto_variant(*A);
-  return BMFP_OK;
+  return bmfp::guarded("bmfp_spmv_auto_variant", [&]() -> int {
+    if (!A || !variant) return fail(BMFP_ERR_INVALID, "NULL argument");
+    *variant = auto_variant(*A);
+    return BMFP_OK;
+  });
 }
