@@ -161,3 +161,50 @@ def test_block_size_detection_skips_non_dividing_sizes():
     assert bm.detect_block_size(c.row_ptr, c.columns, 75, 75)[0] == 3
     with pytest.raises(bm.BmfpError):
         bm.detect_block_size(c.row_ptr, c.columns, 75, 75, candidates=[2, 4])
+
+
+def test_reader_survives_mutated_files(tmp_path):
+    # seeded mutations of valid files (deleted, duplicated and truncated lines,
+    # garbage tokens): every read either fails with a status or returns a
+    # well-formed CSR (row pointers from 0 to nnz, nondecreasing, columns in
+    # range and strictly ascending within a row)
+    rng = np.random.default_rng(2024)
+    junk = ["-1", "0", "2147483647", "99999999999", "1e309", "nan", "x", " ", "%", "3 3", "1 1 1 1", "\n"]
+    path = tmp_path / "f.mtx"
+    parsed = 0
+    for case in range(600):
+        field = ["real", "integer", "pattern"][case % 3]
+        sym = ["general", "symmetric", "skew-symmetric"][(case // 3) % 3]
+        if sym == "skew-symmetric" and field == "pattern":
+            sym = "general"
+        n = int(rng.integers(1, 9))
+        lines = [f"%%MatrixMarket matrix coordinate {field} {sym}", f"{n} {n} 6"]
+        for _ in range(6):
+            i, j = sorted(rng.integers(1, n + 1, 2).tolist(), reverse=sym != "general")
+            lines.append(f"{i} {j}" + ("" if field == "pattern" else f" {rng.uniform(-5, 5):.6g}"))
+        text = "\n".join(lines) + "\n"
+        for _ in range(int(rng.integers(0, 4))):
+            p = int(rng.integers(0, len(text)))
+            op = int(rng.integers(0, 4))
+            if op == 0:
+                text = text[:p] + text[p + int(rng.integers(1, 6)):]
+            elif op == 1:
+                text = text[:p] + junk[int(rng.integers(0, len(junk)))] + text[p:]
+            elif op == 2:
+                text = text[:p]
+            else:
+                q = text.find("\n", p)
+                text = text if q < 0 else text[:q + 1] + text[p:q + 1] + text[q + 1:]
+        path.write_text(text)
+        try:
+            rp, cols, vals, nr, nc = bm.read_mm(path)
+        except bm.BmfpError as e:
+            assert e.status in (bm.ERR_INVALID, bm.ERR_UNSUPPORTED)
+            continue
+        parsed += 1
+        assert rp[0] == 0 and rp[-1] == len(cols) == len(vals) and len(rp) == nr + 1
+        assert np.all(np.diff(rp) >= 0)
+        assert np.all((cols >= 0) & (cols < nc))
+        for r in range(nr):
+            assert np.all(np.diff(cols[rp[r]:rp[r + 1]]) > 0)
+    assert parsed > 100
