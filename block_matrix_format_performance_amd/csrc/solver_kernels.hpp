@@ -343,15 +343,25 @@ struct ShflEngine {
     const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
     const int64_t n_tiles = ((int64_t) M.n_brows + RPW - 1) / RPW;
     const int64_t waves = (int64_t) gridDim.x * 4;
-    for (int64_t wt = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6; wt < n_tiles; wt += waves) {
+    // a tile's row pointers are loaded while the wavefront's previous tile
+    // computes: one dependent global latency less per tile
+    auto row_bounds = [&](int64_t t, int &lo, int &hi) {
+      const int64_t row = t * RPW + lbr;
+      lo = hi = 0;
+      if (t < n_tiles && lane < RPW * BS && row < M.n_brows) {
+        lo = M.rp[row];
+        hi = M.rp[row + 1];
+      }
+    };
+    int64_t wt = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
+    int next_b, next_b1;
+    row_bounds(wt, next_b, next_b1);
+    for (; wt < n_tiles; wt += waves) {
       const int64_t br64 = wt * RPW + lbr;
       const bool active = lane < RPW * BS && br64 < M.n_brows;
       const int br = (int) br64;
-      int b = 0, b1 = 0;
-      if (active) {
-        b = M.rp[br];
-        b1 = M.rp[br + 1];
-      }
+      int b = next_b, b1 = next_b1;
+      row_bounds(wt + waves, next_b, next_b1);
       // all lanes of a block row share b..b1, so every shuffle source below
       // is active in the same iteration
       AT acc = 0;
