@@ -225,20 +225,26 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
   const int lane = threadIdx.x & 63;
   const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
   const gptr<const int> rp = glb(D.rp), cols = glb(D.cols);
+  // a pass's row pointers are loaded while the previous pass computes
+  auto row_bounds = [&](int ps, int &lo, int &hi) {
+    const int brq = br0 + ps * RPW + lbr;
+    lo = hi = 0;
+    if (use_matrix && ps < SERVE_CHUNK_PASSES && lane < RPW * BS && brq < D.n_brows) {
+      lo = ld(rp, (uint32_t) brq);
+      hi = ld(rp, (uint32_t) brq + 1u);
+    }
+  };
+  int next_b, next_b1;
+  row_bounds(0, next_b, next_b1);
 #pragma unroll 1
   for (int ps = 0; ps < SERVE_CHUNK_PASSES; ++ps) {
     const int br = br0 + ps * RPW + lbr;
     const bool on = lane < RPW * BS && br < D.n_brows;
     const uint32_t off = (uint32_t) (br * BS + r) * NR;
-    int b = 0, b1 = 0;
+    int b = next_b, b1 = next_b1;
+    row_bounds(ps + 1, next_b, next_b1);
     OWNT mine;
-    if (on) {
-      if (use_matrix) {
-        b = ld(rp, (uint32_t) br);
-        b1 = ld(rp, (uint32_t) br + 1u);
-      }
-      own(off, mine);
-    }
+    if (on) own(off, mine);
     AT acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[k] = 0;
