@@ -609,10 +609,7 @@ extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int
       BMFP_TRY(staging.alloc(nv * sizeof(double)));
       gen = staging.as<double>();
     }
-    const int64_t rows = nodes * bs;
-    k_gen_stencil_rows<<<(unsigned) ((rows + 255) / 256), 256, 0, nullptr>>>(nx, ny, bs, kind, seed,
-                                                                            m->row_ptr->as<int>(),
-                                                                            m->columns->as<int>(), gen, nullptr);
+    launch_gen_stencil(nx, ny, bs, kind, seed, m->row_ptr->as<int>(), m->columns->as<int>(), gen, nullptr, nullptr);
     BMFP_LAUNCHED();
     if (!direct) BMFP_TRY(convert_values(*m, gen, BMFP_ROW_MAJOR, nullptr));
     BMFP_HIP(hipDeviceSynchronize());

@@ -90,5 +90,117 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows(int nx, int ny, int bs
   if (vals32) vals32[dg] = (float) dv;
 }
 
+// N values at p, whose byte address is a multiple of the lowest set bit of
+// N*sizeof(T), in stores of up to 16 bytes.
+template <class T, int N> __device__ __forceinline__ void store_row(T *p, const T (&v)[N])
+{
+  constexpr int BYTES = N * (int) sizeof(T);
+  constexpr int LOW = BYTES & -BYTES;
+  constexpr int PER = (LOW < 16 ? LOW : 16) / (int) sizeof(T);
+  if constexpr (PER <= 1) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) p[c] = v[c];
+  } else {
+    typedef T vec_t __attribute__((ext_vector_type(PER)));
+#pragma unroll
+    for (int c = 0; c < N; c += PER) {
+      vec_t t;
+#pragma unroll
+      for (int e = 0; e < PER; ++e) t[e] = v[c + e];
+      *reinterpret_cast<vec_t *>(p + c) = t;
+    }
+  }
+}
+
+// k_gen_stencil_rows for a compile-time block size: the same draws, the same
+// arithmetic and the same order of the diagonal row sum, with each block row
+// held in registers and written by up-to-16-byte stores (the generic kernel
+// stores one value at a time).  The diagonal block's row is written once, at
+// the end, with its diagonal entry in place.
+template <int BS>
+__global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int kind, uint64_t seed,
+                                                            const int *__restrict__ rp, int *__restrict__ cols,
+                                                            double *__restrict__ vals, float *__restrict__ vals32)
+{
+#pragma clang fp contract(off)
+  const int64_t t64 = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (t64 >= (int64_t) nx * ny * BS) return;
+  const int t = (int) t64;
+  const int i = t / BS;
+  const int r = t - i * BS;
+  const int iy = i / nx, ix = i - iy * nx;
+  const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
+  constexpr int64_t bs2 = (int64_t) BS * BS;
+  int64_t nb = rp[i];
+  int64_t diag_at = 0;
+  double dg[BS];
+  double sum = 0.0;
+  for (int s = 0; s < 9; s++) {
+    const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
+    if (jy < 0 || jy >= ny || jx < 0 || jx >= nx) continue;
+    if (r == 0) cols[nb] = jy * nx + jx;
+    const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) bs2;
+    const int64_t at = nb * bs2 + (int64_t) r * BS;
+    double v[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      if (kind == 0) {
+        v[c] = rng_sym(ka, base + (uint64_t) r * BS + c);
+      } else if (s == 4) {
+        v[c] = 0.0;
+      } else {
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        const double k = rng_unit(ka, base + (uint64_t) lo * BS + hi);
+        double cc = 0.0;
+        if (r < c) cc = rng_sym(kc, base + (uint64_t) r * BS + c);
+        else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * BS + r);
+        v[c] = -(k + 0.3 * cc);
+      }
+      if (!(s == 4 && c == r)) sum += fabs(v[c]);
+    }
+    if (s == 4) {
+      diag_at = at;
+#pragma unroll
+      for (int c = 0; c < BS; c++) dg[c] = v[c];
+    } else {
+      store_row<double, BS>(vals + at, v);
+      if (vals32) {
+        float f[BS];
+#pragma unroll
+        for (int c = 0; c < BS; c++) f[c] = (float) v[c];
+        store_row<float, BS>(vals32 + at, f);
+      }
+    }
+    nb++;
+  }
+#pragma unroll
+  for (int c = 0; c < BS; c++)
+    if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
+  store_row<double, BS>(vals + diag_at, dg);
+  if (vals32) {
+    float f[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) f[c] = (float) dg[c];
+    store_row<float, BS>(vals32 + diag_at, f);
+  }
+}
+
+// The generator for any block size: the register-row kernel for the block
+// sizes the request loop serves, the generic one otherwise.
+inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
+                               double *vals, float *vals32, hipStream_t s)
+{
+  const unsigned g = (unsigned) (((int64_t) nx * ny * bs + 255) / 256);
+  switch (bs) {
+  case 1: k_gen_stencil_rows_t<1><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 2: k_gen_stencil_rows_t<2><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 3: k_gen_stencil_rows_t<3><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 4: k_gen_stencil_rows_t<4><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 6: k_gen_stencil_rows_t<6><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 8: k_gen_stencil_rows_t<8><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  default: k_gen_stencil_rows<<<g, 256, 0, s>>>(nx, ny, bs, kind, seed, rp, cols, vals, vals32); break;
+  }
+}
+
 }  // namespace
 }  // namespace bmfp

@@ -114,15 +114,17 @@ def test_jacobi_preconditioned_requests():
     assert np.array_equal(xa, oj[ij[2]][1])
 
 
-def test_device_generator_matches_host_generator():
-    # explicit arrays from the host generator vs the on-device generator:
-    # identical matrices and right-hand sides give bitwise identical solutions
-    nx, ny, bs = 14, 11, 4
-    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, 1, seed=5)
+@pytest.mark.parametrize("bs,kind", [(4, 1), (1, 0), (2, 1), (3, 0), (6, 1), (8, 0)])
+def test_device_generator_matches_host_generator(bs, kind):
+    # explicit arrays from the host generator vs the on-device generator (FP64
+    # and FP32 values written at admission): identical matrices and
+    # right-hand sides give bitwise identical solutions
+    nx, ny = 14, 11
+    rp, cols, vals = bm.gen_stencil9(nx, ny, bs, kind, seed=5)
     n = nx * ny * bs
     b = oracle_rhs(3, 1, n)
     S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
-    i1 = S.submit(S.stencil_request(nx, ny, bs, kind=1, seed=5, nrhs=1, rhs_seed=3))
+    i1 = S.submit(S.stencil_request(nx, ny, bs, kind=kind, seed=5, nrhs=1, rhs_seed=3))
     i2 = S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=1, b=b))
     out = S.drain()
     assert np.array_equal(out[i1][1], out[i2][1])

@@ -393,7 +393,11 @@ def test_fill_matches_host_rng():
 
 @pytest.mark.parametrize("bs,kind,layout,dtype", [(1, 0, bm.ROW_MAJOR, bm.F64), (3, 1, bm.ROW_MAJOR, bm.F64),
                                                   (4, 0, bm.COL_MAJOR, bm.F64), (6, 1, bm.ROW_MAJOR, bm.F32),
-                                                  (8, 0, bm.ROW_MAJOR, bm.F16)])
+                                                  (8, 0, bm.ROW_MAJOR, bm.F16), (1, 1, bm.ROW_MAJOR, bm.F64),
+                                                  (2, 0, bm.ROW_MAJOR, bm.F64), (2, 1, bm.COL_MAJOR, bm.F64),
+                                                  (3, 0, bm.ROW_MAJOR, bm.F64), (4, 1, bm.ROW_MAJOR, bm.F64),
+                                                  (6, 0, bm.ROW_MAJOR, bm.F64), (8, 1, bm.ROW_MAJOR, bm.F64),
+                                                  (5, 0, bm.ROW_MAJOR, bm.F64), (12, 1, bm.ROW_MAJOR, bm.F64)])
 def test_device_stencil_matches_host_generator(bs, kind, layout, dtype):
     # bmfp_stencil_create generates on the device; the matrix must equal the
     # one uploaded from the oracle's generator, bit for bit
