@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "spmv_kernels.hpp"
 
 namespace bmfp {
 
@@ -185,19 +186,131 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
   }
 }
 
+// Stores the wavefront's blocks staged in LDS (NPW blocks of BS*BS values,
+// node q's at q*BS*BS): lanes take 16-byte chunks in memory order, block q
+// going to block index nb of the lane holding node q's row 0 (skipped where
+// that node has no such block); the FP32 copy in 8-byte chunks.
+template <int BS>
+__device__ __forceinline__ void store_staged_blocks(const double *blk, int nb, bool present, double *vals,
+                                                    float *vals32, int lane)
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  constexpr int NPW = 64 / BS, BS2 = BS * BS, CPB = BS2 / 2, NCH = (NPW * CPB + 63) / 64;
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    const int ch = lane + 64 * k;
+    const int qq = ch / CPB < NPW ? ch / CPB : NPW - 1;
+    const int nbq = __shfl(nb, qq * BS);
+    const int pq = __shfl((int) present, qq * BS);
+    if (ch < NPW * CPB && pq) {
+      const int off = (ch - qq * CPB) * 2;
+      const d2 v = *reinterpret_cast<const d2 *>(blk + qq * BS2 + off);
+      const int64_t at = (int64_t) nbq * BS2 + off;
+      *reinterpret_cast<d2 *>(vals + at) = v;
+      if (vals32) *reinterpret_cast<f2 *>(vals32 + at) = f2{(float) v.x, (float) v.y};
+    }
+  }
+}
+
+// k_gen_stencil_rows_t with the stores coalesced, for bs 4, 6 and 8.  There a
+// lane's 16-byte stores of its block row still leave every store instruction
+// spread over one cache line per row.  A wavefront holds NPW = 64/BS
+// consecutive nodes; for each neighbour s the lanes put their block rows in
+// the wavefront's LDS slice, then store the NPW blocks (each contiguous in
+// memory) chunk by chunk in memory order.  The diagonal block goes last, its
+// diagonal entries in place.  Same draws, arithmetic and row-sum order as
+// k_gen_stencil_rows_t; no workgroup barrier (each wavefront uses its own
+// slice).
+template <int BS>
+__global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int kind, uint64_t seed,
+                                                            const int *__restrict__ rp, int *__restrict__ cols,
+                                                            double *__restrict__ vals, float *__restrict__ vals32)
+{
+#pragma clang fp contract(off)
+  constexpr int NPW = 64 / BS, BS2 = BS * BS;
+  static_assert(BS2 % 2 == 0, "blocks of whole 16-byte chunks");
+  __shared__ __align__(16) double blk_s[4][NPW * BS2];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int64_t n_nodes = (int64_t) nx * ny;
+  const int64_t i0 = ((int64_t) blockIdx.x * 4 + wib) * NPW;
+  if (i0 >= n_nodes) return;  // the whole wavefront leaves; no barrier follows
+  const int q = lane / BS, r = lane - q * BS;
+  const bool on = q < NPW && i0 + q < n_nodes;
+  const int i = (int) (on ? i0 + q : i0);
+  const int iy = i / nx, ix = i - iy * nx;
+  const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
+  double *blk = blk_s[wib];
+  int nb = rp[i], diag_nb = 0;
+  double dg[BS];
+  double sum = 0.0;
+  for (int s = 0; s < 9; s++) {
+    const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
+    const bool present = on && jy >= 0 && jy < ny && jx >= 0 && jx < nx;
+    double v[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) v[c] = 0.0;
+    if (present) {
+      if (r == 0) cols[nb] = jy * nx + jx;
+      const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) BS2;
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        if (kind == 0) {
+          v[c] = rng_sym(ka, base + (uint64_t) r * BS + c);
+        } else if (s == 4) {
+          v[c] = 0.0;
+        } else {
+          const int lo = r < c ? r : c, hi = r < c ? c : r;
+          const double k = rng_unit(ka, base + (uint64_t) lo * BS + hi);
+          double cc = 0.0;
+          if (r < c) cc = rng_sym(kc, base + (uint64_t) r * BS + c);
+          else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * BS + r);
+          v[c] = -(k + 0.3 * cc);
+        }
+        if (!(s == 4 && c == r)) sum += fabs(v[c]);
+      }
+    }
+    if (s == 4) {
+      diag_nb = nb;
+#pragma unroll
+      for (int c = 0; c < BS; c++) dg[c] = v[c];
+    } else {
+      if (present) {
+#pragma unroll
+        for (int c = 0; c < BS; c++) blk[q * BS2 + r * BS + c] = v[c];
+      }
+      wave_lds_sync();
+      store_staged_blocks<BS>(blk, nb, present, vals, vals32, lane);
+      wave_lds_sync();  // the stores' LDS reads come before the next block's writes
+    }
+    if (present) nb++;
+  }
+  if (on) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
+      blk[q * BS2 + r * BS + c] = dg[c];
+    }
+  }
+  wave_lds_sync();
+  store_staged_blocks<BS>(blk, diag_nb, on, vals, vals32, lane);
+}
+
 // The generator for any block size: the register-row kernel for the block
-// sizes the request loop serves, the generic one otherwise.
+// sizes the request loop serves (coalesced through LDS for 4, 6 and 8), the
+// generic one otherwise.
 inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
                                double *vals, float *vals32, hipStream_t s)
 {
   const unsigned g = (unsigned) (((int64_t) nx * ny * bs + 255) / 256);
+  auto waves4 = [&](int npw) { return (unsigned) (((int64_t) nx * ny + 4 * npw - 1) / (4 * npw)); };
   switch (bs) {
   case 1: k_gen_stencil_rows_t<1><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
   case 2: k_gen_stencil_rows_t<2><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
   case 3: k_gen_stencil_rows_t<3><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 4: k_gen_stencil_rows_t<4><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 6: k_gen_stencil_rows_t<6><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 8: k_gen_stencil_rows_t<8><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 4: k_gen_stencil_rows_c<4><<<waves4(16), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 6: k_gen_stencil_rows_c<6><<<waves4(10), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 8: k_gen_stencil_rows_c<8><<<waves4(8), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
   default: k_gen_stencil_rows<<<g, 256, 0, s>>>(nx, ny, bs, kind, seed, rp, cols, vals, vals32); break;
   }
 }
