@@ -790,7 +790,8 @@ constexpr int CSR_STAGE_CAP = 640;
 template <class VT, class AT, int G, int WPG>
 __global__ __launch_bounds__(64 * WPG) void k_csr_stage(CsrArgs<VT, AT> a)
 {
-  constexpr int RPW = 64 / G, CAP = CSR_STAGE_CAP, U = G == 1 ? 9 : 4;  // G = 1: a 9-point row in one batch
+  // wavefronts hold ~576 nonzeros, ~9 per lane: a lane's entries in one batch
+  constexpr int RPW = 64 / G, CAP = CSR_STAGE_CAP, U = 9;
   constexpr int VCAP = CAP * (int) sizeof(VT), CCAP = CAP * 4;
   __shared__ __align__(16) char vals_s[WPG][staged_bytes(VCAP)];
   __shared__ __align__(16) char cols_s[WPG][staged_bytes(CCAP)];
