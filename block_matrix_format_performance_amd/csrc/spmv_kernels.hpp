@@ -269,13 +269,29 @@ template <class VT, int BS> constexpr DenseShape dense_shape()
   if constexpr (sizeof(VT) != 8) return {0, 0};
   switch (BS) {
   case 6: return {2, 3};
-  case 8: return {2, 3};
+  case 8: return {2, 9};
   case 12: return {4, 1};
   case 16: return {4, 2};
   case 24: return {4, 1};
   case 32: return {4, 2};
   default: return {0, 0};
   }
+}
+
+// Sum over the G consecutive lanes of group lr (lane = lr*G + g), left in
+// lane g = 0: lane g += lane g + s for s = 1, 2, 4, ... (all lanes take part;
+// the spare lanes past the last whole group read only each other).
+template <int G, class AT>
+__device__ __forceinline__ AT group_tree_sum(AT acc, int lane, int lr, int g)
+{
+  const int end = lr * G + G < 64 ? lr * G + G : 64;
+#pragma unroll
+  for (int s2 = 1; s2 < G; s2 *= 2) {
+    const int src = lane + s2 < end ? lane + s2 : lane;
+    const AT o = __shfl(acc, src);
+    if (g % (2 * s2) == 0 && g + s2 < G) acc += o;
+  }
+  return acc;
 }
 
 template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
@@ -323,15 +339,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 #pragma unroll
     for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
   }
-  // group sum: lane g += lane g + s for s = 1, 2, 4, ... (all lanes take part;
-  // the spare lanes past the last whole group read only each other)
-  const int end = lr * G + G < 64 ? lr * G + G : 64;
-#pragma unroll
-  for (int s2 = 1; s2 < G; s2 *= 2) {
-    const int src = lane + s2 < end ? lane + s2 : lane;
-    const AT o = __shfl(acc, src);
-    if (g % (2 * s2) == 0 && g + s2 < G) acc += o;
-  }
+  acc = group_tree_sum<G>(acc, lane, lr, g);
   if (on && g == 0) {
     if constexpr (SCALED) acc *= (AT) a.scale[br];
     a.y[row] = acc;
