@@ -259,7 +259,9 @@ __global__ __launch_bounds__(256) void k_bcsr_gpr(BcsrArgs<VT, AT> a)
 // lanes idle).  Lane g owns columns [g*VEC, g*VEC+VEC) of every block of its
 // row; loads of U blocks are issued together; partial sums are combined by a
 // fixed shuffle tree over the group.  Shapes per block size (FP64) from
-// timing on the HBM-resident stencils: dense_shape().
+// timing on the HBM-resident stencils, dense_shape(): VEC = 2 with U = 9 (a
+// whole 9-block stencil row in one batch) is the fastest everywhere but at
+// bs 6 (U = 3) and bs 24 (VEC = 4, U = 1).
 // ============================================================================
 struct DenseShape {
   int vec, u;
@@ -268,12 +270,13 @@ template <class VT, int BS> constexpr DenseShape dense_shape()
 {
   if constexpr (sizeof(VT) != 8) return {0, 0};
   switch (BS) {
+  case 4: return {2, 9};
   case 6: return {2, 3};
   case 8: return {2, 9};
-  case 12: return {4, 1};
-  case 16: return {4, 2};
+  case 12: return {2, 9};
+  case 16: return {2, 9};
   case 24: return {4, 1};
-  case 32: return {4, 2};
+  case 32: return {2, 9};
   default: return {0, 0};
   }
 }
