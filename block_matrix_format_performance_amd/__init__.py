@@ -117,6 +117,7 @@ def _declare(L):
     L.bmfp_spmv_timed.argtypes = [vp, c_int, vp, vp, c_int, c_int, vp, dp]
     L.bmfp_spmv_auto_variant.argtypes = [vp, ip]
     L.bmfp_jit_build.argtypes = [c_int, c_int, c_int, ctypes.POINTER(sz), dp]
+    L.bmfp_csr_spmv_noalloc.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp]
     _declare_solver(L)
 
 

@@ -52,4 +52,7 @@ int auto_variant(const bmfp_matrix_t &A);
 int csr_group_for(double mean_len);
 int csr_stage_group(double mean_len, double max_len, int lo);
 bool bcsr_supported(int variant, int dtype, int layout, int bs);
+// Device address of the allocation-free entry point's scratch on `device`
+// (the current device), resolved once per device.
+int noalloc_setup(int device, void **out);
 }  // namespace bmfp

@@ -181,7 +181,7 @@ extern "C" int bmfp_init(int device)
     BMFP_HIP(hipGetDeviceProperties(&p, device));
     if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
       return fail(BMFP_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, p.gcnArchName);
-    return BMFP_OK;
+    return noalloc_setup(device, nullptr);
   });
 }
 

@@ -367,6 +367,26 @@ int bmfp_server_stats(bmfp_server s, size_t *used_bytes, size_t *budget_bytes,
  * ctor memory estimate, bicgstab.cu:30-45). */
 int bmfp_bicgstab_workspace_bytes(bmfp_matrix A, int use_jacobi, size_t *bytes);
 
+/* ---- allocation-free CSR SpMV ----------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  y = A x by the merge-path kernels of BMFP_CSR_MERGE,
+ * bitwise equal to bmfp_spmv(A, BMFP_CSR_MERGE, ...) on a handle holding the
+ * same arrays, but over caller-owned device arrays (row_ptr[n_rows+1],
+ * columns[nnz], values[nnz], x[n_cols], y[n_rows]; values, x and y of dtype
+ * BMFP_F64 or BMFP_F32) and with the kernels' scratch (per-wavefront carries
+ * and tile boundaries) in one fixed device array of BMFP_NOALLOC_SCRATCH_BYTES
+ * that the library holds on each device it runs on until the process ends.
+ * Calls no allocating routine.  n_rows + nnz above BMFP_NOALLOC_MAX_ITEMS is
+ * refused with BMFP_ERR_INVALID.  Every call on a device shares that device's
+ * scratch: calls that overlap in time on one device (two streams, two host
+ * threads) race on it and must not be issued.  Stream-ordered. */
+#define BMFP_NOALLOC_MAX_ITEMS (1 << 28)
+#define BMFP_NOALLOC_SCRATCH_BYTES ((size_t) 8388864)
+int bmfp_csr_spmv_noalloc(int dtype, int n_rows, int n_cols, int nnz, const int *row_ptr,
+                          const int *columns, const void *values, const void *x, void *y,
+                          bmfp_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
