@@ -35,6 +35,15 @@
 //   --solve                    also run the FP64 BiCGStab of the 256x256
 //                              bs=2 stencil (b = A*1, x0 = 0, eps 1e-8) and
 //                              report its iterations, error and time
+//   --solve-csr                solve through the reference caller's exact
+//                              signature (main.cpp:372-373):
+//                              gpu_bicgstab<T, C>(csr, precond) and
+//                              solve(csr, b, eps, max_it), with T = --type and
+//                              C = --index int|unsigned, on the side x side
+//                              9-point stencil of --solve-bs (kind 0, seed 0;
+//                              default 256, 2: config 0), b = A*1, x0 = 0;
+//                              --solve-eps (default 1e-8), --precond 0|1,
+//                              --x-out FILE writes x as raw doubles
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -77,6 +86,11 @@ struct Options {
   std::string out = "result.json";
   std::string out_gbs;
   bool solve = false;
+  bool solve_csr = false;
+  std::string index = "int";
+  int solve_side = 256, solve_bs = 2, precond = 0;
+  double solve_eps = 1e-8;
+  std::string x_out;
 };
 
 // main.cpp:21-63 with x as an input (the reference overwrites x with 1):
@@ -307,6 +321,37 @@ int run_solve()
   return info.status == BMFP_OK ? 0 : 1;
 }
 
+// The reference caller's solver path: CSR constructor and CSR solve
+// (main.cpp:372-373, bicgstab.h:17-20) for one (T, C) instantiation.
+template <class T, class C>
+int run_solve_csr(const Options &o)
+{
+  auto A = gen_stencil9_bcsr<T, C>(o.solve_side, o.solve_side, o.solve_bs, 0, 0);
+  const csr_matrix_class<T, C> csr(*A);
+  const size_t n = (size_t) csr.n_rows;
+  std::vector<T> b(n);
+  for (size_t row = 0; row < n; row++) {
+    T dot = 0;  // b = A * 1 in CSR order
+    for (C e = csr.row_ptr[row]; e < csr.row_ptr[row + 1]; e++) dot += csr.values[e];
+    b[row] = dot;
+  }
+  gpu_bicgstab<T, C> solver(csr, o.precond != 0);
+  const T *x = solver.solve(csr, b.data(), (T) o.solve_eps, 100000);
+  const bmfp_solve_info &info = solver.last_info();
+  double err = 0;
+  for (size_t i = 0; i < n; i++) err = std::max(err, std::fabs((double) x[i] - 1.0));
+  std::cout << "BiCGStab CSR<" << o.type << ", " << o.index << "> " << o.solve_side << "x" << o.solve_side
+            << " bs=" << o.solve_bs << ": status " << info.status << ", iterations " << info.iterations
+            << ", ||r||/||b|| " << info.rel_residual << ", true " << info.true_rel_residual << ", max|x-1| " << err
+            << std::endl;
+  if (!o.x_out.empty()) {
+    std::vector<double> xd(x, x + n);
+    std::ofstream f(o.x_out, std::ios::binary);
+    f.write(reinterpret_cast<const char *>(xd.data()), (std::streamsize) (n * sizeof(double)));
+  }
+  return info.status == BMFP_OK ? 0 : 1;
+}
+
 std::vector<int> parse_list(const char *s)
 {
   std::vector<int> v;
@@ -349,6 +394,13 @@ int main(int argc, char **argv)
     else if (a == "--out") o.out = next();
     else if (a == "--out-gbs") o.out_gbs = next();
     else if (a == "--solve") o.solve = true;
+    else if (a == "--solve-csr") o.solve_csr = true;
+    else if (a == "--index") o.index = next();
+    else if (a == "--solve-side") o.solve_side = std::atoi(next());
+    else if (a == "--solve-bs") o.solve_bs = std::atoi(next());
+    else if (a == "--solve-eps") o.solve_eps = std::atof(next());
+    else if (a == "--precond") o.precond = std::atoi(next());
+    else if (a == "--x-out") o.x_out = next();
     else {
       std::cerr << "unknown option " << a << std::endl;
       return 2;
@@ -359,6 +411,16 @@ int main(int argc, char **argv)
     return 1;
   }
   if (o.bridge_solve) return run_bridge_solve(o);
+  if (o.solve_csr) {
+    try {
+      const bool dbl = o.type == "double", uns = o.index == "unsigned";
+      if (dbl) return uns ? run_solve_csr<double, unsigned>(o) : run_solve_csr<double, int>(o);
+      return uns ? run_solve_csr<float, unsigned>(o) : run_solve_csr<float, int>(o);
+    } catch (const std::exception &e) {
+      std::cerr << "--solve-csr: " << e.what() << std::endl;
+      return 1;
+    }
+  }
   std::map<int, Rows> all;
   if (o.bridge) {
     auto bridge = make_bridge(o);

@@ -40,6 +40,7 @@ def lib():
         L.orc_rng_unit.argtypes = [u64, u64, u64]
         L.orc_rng_sym.restype = c_dbl
         L.orc_rng_sym.argtypes = [u64, u64, u64]
+        L.orc_rng_fill.argtypes = [u64, u64, c_int, ctypes.c_int64, _f64p]
         L.orc_gen_n_diag_bcsr.argtypes = [c_int, c_int, c_int, _i32p, _i32p, _f64p]
         L.orc_stencil_nnzb.restype = ctypes.c_int64
         L.orc_stencil_nnzb.argtypes = [c_int, c_int]
@@ -90,13 +91,15 @@ class CSR:
 
 
 def rng_unit_vec(seed, stream, n):
-    L = lib()
-    return np.array([L.orc_rng_unit(seed, stream, i) for i in range(n)], dtype=np.float64)
+    out = np.empty(int(n), np.float64)
+    lib().orc_rng_fill(seed, stream, 0, int(n), out)
+    return out
 
 
 def rng_sym_vec(seed, stream, n):
-    L = lib()
-    return np.array([L.orc_rng_sym(seed, stream, i) for i in range(n)], dtype=np.float64)
+    out = np.empty(int(n), np.float64)
+    lib().orc_rng_fill(seed, stream, 1, int(n), out)
+    return out
 
 
 def gen_n_diag_bcsr(n_rows, blocks_per_row, bs):

@@ -55,6 +55,13 @@ double orc_rng_sym(uint64_t seed, uint64_t stream, uint64_t index)
   return 2.0 * orc_rng_unit(seed, stream, index) - 1.0;
 }
 
+void orc_rng_fill(uint64_t seed, uint64_t stream, int kind, int64_t n, double *out)
+{
+  /* element i = orc_rng_unit (kind 0) or orc_rng_sym (kind 1) of index i */
+  for (int64_t i = 0; i < n; i++)
+    out[i] = kind == 0 ? orc_rng_unit(seed, stream, (uint64_t) i) : orc_rng_sym(seed, stream, (uint64_t) i);
+}
+
 /* ------------------------------------------------------------------------ */
 /* gen_n_diag_bcsr: matrix_converters.h:157-196.                             */
 /* ------------------------------------------------------------------------ */

@@ -23,6 +23,8 @@ extern "C" {
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);
 double   orc_rng_unit(uint64_t seed, uint64_t stream, uint64_t index);  /* [0,1)  */
 double   orc_rng_sym(uint64_t seed, uint64_t stream, uint64_t index);   /* [-1,1) */
+/* out[i] = orc_rng_unit (kind 0) or orc_rng_sym (kind 1) of (seed, stream, i) */
+void     orc_rng_fill(uint64_t seed, uint64_t stream, int kind, int64_t n, double *out);
 
 /* ---- generators ---------------------------------------------------------- */
 /* gen_n_diag_bcsr, matrix_converters.h:157-196 (double values). */

@@ -4,7 +4,8 @@ bicgstab.cu:388-513.
 Bar: same stop reason; iteration counts within a small window (the device
 reduces dot products in a different, fixed order, so trajectories agree to
 rounding); solution within 1e-9 (relative 2-norm) of the oracle's solution at
-eps = 1e-10 and of the manufactured solution where the system is well
+eps = 1e-10 (measured: 2e-16 to 1e-15 on these systems, 1.2e-12 on the full
+2048x1024 one) and of the manufactured solution where the system is well
 conditioned; bitwise-identical results run to run.
 """
 import numpy as np
@@ -61,7 +62,7 @@ def test_convective_system_all_block_sizes(bs, layout):
     xo, oi = oracle.bicgstab(c, b, 1e-10, 5000)
     assert info.status == bm.OK and oi.status == 0
     assert abs(info.iterations - oi.iterations) <= max(3, oi.iterations // 20)
-    assert rel(x, xo) < 1e-8
+    assert rel(x, xo) < 1e-9
     assert info.true_rel_residual < 1e-9
 
 

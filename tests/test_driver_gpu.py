@@ -109,8 +109,52 @@ def test_driver_bridge_spmv_rows(tmp_path):
 
 
 def test_driver_solve():
+    # the 256x256 bs=2 system of the drop-in driver (b = A*1, eps 1e-8):
+    # the printed max|x-1| is parsed and bounded
     print("\n[gpu-case] driver --solve", flush=True)
     p = subprocess.run([bm.MAIN_PATH, "--bs", "2", "--n-rows", "500", "--reps", "1", "--out", os.devnull, "--solve"],
                        capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
-    assert "BiCGStab 256x256 bs=2" in p.stdout
+    line = next(l for l in p.stdout.splitlines() if l.startswith("BiCGStab 256x256 bs=2"))
+    err = float(line.split("max|x-1| ")[1].split(",")[0])
+    true_res = float(line.split(", true ")[1].split(",")[0])
+    print(line)
+    assert true_res < 2e-8
+    assert err < 2e-5   # the oracle's solve at eps 1e-8 has max|x-1| = 8.2e-6
+
+
+@pytest.mark.parametrize("typ,index", [("double", "int"), ("double", "unsigned"), ("float", "int"),
+                                       ("float", "unsigned")])
+def test_driver_solve_through_reference_csr_signature(tmp_path, typ, index):
+    # gpu_bicgstab<T, C>(const csr_matrix_class<T, C>&, bool) + solve(csr, ...)
+    # (bicgstab.h:17-20), as the reference's caller uses it (main.cpp:372-373),
+    # for every instantiation of bicgstab.cu:528-531; x against the oracle's
+    # BiCGStab on the same matrix (FP32: the matrix and b rounded to float)
+    side, bs = 64, 2
+    eps = 1e-10 if typ == "double" else 1e-5
+    print(f"\n[gpu-case] driver --solve-csr {typ} {index}", flush=True)
+    xfile = tmp_path / "x.bin"
+    p = subprocess.run([bm.MAIN_PATH, "--solve-csr", "--type", typ, "--index", index, "--solve-side", str(side),
+                        "--solve-bs", str(bs), "--solve-eps", str(eps), "--x-out", str(xfile)],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert f"BiCGStab CSR<{typ}, {index}>" in p.stdout
+    x = np.fromfile(xfile, dtype=np.float64)
+    m = oracle.gen_stencil9(side, side, bs, 0, 0)
+    c = oracle.bcsr_to_csr(m)
+    if typ == "float":
+        c = oracle.CSR(c.n_rows, c.n_cols, c.row_ptr, c.columns, c.values.astype(np.float32).astype(np.float64))
+    b = oracle.csr_spmv(c, np.ones(c.n_rows))
+    if typ == "float":
+        b = b.astype(np.float32).astype(np.float64)
+    xo, oi = oracle.bicgstab(c, b, eps, 100000)
+    assert oi.status == 0
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    res = np.linalg.norm(b - oracle.csr_spmv(c, x)) / np.linalg.norm(b)
+    print(p.stdout.strip().splitlines()[-1], f"rel(x, x_oracle) {err:.2e}, FP64 residual {res:.2e}")
+    if typ == "double":
+        assert err < 1e-9 and res < 2e-10
+    else:
+        # FP32 vectors: the recurrence residual reaches 1e-5, the true one
+        # stalls near the FP32 rounding level of A x
+        assert err < 1e-4 and res < 1e-4

@@ -55,6 +55,11 @@ def test_rng_known_answers():
     assert abs(u.mean() - 0.5) < 0.01
     s = oracle.lib().orc_rng_sym(3, 4, 7)
     assert s == 2.0 * oracle.lib().orc_rng_unit(3, 4, 7) - 1.0
+    # the vector fills are the scalar functions element by element
+    idx = [0, 1, 7, 19999]
+    assert [u[i] for i in idx] == [L.orc_rng_unit(3, 4, i) for i in idx]
+    v = oracle.rng_sym_vec(3, 4, 20000)
+    assert [v[i] for i in idx] == [L.orc_rng_sym(3, 4, i) for i in idx]
 
 
 # ---- generators and conversions vs the reference's compiled host code ---------

@@ -84,7 +84,10 @@ def test_bcsr_f64_stencil_all_variants(bs, layout):
     check_bcsr(m, x, layout, bm.SPMV_AUTO)
 
 
-@pytest.mark.parametrize("bs", [1, 2, 3, 4, 8, 24])
+# rows of 0-40 blocks: the dense lane-group shapes take full batches of U
+# blocks, a batch and a tail, several batches, and empty rows (group sums of
+# 6 and 16 lanes included)
+@pytest.mark.parametrize("bs", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32])
 @pytest.mark.parametrize("layout", [bm.ROW_MAJOR, bm.COL_MAJOR])
 def test_bcsr_f64_ragged_empty_rows(bs, layout):
     m, x = ragged_case(97, 83, bs, max_len=40, seed=100 + bs)
