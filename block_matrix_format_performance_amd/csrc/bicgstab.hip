@@ -2,6 +2,7 @@
 // vectors, device-side scalars, fused kernels replayed from a HIP graph.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "matrix.hpp"
@@ -47,19 +48,28 @@ SolverVecs<AT> vec_args(const bmfp_solver_t &S, const void *b)
   V.partA = parts;
   V.partB = parts + SOLVER_MAX_GROUPS;
   V.partC = parts + 3 * SOLVER_MAX_GROUPS;
+  V.partW = S.split ? parts + 5 * SOLVER_MAX_GROUPS : nullptr;
   V.st = S.state.as<SolverState>();
   V.n = n;
+  V.npa = V.npb = S.split ? REDUCE_GROUPS : S.np;
   return V;
 }
 
 template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
 struct Ops {
   using Engine = SolverEngine<VT, AT, BS, LAYOUT, SCALED>;
+  using Split = SplitEngine<VT, AT, BS, LAYOUT, SCALED>;
 
-  static int groups(const bmfp_matrix_t &A)
+  static int groups(const bmfp_matrix_t &A, bool split)
   {
-    const int64_t t = Engine::threads(A.n_rows);
+    const int64_t t = split ? Split::threads(A.n_rows) : Engine::threads(A.n_rows);
     return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
+  }
+  // workgroups of the split schedule's SpMV launches
+  static int spmv_groups(const bmfp_solver_t &S)
+  {
+    if constexpr (one_tile<Split>::value) return (int) std::max<int64_t>(1, Split::groups1(S.A->n_rows));
+    return S.np;
   }
 
   static int init(const bmfp_solver_t &S, const void *b, double eps, int max_iter, hipStream_t s)
@@ -73,9 +83,20 @@ struct Ops {
   {
     const MatArgs<VT, AT> M = mat_args<VT, AT, SCALED>(*S.A);
     const SolverVecs<AT> V = vec_args<AT>(S, nullptr);
-    k_bicg_a<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
-    k_bicg_b<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
-    k_bicg_c<AT><<<S.np, 256, 0, s>>>(V);
+    if (S.split) {
+      const int gw = spmv_groups(S);
+      k_bicg_sp<AT><<<S.np, 256, 0, s>>>(V);
+      k_bicg_sv<VT, AT, Split><<<gw, 256, 0, s>>>(M, V);
+      k_bicg_reduce<1><<<REDUCE_GROUPS, 256, 0, s>>>(V.partW, gw, V.partA, V.st);
+      k_bicg_ss<AT><<<S.np, 256, 0, s>>>(V);
+      k_bicg_st<VT, AT, Split><<<gw, 256, 0, s>>>(M, V);
+      k_bicg_reduce<2><<<REDUCE_GROUPS, 256, 0, s>>>(V.partW, gw, V.partB, V.st);
+      k_bicg_sx<AT><<<S.np, 256, 0, s>>>(V);
+    } else {
+      k_bicg_a<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
+      k_bicg_b<VT, AT, Engine><<<S.np, 256, 0, s>>>(M, V);
+      k_bicg_c<AT><<<S.np, 256, 0, s>>>(V);
+    }
     BMFP_LAUNCHED();
     return BMFP_OK;
   }
@@ -98,7 +119,8 @@ bool pick_bs(bmfp_solver_t &S)
     S.ops_init = Ops<VT, AT, SCALED, B, LAYOUT>::init;           \
     S.ops_iterate = Ops<VT, AT, SCALED, B, LAYOUT>::iterate;     \
     S.ops_jacobi = Ops<VT, AT, SCALED, B, LAYOUT>::jacobi;       \
-    S.np = Ops<VT, AT, SCALED, B, LAYOUT>::groups(*S.A);         \
+    S.np = Ops<VT, AT, SCALED, B, LAYOUT>::groups(*S.A, S.split); \
+    S.spmv_groups = Ops<VT, AT, SCALED, B, LAYOUT>::spmv_groups;  \
     return true;
   switch (S.A->bs) {
     BMFP_SOLVER_BS(1)
@@ -139,13 +161,31 @@ __global__ __launch_bounds__(256) void k_diff_norm2(const AT *__restrict__ b, co
   write_partials<2>(part, v, lds);
 }
 
+// Schedule: the split schedule moves the matrix at the standalone SpMV rate
+// but runs seven kernels per iteration against three, so it pays from about
+// 32 MB of stored values on (plain FP64 solves of the serving stream's
+// 9-point systems: up to 13% faster above, up to 27% slower below, where a
+// solve is bound by launches).  BMFP_SOLVER_FUSED=1 forces the fused
+// schedule (measurement switch).
+int split_schedule(const bmfp_matrix_t &A)
+{
+  static const int forced = [] {
+    const char *e = std::getenv("BMFP_SOLVER_FUSED");
+    return e ? std::atoi(e) : 0;
+  }();
+  constexpr double SPLIT_MIN_VALUE_BYTES = 32e6;
+  return !forced && (double) A.n_vals() * dtype_size(A.dtype) >= SPLIT_MIN_VALUE_BYTES;
+}
+
 }  // namespace
 
 size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi)
 {
   const int64_t n = A.y_len();
+  // partial sums: 5 * SOLVER_MAX_GROUPS, plus 2 per workgroup of the split
+  // schedule's SpMVs (at most one workgroup per 16 rows)
   return (size_t) n * vec_size(A.dtype) * (use_jacobi ? 10 : 9) + sizeof(double) * 5 * SOLVER_MAX_GROUPS +
-         sizeof(SolverState) + 256;
+         sizeof(double) * 2 * (size_t) ((n + 15) / 16 + SOLVER_MAX_GROUPS) + sizeof(SolverState) + 256;
 }
 
 int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
@@ -153,6 +193,7 @@ int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
   S.A = A;
   S.use_jacobi = use_jacobi ? 1 : 0;
   S.n = A->y_len();
+  S.split = split_schedule(*A);
   if (A->n_rows != A->n_cols) return fail(BMFP_ERR_INVALID, "BiCGStab needs a square matrix");
   if (!A->is_bcsr) return fail(BMFP_ERR_UNSUPPORTED, "BiCGStab runs on BCSR handles (CSR: use bs = 1)");
   bool ok = false;
@@ -164,7 +205,9 @@ int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
   if (!ok) return fail(BMFP_ERR_UNSUPPORTED, "no BiCGStab kernels for bs %d", A->bs);
   const size_t vb = vec_size(A->dtype);
   BMFP_TRY(S.vecs.alloc((size_t) std::max<int64_t>(S.n, 1) * vb * (S.use_jacobi ? 10 : 9)));
-  BMFP_TRY(S.parts.alloc(sizeof(double) * 5 * SOLVER_MAX_GROUPS));
+  // fused: partA, partB, partC; split adds the per-workgroup partials of its SpMVs
+  const size_t wparts = S.split ? (size_t) 2 * std::max(S.spmv_groups(S), (int) ((S.n + 15) / 16)) : 0;
+  BMFP_TRY(S.parts.alloc(sizeof(double) * (5 * SOLVER_MAX_GROUPS + wparts)));
   BMFP_TRY(S.state.alloc(sizeof(SolverState)));
   BMFP_HIP(hipHostMalloc(reinterpret_cast<void **>(&S.h_state), sizeof(SolverState), hipHostMallocDefault));
   BMFP_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
@@ -178,25 +221,38 @@ int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
   return BMFP_OK;
 }
 
-static int build_graph(bmfp_solver_t &S)
+// Captures graphs[k] = 2^k iterations (k = 0..3).
+static int build_graphs(bmfp_solver_t &S)
 {
-  if (S.graph) return BMFP_OK;
-  hipGraph_t g = nullptr;
-  BMFP_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
-  int st = BMFP_OK;
-  for (int i = 0; i < GRAPH_ITERS && st == BMFP_OK; i++) st = S.ops_iterate(S, S.stream);
-  hipError_t e = hipStreamEndCapture(S.stream, &g);
-  if (st != BMFP_OK) {
-    if (g) (void) hipGraphDestroy(g);
-    return st;
+  for (int k = 0; k < 4; k++) {
+    if (S.graphs[k]) continue;
+    hipGraph_t g = nullptr;
+    BMFP_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    int st = BMFP_OK;
+    for (int i = 0; i < (1 << k) && st == BMFP_OK; i++) st = S.ops_iterate(S, S.stream);
+    hipError_t e = hipStreamEndCapture(S.stream, &g);
+    if (st != BMFP_OK) {
+      if (g) (void) hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    e = hipGraphInstantiate(&S.graphs[k], g, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      S.graphs[k] = nullptr;
+      return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
   }
-  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
-  e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
-  (void) hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    S.graph = nullptr;
-    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
-  }
+  return BMFP_OK;
+}
+
+// Launches exactly `iters` iterations: whole 8-iteration graphs, then the
+// binary digits of the remainder.
+static int launch_iterations(bmfp_solver_t &S, int iters, hipStream_t s)
+{
+  for (; iters >= GRAPH_ITERS; iters -= GRAPH_ITERS) BMFP_HIP(hipGraphLaunch(S.graphs[3], s));
+  for (int k = 2; k >= 0; k--)
+    if (iters & (1 << k)) BMFP_HIP(hipGraphLaunch(S.graphs[k], s));
   return BMFP_OK;
 }
 
@@ -209,31 +265,35 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
     return BMFP_OK;
   }
   if (!b || !x) return fail(BMFP_ERR_INVALID, "NULL vector");
-  BMFP_TRY(build_graph(S));
+  BMFP_TRY(build_graphs(S));
   hipStream_t s = S.stream;
   // order after the caller's prior work on its stream
   BMFP_HIP(hipEventRecord(S.ev_sync, user));
   BMFP_HIP(hipStreamWaitEvent(s, S.ev_sync, 0));
   BMFP_HIP(hipEventRecord(S.ev0, s));
   BMFP_TRY(S.ops_init(S, b, eps, max_iterations, s));
-  // Launch graph chunks of GRAPH_ITERS iterations; read the state back
-  // between batches.  Kernels after the stop are no-ops but still cost a
-  // launch each, so the next batch covers the iterations the residual's
-  // decrease so far predicts are left (doubling while it has not decreased).
-  int batch = 1;
+  // Iterations run in batches between reads of the solver state.  Kernels
+  // after the stop are no-ops but each still costs a launch, so after the
+  // first batch (8) each batch is the number of iterations the residual's
+  // decrease so far predicts are left, plus the one whose first kernel sees
+  // the stop (doubling while the residual has not decreased), launched as
+  // graphs of 8, 4, 2 and 1 iterations.
+  int want = GRAPH_ITERS;
   for (;;) {
-    for (int k = 0; k < batch; k++) BMFP_HIP(hipGraphLaunch(S.graph, s));
+    BMFP_TRY(launch_iterations(S, want, s));
     BMFP_HIP(hipMemcpyAsync(S.h_state, S.state.ptr, sizeof(SolverState), hipMemcpyDeviceToHost, s));
     BMFP_HIP(hipStreamSynchronize(s));
     const SolverState &st = *S.h_state;
     if (st.done) break;
-    batch = std::min(batch * 2, 8);
+    int next = std::min(want * 2, 8 * GRAPH_ITERS);
     const double r0 = std::sqrt(st.norm_b2), rk = std::sqrt(st.rr), goal = st.eps * r0;
     if (st.iter > 0 && rk > 0 && rk < r0 && goal > 0) {
       const double per_iter = std::log(rk / r0) / st.iter;  // < 0
       const double left = std::log(goal / rk) / per_iter;
-      batch = (int) std::min(8.0, std::max(1.0, std::ceil(left / GRAPH_ITERS)));
+      next = (int) std::min<double>(8 * GRAPH_ITERS, std::max(0.0, std::ceil(left)) + 1);
     }
+    // never past the iteration whose first kernel reports max_iterations
+    want = std::max(1, std::min(next, st.max_iter - st.iter + 1));
   }
   const size_t vb = vec_size(S.A->dtype);
   BMFP_HIP(hipMemcpyAsync(x, S.vecs.ptr, (size_t) S.n * vb, hipMemcpyDeviceToDevice, s));
@@ -279,7 +339,8 @@ using namespace bmfp;
 bmfp_solver_t::~bmfp_solver_t()
 {
   if (refine) refine_destroy(refine);
-  if (graph) (void) hipGraphExecDestroy(graph);
+  for (hipGraphExec_t g : graphs)
+    if (g) (void) hipGraphExecDestroy(g);
   if (h_state) (void) hipHostFree(h_state);
   if (ev0) (void) hipEventDestroy(ev0);
   if (ev1) (void) hipEventDestroy(ev1);

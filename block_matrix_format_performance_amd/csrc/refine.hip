@@ -11,15 +11,75 @@
 
 namespace bmfp {
 
+namespace {
+
+// Refinement step 0 (x = 0): r = b, so no matrix read; r32 = (float) b and
+// partial sums of (b, b) in both slots, as the residual kernel writes them.
+__global__ __launch_bounds__(256) void k_refine_first(const double *__restrict__ b, int64_t n,
+                                                      float *__restrict__ r32, double *__restrict__ part)
+{
+  __shared__ double lds[2][4];
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) {
+    const double bi = b[i];
+    r32[i] = (float) bi;
+    acc[0] += bi * bi;
+  }
+  acc[1] = acc[0];
+  write_partials<2>(part, acc, lds);
+}
+
+// Outer decision of the refinement loop, taken on the device: reduces the
+// residual kernel's partial sums (fixed order), forms ||r_k|| / ||b|| and
+// decides converged / non-finite / escalate FP16 -> FP32 (the residual
+// shrank by less than 2x in a step, from step 2) / continue.  The host reads
+// this record and acts on `decision`.
+enum : int { REFINE_CONTINUE = 0, REFINE_CONVERGED = 1, REFINE_NONFINITE = 2, REFINE_ESCALATE = 3 };
+struct RefineCtl {
+  double norm_b;    // ||b||, from step 0
+  double rel;       // ||r_k|| / ||b||
+  double prev_rel;  // the previous step's rel
+  double eps;       // outer tolerance
+  int decision;
+  int pad[3];
+};
+
+__global__ __launch_bounds__(256) void k_refine_decide(const double *__restrict__ part, int np, int k, int fp16,
+                                                       RefineCtl *__restrict__ c)
+{
+  __shared__ double lds[2][4];
+  double red[2];
+  reduce_partials<2>(part, np, red, lds);
+  if (threadIdx.x == 0) {
+    if (k == 0) {
+      c->norm_b = sqrt(red[1]);
+      c->prev_rel = INFINITY;
+    }
+    const double rel = c->norm_b > 0 ? sqrt(red[0]) / c->norm_b : 0.0;
+    int d = REFINE_CONTINUE;
+    if (!isfinite(rel)) d = REFINE_NONFINITE;
+    else if (rel < c->eps) d = REFINE_CONVERGED;
+    else if (fp16 && k >= 2 && rel > 0.5 * c->prev_rel) d = REFINE_ESCALATE;
+    c->rel = rel;
+    c->prev_rel = rel;
+    c->decision = d;
+  }
+}
+
+}  // namespace
+
 struct RefineState {
   bmfp_matrix_t *A = nullptr;                // FP64 matrix (borrowed)
   int use_jacobi = 0;
   int dtype = BMFP_F32;                      // current inner storage type
+  int dtype0 = BMFP_F32;                     // requested inner storage type
   std::unique_ptr<bmfp_matrix_t> low[3];     // indexed by dtype (F32, F16 used)
   std::unique_ptr<bmfp_solver_t> inner[3];
   DeviceBuffer x;                            // two FP64 iterates (ping-pong)
   DeviceBuffer r32, d32;                     // FP32 residual / correction
   DeviceBuffer part;                         // residual partial sums
+  DeviceBuffer ctl;                          // RefineCtl: the outer decision
+  RefineCtl *h_ctl = nullptr;                // pinned mirror
   int np = 1;
   int64_t n = 0;
   hipStream_t stream = nullptr;
@@ -27,6 +87,8 @@ struct RefineState {
   int (*resid)(const RefineState &, const double *b, int xin, hipStream_t) = nullptr;
   ~RefineState()
   {
+    if (stream) (void) hipStreamSynchronize(stream);
+    if (h_ctl) (void) hipHostFree(h_ctl);
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (ev_sync) (void) hipEventDestroy(ev_sync);
@@ -111,7 +173,7 @@ int refine_create(bmfp_solver_t &H, bmfp_matrix_t *A, int inner_dtype, int use_j
   std::unique_ptr<RefineState> R(new RefineState);
   R->A = A;
   R->use_jacobi = use_jacobi ? 1 : 0;
-  R->dtype = inner_dtype;
+  R->dtype = R->dtype0 = inner_dtype;
   R->n = A->y_len();
   const bool ok = A->layout == BMFP_ROW_MAJOR ? pick<BMFP_ROW_MAJOR>(*R) : pick<BMFP_COL_MAJOR>(*R);
   if (!ok) return fail(BMFP_ERR_UNSUPPORTED, "no refinement kernels for bs %d", A->bs);
@@ -120,6 +182,8 @@ int refine_create(bmfp_solver_t &H, bmfp_matrix_t *A, int inner_dtype, int use_j
   BMFP_TRY(R->r32.alloc(n * sizeof(float)));
   BMFP_TRY(R->d32.alloc(n * sizeof(float)));
   BMFP_TRY(R->part.alloc(2 * SOLVER_MAX_GROUPS * sizeof(double)));
+  BMFP_TRY(R->ctl.alloc(sizeof(RefineCtl)));
+  BMFP_HIP(hipHostMalloc(reinterpret_cast<void **>(&R->h_ctl), sizeof(RefineCtl), hipHostMallocDefault));
   BMFP_HIP(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
   BMFP_HIP(hipEventCreate(&R->ev0));
   BMFP_HIP(hipEventCreate(&R->ev1));
@@ -137,7 +201,7 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
   if (!(eps > 0.0) || !(inner_tol > 0.0) || max_outer < 1 || max_inner < 1)
     return fail(BMFP_ERR_INVALID, "refinement: eps, inner_tol > 0 and max_outer, max_inner >= 1 required");
   bmfp_solve_info out{};
-  out.inner_dtype = R.dtype;
+  out.inner_dtype = R.dtype0;
   if (R.n == 0) {
     if (info) *info = out;
     return BMFP_OK;
@@ -147,43 +211,49 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
   BMFP_HIP(hipEventRecord(R.ev_sync, user));
   BMFP_HIP(hipStreamWaitEvent(s, R.ev_sync, 0));
   BMFP_HIP(hipEventRecord(R.ev0, s));
+  // each solve starts from the requested inner precision (an escalation
+  // lasts for the rest of its own solve only)
+  R.dtype = R.dtype0;
+  out.inner_dtype = R.dtype;
   BMFP_HIP(hipMemsetAsync(R.x.ptr, 0, (size_t) R.n * sizeof(double), s));
-  BMFP_HIP(hipMemsetAsync(R.d32.ptr, 0, (size_t) R.n * sizeof(float), s));
-  std::vector<double> h(2 * (size_t) R.np);
+  *R.h_ctl = RefineCtl{};
+  R.h_ctl->eps = eps;
+  BMFP_HIP(hipMemcpyAsync(R.ctl.ptr, R.h_ctl, sizeof(RefineCtl), hipMemcpyHostToDevice, s));
   int xin = 0;
-  double norm_b = 0.0, prev_rel = INFINITY;
   int status = BMFP_ERR_NOT_CONVERGED;
   for (int k = 0; k <= max_outer; k++) {
-    // x_{k} = x_{k-1} + d, r = b - A x_k (FP64); x_k lands in buffer 1 - xin
-    BMFP_TRY(R.resid(R, b, xin, s));
-    xin = 1 - xin;
-    BMFP_HIP(hipMemcpyAsync(h.data(), R.part.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    BMFP_HIP(hipStreamSynchronize(s));
-    double rr = 0, bb = 0;
-    for (int g = 0; g < R.np; g++) {
-      rr += h[2 * g];
-      bb += h[2 * g + 1];
+    if (k == 0) {
+      // x_0 = 0: r = b without reading the matrix
+      k_refine_first<<<R.np, 256, 0, s>>>(b, R.n, R.r32.as<float>(), R.part.as<double>());
+      BMFP_LAUNCHED();
+    } else {
+      // x_k = x_{k-1} + d, r = b - A x_k (FP64); x_k lands in buffer 1 - xin
+      BMFP_TRY(R.resid(R, b, xin, s));
+      xin = 1 - xin;
     }
-    if (k == 0) norm_b = std::sqrt(bb);
-    const double rel = norm_b > 0 ? std::sqrt(rr) / norm_b : 0.0;
-    out.rel_residual = out.true_rel_residual = rel;
+    k_refine_decide<<<1, 256, 0, s>>>(R.part.as<double>(), R.np, k, R.dtype == BMFP_F16 ? 1 : 0,
+                                      R.ctl.as<RefineCtl>());
+    BMFP_LAUNCHED();
+    BMFP_HIP(hipMemcpyAsync(R.h_ctl, R.ctl.ptr, sizeof(RefineCtl), hipMemcpyDeviceToHost, s));
+    BMFP_HIP(hipStreamSynchronize(s));
+    const RefineCtl &c = *R.h_ctl;
+    out.rel_residual = out.true_rel_residual = c.rel;
     out.outer_iterations = k;
-    if (!std::isfinite(rel)) {
+    if (c.decision == REFINE_NONFINITE) {
       status = BMFP_ERR_NONFINITE;
       break;
     }
-    if (rel < eps) {
+    if (c.decision == REFINE_CONVERGED) {
       status = BMFP_OK;
       break;
     }
     if (k == max_outer) break;
     // FP16 inner solves that stopped paying off escalate to FP32
-    if (R.dtype == BMFP_F16 && k >= 2 && rel > 0.5 * prev_rel) {
+    if (c.decision == REFINE_ESCALATE) {
       BMFP_TRY(ensure_level(R, BMFP_F32));
       R.dtype = BMFP_F32;
       out.escalations++;
     }
-    prev_rel = rel;
     bmfp_solve_info in{};
     int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false);
     out.inner_iterations += in.iterations;

@@ -14,6 +14,7 @@ struct bmfp_solver_t {
   bmfp_matrix_t *A = nullptr;   // borrowed
   int use_jacobi = 0;
   int np = 1;                   // persistent workgroups of every solver kernel
+  int split = 0;                // 1: five-kernel split schedule, 0: three fused kernels
   int64_t n = 0;                // scalar rows
   bmfp::DeviceBuffer vecs;      // x r r^ p0 p1 v0 v1 s t [pinv]
   bmfp::DeviceBuffer parts;     // partial sums
@@ -21,10 +22,11 @@ struct bmfp_solver_t {
   bmfp::SolverState *h_state = nullptr;  // pinned mirror of the device state
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_sync = nullptr;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graphs[4] = {};  // 1, 2, 4 and 8 iterations
   int (*ops_init)(const bmfp_solver_t &, const void *, double, int, hipStream_t) = nullptr;
   int (*ops_iterate)(const bmfp_solver_t &, hipStream_t) = nullptr;
   int (*ops_jacobi)(const bmfp_solver_t &, hipStream_t) = nullptr;
+  int (*spmv_groups)(const bmfp_solver_t &) = nullptr;  // split schedule's SpMV grid
   bmfp::RefineState *refine = nullptr;  // set for bmfp_refine_* handles only
   ~bmfp_solver_t();
 };

@@ -51,11 +51,13 @@ struct SolverVecs {
   AT *__restrict__ s;
   AT *__restrict__ t;
   const AT *__restrict__ pinv;   // Jacobi inverse diagonal, or null
-  double *__restrict__ partA;    // [NP]
-  double *__restrict__ partB;    // [NP][2]
+  double *__restrict__ partA;    // [npa]
+  double *__restrict__ partB;    // [npb][2]
   double *__restrict__ partC;    // [NP][2]
+  double *__restrict__ partW;    // split schedule: per-workgroup partials of its SpMVs
   SolverState *st;
   int64_t n;                     // scalar rows
+  int npa, npb;                  // partial sums in partA / partB (NP fused, 1 split)
 };
 
 // ---- fixed-order reductions ----------------------------------------------------
@@ -202,6 +204,19 @@ struct OperandXD {
     load_n<float, N, cmin(ALIGN / 2 > 0 ? ALIGN / 2 : 1, 16)>(d + i, dd);
 #pragma unroll
     for (int c = 0; c < N; ++c) out[c] = xx[c] + (double) dd[c];
+  }
+};
+
+// Plain vector operand (split schedule): the SpMV reads x as stored.
+template <class AT>
+struct OperandVec {
+  const AT *__restrict__ x;
+  __device__ __forceinline__ AT raw(int64_t i) const { return x[i]; }
+  __device__ __forceinline__ AT operator()(int64_t i) const { return x[i]; }
+  template <int N, int ALIGN>
+  __device__ __forceinline__ void load(int64_t i, AT (&out)[N]) const
+  {
+    load_n<AT, N, ALIGN>(x + i, out);
   }
 };
 
@@ -495,6 +510,179 @@ struct StagedShflEngine {
   }
 };
 
+// DenseEngine: the lane mapping of k_bcsr_dgr (row-major blocks).  G =
+// BS/VEC lanes per scalar row, lane g owning columns [g*VEC, (g+1)*VEC) of
+// every block, U blocks' loads issued together, the group's partial sums
+// combined by the same fixed shuffle tree.  The split schedule launches it
+// like the standalone kernel, one wavefront per tile of 64/G rows (tiles()
+// wavefronts, 4 per workgroup): a persistent loop around the same body held
+// 102 instead of 68 VGPRs at bs 4 and ran the SpMV 1.4x slower.
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+struct DenseEngine {
+  static constexpr int G = BS / VEC, RPW = 64 / G;
+  static_assert(BS % VEC == 0 && G <= 64, "bad group shape");
+  static int64_t threads(int64_t n_brows) { return (n_brows * BS + RPW - 1) / RPW * 64; }
+  // workgroups of a one-tile-per-wavefront launch
+  static int64_t groups1(int64_t n_brows) { return ((n_brows * BS + RPW - 1) / RPW + 3) / 4; }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void tile(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t wt)
+  {
+    constexpr int ALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
+    constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * BS), lowbit((int) sizeof(AT) * VEC));
+    const int lane = threadIdx.x & 63;
+    const int lr = lane / G, g = lane - lr * G;
+    const int64_t rows = (int64_t) M.n_brows * BS;
+    const int64_t row = wt * RPW + lr;
+    const bool on = lane < RPW * G && row < rows;
+    const int64_t br = on ? row / BS : 0;
+    const int r = (int) (row - br * BS);
+    int b = 0, b1 = 0;
+    AT sc = 1, pre = 0;  // the row's scale and epilogue operand, loaded with its bounds
+    if (on) {
+      b = M.rp[br];
+      b1 = M.rp[br + 1];
+      if constexpr (SCALED) sc = (AT) M.scale[br];
+      if (g == 0) pre = epi.pre(row);
+    }
+    AT acc = 0;
+    for (; b + U <= b1; b += U) {
+      int col[U];
+      VT v[U][VEC];
+      AT xv[U][VEC];
+#pragma unroll
+      for (int k = 0; k < U; ++k) col[k] = M.cols[b + k];
+#pragma unroll
+      for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN>(M.vals + ((int64_t) (b + k) * BS + r) * BS + g * VEC, v[k]);
+#pragma unroll
+      for (int k = 0; k < U; ++k) op.template load<VEC, XALIGN>((int64_t) col[k] * BS + g * VEC, xv[k]);
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[k][c]), xv[k][c], acc);
+    }
+    for (; U > 1 && b < b1; ++b) {
+      VT v[VEC];
+      AT xv[VEC];
+      load_n<VT, VEC, ALIGN>(M.vals + ((int64_t) b * BS + r) * BS + g * VEC, v);
+      op.template load<VEC, XALIGN>((int64_t) M.cols[b] * BS + g * VEC, xv);
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
+    }
+    acc = group_tree_sum<G>(acc, lane, lr, g);
+    if (on && g == 0) {
+      if constexpr (SCALED) acc *= sc;
+      epi(row, acc, pre);
+    }
+  }
+  // one tile per wavefront (grid of groups1 workgroups)
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    tile(M, op, epi, ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6);
+  }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    const int64_t n_tiles = ((int64_t) M.n_brows * BS + RPW - 1) / RPW;
+    const int64_t waves = (int64_t) gridDim.x * 4;
+    for (int64_t wt = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6; wt < n_tiles; wt += waves) tile(M, op, epi, wt);
+  }
+};
+
+// Rows per thread for RowsEngine: as many consecutive rows of a row-major
+// block as fill one 16-byte load (1 when a single row already does).
+template <class VT, int BS> constexpr int rows_per_thread()
+{
+  int r = 1;
+  while (BS % (2 * r) == 0 && 2 * r * BS * (int) sizeof(VT) <= 16) r *= 2;
+  return r;
+}
+
+// RowsEngine: one thread per R consecutive scalar rows of a block row (row-
+// major blocks), R = rows_per_thread: its R block rows are one contiguous
+// load (16 bytes for FP16 bs 4 and FP32 bs 2, where a single row is 8), the
+// x segment one more; U blocks' loads issued together; no cross-lane sums.
+template <class VT, class AT, int BS, int U, bool SCALED>
+struct RowsEngine {
+  static constexpr int R = rows_per_thread<VT, BS>();
+  static constexpr int TPB = BS / R;  // threads per block row
+  static int64_t threads(int64_t n_brows) { return n_brows * TPB; }
+  static int64_t groups1(int64_t n_brows) { return (n_brows * TPB + 255) / 256; }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void item(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t t)
+  {
+    constexpr int CC = BS % 8 == 0 ? 8 : BS;  // columns per step (bounds registers)
+    constexpr int VALIGN = cmin(lowbit((int) sizeof(VT) * BS * R), 16);
+    constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * CC), lowbit((int) sizeof(AT) * BS));
+    const int br = (int) (t / TPB);
+    const int r0 = (int) (t - (int64_t) br * TPB) * R;
+    const int b0 = M.rp[br], b1 = M.rp[br + 1];
+    // the scale and the epilogue operands load with the bounds, not after the row
+    const AT sc = SCALED ? (AT) M.scale[br] : (AT) 1;
+    AT pre[R], acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      pre[i] = epi.pre((int64_t) br * BS + r0 + i);
+      acc[i] = 0;
+    }
+    int b = b0;
+    if constexpr (U > 1) {
+      for (; b + U <= b1; b += U) {
+        int col[U];
+        VT v[U][R * BS];
+        AT xv[U][BS];
+#pragma unroll
+        for (int k = 0; k < U; ++k) col[k] = M.cols[b + k];
+#pragma unroll
+        for (int k = 0; k < U; ++k) load_n<VT, R * BS, VALIGN>(M.vals + ((int64_t) (b + k) * BS + r0) * BS, v[k]);
+#pragma unroll
+        for (int k = 0; k < U; ++k) op.template load<BS, XALIGN>((int64_t) col[k] * BS, xv[k]);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc[i] = fmadd(cvt<AT>(v[k][i * BS + c]), xv[k][c], acc[i]);
+      }
+    }
+#pragma unroll 3
+    for (; b < b1; ++b) {
+      const int64_t x0 = (int64_t) M.cols[b] * BS;
+      const VT *blk = M.vals + ((int64_t) b * BS + r0) * BS;
+#pragma unroll
+      for (int c0 = 0; c0 < BS; c0 += CC) {
+        AT xv[CC];
+        op.template load<CC, XALIGN>(x0 + c0, xv);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          VT v[CC];
+          load_n<VT, CC, cmin(VALIGN, lowbit((int) sizeof(VT) * CC))>(blk + i * BS + c0, v);
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc[i] = fmadd(cvt<AT>(v[c]), xv[c], acc[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if constexpr (SCALED) acc[i] *= sc;
+      epi((int64_t) br * BS + r0 + i, acc[i], pre[i]);
+    }
+  }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    if (t < (int64_t) M.n_brows * TPB) item(M, op, epi, t);
+  }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  {
+    const int64_t total = (int64_t) M.n_brows * TPB;
+    for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t) gridDim.x * 256)
+      item(M, op, epi, t);
+  }
+};
+
 // Columns per lane for GroupEngine: one 16-byte load, two for FP64 bs >= 12.
 template <class VT> constexpr int engine_vec(int bs)
 {
@@ -517,6 +705,25 @@ using SolverEngine = typename std::conditional<
     typename std::conditional<(sizeof(VT) == 4 && BS <= 4),
                               StagedShflEngine<VT, AT, BS, LAYOUT, SCALED>,
                               ShflEngine<VT, AT, BS, LAYOUT, SCALED>>::type>::type;
+
+// Engine of the split schedule's SpMVs (a stored operand), row-major blocks:
+// thread per row for FP64 bs 4 (on the 2048x1024 bs 4 system 497 / 478 us
+// per SpMV against 546 / 534 us with the dense lane groups), the dense lane
+// groups of the standalone FP64 kernel for 6+ columns, 16-byte row groups
+// for FP32 / FP16 blocks of 4-8 columns (FP16 bs 4: batches of 9 blocks,
+// 168 / 156 us against 178 / 168 with 3); the fused schedule's engines
+// otherwise.
+template <class VT, int BS> constexpr int split_vec() { return dense_shape<VT, BS>().vec > 0 ? dense_shape<VT, BS>().vec : 1; }
+template <class VT, int BS> constexpr int split_u() { return dense_shape<VT, BS>().u > 0 ? dense_shape<VT, BS>().u : 1; }
+template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
+using SplitEngine = typename std::conditional<
+    (LAYOUT == BMFP_ROW_MAJOR && sizeof(VT) == 8 && BS == 4), RowsEngine<VT, AT, BS, 1, SCALED>,
+    typename std::conditional<
+        (LAYOUT == BMFP_ROW_MAJOR && sizeof(VT) == 8 && BS >= 6 && dense_shape<VT, BS>().vec > 0),
+        DenseEngine<VT, AT, BS, split_vec<VT, BS>(), split_u<VT, BS>(), SCALED>,
+        typename std::conditional<(LAYOUT == BMFP_ROW_MAJOR && sizeof(VT) < 8 && BS >= 4 && BS <= 8),
+                                  RowsEngine<VT, AT, BS, (BS <= 4 ? 9 : 3), SCALED>,
+                                  SolverEngine<VT, AT, BS, LAYOUT, SCALED>>::type>::type>::type;
 
 // Init: x = 0, r = r^ = b, p = v = 0 (both parities); partC = ((b,b), (b,b)).
 template <class AT>
@@ -550,13 +757,16 @@ __global__ __launch_bounds__(256) void k_bicg_init(SolverVecs<AT> V, double eps,
   }
 }
 
-template <class VT, class AT, class ENGINE>
-__global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT> V)
+// Head of kernel A (and of the split schedule's p kernel): reduces the
+// previous iteration's (r^, r), (r, r), applies the stopping tests in a fixed
+// order and, when the iteration goes on, returns beta and omega_prev.  WG 0
+// records the outcome in the state.  Uniform over the workgroup.
+template <class AT>
+__device__ __forceinline__ bool bicg_head_a(const SolverVecs<AT> &V, double (*lds)[4], int &it, double &beta,
+                                            double &omega_p)
 {
-  __shared__ double lds[2][4];
   SolverState *st = V.st;
-  if (solver_stopped(st)) return;
-  const int it = st->iter;
+  it = st->iter;
   double red[2];
   reduce_partials<2>(V.partC, gridDim.x, red, lds);
   const double rho_c = red[0], rr = red[1];
@@ -579,12 +789,75 @@ __global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT
       st->iter_b = it;
     }
   }
-  if (stop != 1) return;
-  const int pv = (it - 1) & 1, cu = it & 1;
+  if (stop != 1) return false;
+  const int pv = (it - 1) & 1;
   const double rho_p = it == 0 ? 1.0 : st->rho[pv];
   const double alpha_p = it == 0 ? 1.0 : st->alpha[pv];
-  const double omega_p = it == 0 ? 1.0 : st->omega[pv];
-  const double beta = rho_c / rho_p * alpha_p / omega_p;
+  omega_p = it == 0 ? 1.0 : st->omega[pv];
+  beta = rho_c / rho_p * alpha_p / omega_p;
+  return true;
+}
+
+// Head of kernel B (and of the split schedule's s kernel): reduces (r^, v)
+// and returns alpha = rho / (r^, v), or stops on a zero or non-finite (r^, v).
+template <class AT>
+__device__ __forceinline__ bool bicg_head_b(const SolverVecs<AT> &V, double (*lds)[4], int &it, double &alpha)
+{
+  SolverState *st = V.st;
+  it = st->iter_b;
+  double red[1];
+  reduce_partials<1>(V.partA, V.npa, red, lds);
+  const double rhv = red[0];
+  const double rho_c = st->rho[it & 1];
+  const int stop = !isfinite(rhv) ? BMFP_ERR_NONFINITE : rhv == 0.0 ? BMFP_ERR_BREAKDOWN : 1;
+  alpha = rho_c / rhv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (stop != 1) {
+      st->status = stop;
+      st->done = 1;
+    } else {
+      st->alpha[it & 1] = alpha;
+      st->iter_c = it;
+    }
+  }
+  return stop == 1;
+}
+
+// Head of kernel C: reduces (t, s), (t, t) into omega.
+template <class AT>
+__device__ __forceinline__ void bicg_head_c(const SolverVecs<AT> &V, double (*lds)[4], int &it, double &alpha,
+                                            double &omega, double &tt)
+{
+  SolverState *st = V.st;
+  it = st->iter_c;
+  double red[2];
+  reduce_partials<2>(V.partB, V.npb, red, lds);
+  const double ts = red[0];
+  tt = red[1];
+  omega = tt == 0.0 ? 0.0 : ts / tt;
+  alpha = st->alpha[it & 1];
+}
+
+template <class AT>
+__device__ __forceinline__ void bicg_tail_c(const SolverVecs<AT> &V, int it, double omega, double tt)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    SolverState *st = V.st;
+    st->omega[it & 1] = omega;
+    st->tt[it & 1] = tt;
+    st->iter = it + 1;
+  }
+}
+
+template <class VT, class AT, class ENGINE>
+__global__ __launch_bounds__(256) void k_bicg_a(MatArgs<VT, AT> M, SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  if (solver_stopped(V.st)) return;
+  int it;
+  double beta, omega_p;
+  if (!bicg_head_a(V, lds, it, beta, omega_p)) return;
+  const int pv = (it - 1) & 1, cu = it & 1;
   OperandP<AT> op{V.r, V.p[pv], V.v[pv], V.pinv, (AT) beta, (AT) omega_p};
   double part[1] = {0.0};
   AT *__restrict__ pc = V.p[cu];
@@ -601,25 +874,10 @@ template <class VT, class AT, class ENGINE>
 __global__ __launch_bounds__(256) void k_bicg_b(MatArgs<VT, AT> M, SolverVecs<AT> V)
 {
   __shared__ double lds[2][4];
-  SolverState *st = V.st;
-  if (solver_stopped(st)) return;
-  const int it = st->iter_b;
-  double red[1];
-  reduce_partials<1>(V.partA, gridDim.x, red, lds);
-  const double rhv = red[0];
-  const double rho_c = st->rho[it & 1];
-  const int stop = !isfinite(rhv) ? BMFP_ERR_NONFINITE : rhv == 0.0 ? BMFP_ERR_BREAKDOWN : 1;
-  const double alpha = rho_c / rhv;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (stop != 1) {
-      st->status = stop;
-      st->done = 1;
-    } else {
-      st->alpha[it & 1] = alpha;
-      st->iter_c = it;
-    }
-  }
-  if (stop != 1) return;
+  if (solver_stopped(V.st)) return;
+  int it;
+  double alpha;
+  if (!bicg_head_b(V, lds, it, alpha)) return;
   OperandS<AT> op{V.r, V.v[it & 1], V.pinv, (AT) alpha};
   double part[2] = {0.0, 0.0};
   ENGINE::run(M, op, [&](int64_t row, AT y) {
@@ -636,14 +894,10 @@ template <class AT>
 __global__ __launch_bounds__(256) void k_bicg_c(SolverVecs<AT> V)
 {
   __shared__ double lds[2][4];
-  SolverState *st = V.st;
-  if (solver_stopped(st)) return;
-  const int it = st->iter_c;
-  double red[2];
-  reduce_partials<2>(V.partB, gridDim.x, red, lds);
-  const double ts = red[0], tt = red[1];
-  const double omega = tt == 0.0 ? 0.0 : ts / tt;
-  const double alpha = st->alpha[it & 1];
+  if (solver_stopped(V.st)) return;
+  int it;
+  double alpha, omega, tt;
+  bicg_head_c(V, lds, it, alpha, omega, tt);
   const AT a = (AT) alpha, w = (AT) omega;
   const AT *__restrict__ p = V.p[it & 1];
   double part[2] = {0.0, 0.0};
@@ -659,11 +913,181 @@ __global__ __launch_bounds__(256) void k_bicg_c(SolverVecs<AT> V)
     part[1] += (double) rn * (double) rn;
   }
   write_partials<2>(V.partC, part, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->omega[it & 1] = omega;
-    st->tt[it & 1] = tt;
-    st->iter = it + 1;
+  bicg_tail_c(V, it, omega, tt);
+}
+
+// ---- split schedule ------------------------------------------------------------------
+// The same iteration as kernels A, B, C, in five kernels whose SpMVs read a
+// stored operand instead of forming it at every gathered index:
+//   P   p = r + beta (p - omega v)  [q = P^-1 p]            (elementwise)
+//   SV  v = A q, partial (r^, v) per workgroup              (SpMV)
+//   R   (r^, v) summed in a fixed order                     (16 workgroups)
+//   S   s = r - alpha v  [z = P^-1 s]                       (elementwise)
+//   ST  t = A z, partial (t, s), (t, t) per workgroup       (SpMV)
+//   R   (t, s), (t, t) summed in a fixed order              (16 workgroups)
+//   X   r = s - omega t, x = (x + alpha q) + omega z, partial (r^, r), (r, r)
+// Same formulas and the same fused multiply-adds per element as A, B, C; the
+// SpMV runs on the dense engines of the standalone kernels, so it moves its
+// bytes at the standalone rate (forming the operand at each gather cost the
+// fused kernels 30-60% per SpMV against about 8% for the extra passes).
+// Buffers: p = p[0], q = p[1], v = v[0], z = v[1] (q, z with Jacobi only;
+// without it the SpMVs read p and s directly).
+template <class AT>
+__global__ __launch_bounds__(256) void k_bicg_sp(SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  if (solver_stopped(V.st)) return;
+  int it;
+  double beta, omega_p;
+  if (!bicg_head_a(V, lds, it, beta, omega_p)) return;
+  const AT bt = (AT) beta, om = (AT) omega_p;
+  AT *__restrict__ p = V.p[0];
+  AT *__restrict__ q = V.p[1];
+  const AT *__restrict__ v = V.v[0];
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < V.n; i += (int64_t) gridDim.x * 256) {
+    const AT pn = p_update(V.r[i], p[i], v[i], bt, om);
+    p[i] = pn;
+    if (V.pinv) q[i] = V.pinv[i] * pn;
   }
+}
+
+// Engines with a one-tile-per-wavefront launch (groups1) run the split
+// schedule's SpMVs in that shape; others in their persistent grid.
+template <class E, class = void> struct one_tile : std::false_type {};
+template <class E> struct one_tile<E, decltype((void) &E::groups1)> : std::true_type {};
+
+// The stop flag: a one-tile launch has tens of thousands of workgroups, so
+// each wavefront tests the flag itself (a scalar load, no workgroup barrier)
+// and a stopped solve's launches end at once; persistent engines check it
+// once per workgroup.  The flag is written only by earlier kernels, so every
+// wavefront of a workgroup takes the same branch.
+template <class ENGINE, class VT, class AT, class OP, class EPI>
+__device__ __forceinline__ bool split_run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, const SolverState *st)
+{
+  if constexpr (one_tile<ENGINE>::value) {
+    if (__builtin_amdgcn_readfirstlane(st->done)) return false;
+    ENGINE::run1(M, op, epi);
+  } else {
+    if (solver_stopped(st)) return false;
+    ENGINE::run(M, op, epi);
+  }
+  return true;
+}
+
+// Split SpMV epilogues: store y and accumulate its dot products.  The
+// one-tile engines call pre(row) with the row bounds and pass the value on;
+// the persistent engines call the two-argument form.
+template <class AT>
+struct EpiV {  // v = y; (r^, v)
+  AT *__restrict__ v;
+  const AT *__restrict__ rh;
+  double part[1] = {0.0};
+  __device__ __forceinline__ AT pre(int64_t row) const { return rh[row]; }
+  __device__ __forceinline__ void operator()(int64_t row, AT y, AT rhv)
+  {
+    v[row] = y;
+    part[0] += (double) rhv * (double) y;
+  }
+  __device__ __forceinline__ void operator()(int64_t row, AT y) { (*this)(row, y, rh[row]); }
+};
+template <class AT>
+struct EpiT {  // t = y; (t, s), (t, t)
+  AT *__restrict__ t;
+  const AT *__restrict__ s;
+  double part[2] = {0.0, 0.0};
+  __device__ __forceinline__ AT pre(int64_t row) const { return s[row]; }
+  __device__ __forceinline__ void operator()(int64_t row, AT y, AT si)
+  {
+    t[row] = y;
+    part[0] += (double) y * (double) si;
+    part[1] += (double) y * (double) y;
+  }
+  __device__ __forceinline__ void operator()(int64_t row, AT y) { (*this)(row, y, s[row]); }
+};
+
+template <class VT, class AT, class ENGINE>
+__global__ __launch_bounds__(256) void k_bicg_sv(MatArgs<VT, AT> M, SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  const OperandVec<AT> op{V.pinv ? V.p[1] : V.p[0]};
+  EpiV<AT> epi{V.v[0], V.rh};
+  if (split_run<ENGINE>(M, op, epi, V.st)) write_partials<1>(V.partW, epi.part, lds);
+}
+
+// Sums the `count` per-workgroup partials of a split SpMV (K values each):
+// workgroup g of the REDUCE_GROUPS adds the contiguous slice g of them in a
+// fixed order and writes out[g*K + k]; the next kernel's head adds those
+// REDUCE_GROUPS values in order (V.npa / V.npb).  One workgroup alone took
+// 8-10 us for 33k partials.
+constexpr int REDUCE_GROUPS = 16;
+template <int K>
+__global__ __launch_bounds__(256) void k_bicg_reduce(const double *__restrict__ in, int count,
+                                                     double *__restrict__ out, const SolverState *st)
+{
+  __shared__ double lds[2][4];
+  if (st->done) return;  // uniform: written only by earlier kernels
+  const int per = (count + REDUCE_GROUPS - 1) / REDUCE_GROUPS;
+  const int g0 = blockIdx.x * per, g1 = g0 + per < count ? g0 + per : count;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+#pragma unroll 4
+  for (int g = g0 + threadIdx.x; g < g1; g += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += in[(int64_t) g * K + k];
+  write_partials<K>(out, v, lds);
+}
+
+template <class AT>
+__global__ __launch_bounds__(256) void k_bicg_ss(SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  if (solver_stopped(V.st)) return;
+  int it;
+  double alpha;
+  if (!bicg_head_b(V, lds, it, alpha)) return;
+  const AT a = (AT) alpha;
+  const AT *__restrict__ v = V.v[0];
+  AT *__restrict__ z = V.v[1];
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < V.n; i += (int64_t) gridDim.x * 256) {
+    const AT si = s_update(V.r[i], v[i], a);
+    V.s[i] = si;
+    if (V.pinv) z[i] = V.pinv[i] * si;
+  }
+}
+
+template <class VT, class AT, class ENGINE>
+__global__ __launch_bounds__(256) void k_bicg_st(MatArgs<VT, AT> M, SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  const OperandVec<AT> op{V.pinv ? V.v[1] : V.s};
+  EpiT<AT> epi{V.t, V.s};
+  if (split_run<ENGINE>(M, op, epi, V.st)) write_partials<2>(V.partW, epi.part, lds);
+}
+
+template <class AT>
+__global__ __launch_bounds__(256) void k_bicg_sx(SolverVecs<AT> V)
+{
+  __shared__ double lds[2][4];
+  if (solver_stopped(V.st)) return;
+  int it;
+  double alpha, omega, tt;
+  bicg_head_c(V, lds, it, alpha, omega, tt);
+  const AT a = (AT) alpha, w = (AT) omega;
+  const AT *__restrict__ q = V.pinv ? V.p[1] : V.p[0];
+  const AT *__restrict__ z = V.pinv ? V.v[1] : V.s;
+  double part[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < V.n; i += (int64_t) gridDim.x * 256) {
+    const AT si = V.s[i];
+    const AT h = fmadd(a, q[i], V.x[i]);
+    const AT rn = fmadd(-w, V.t[i], si);
+    V.x[i] = fmadd(w, z[i], h);
+    V.r[i] = rn;
+    part[0] += (double) V.rh[i] * (double) rn;
+    part[1] += (double) rn * (double) rn;
+  }
+  write_partials<2>(V.partC, part, lds);
+  bicg_tail_c(V, it, omega, tt);
 }
 
 // Point-Jacobi inverse diagonal from BCSR diagonal blocks (bicgstab.cu:201-225:
