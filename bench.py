@@ -195,30 +195,35 @@ SERVE_STREAM = 20000
 SERVE_BS = (2, 3, 4, 6, 8)
 
 
-def serving_stream(n_sample, seed=2):
-    """The request stream: 20,000 requests from `seed`; the first n_sample are used.
-    Per request: square grid side U{16..512}, bs from {2,3,4,6,8}, 1-4
-    right-hand sides uniform[-1,1], tolerance 1e-10; unit-rate exponential
-    inter-arrival gaps (divided by the arrival rate)."""
+def serving_stream(seed=2):
+    """The request stream: 20,000 requests from `seed`.  Per request: square
+    grid side U{16..512}, bs from {2,3,4,6,8}, 1-4 right-hand sides
+    uniform[-1,1], tolerance 1e-10; unit-rate exponential inter-arrival gaps
+    (divided by the arrival rate)."""
     rng = np.random.default_rng(seed)
     side = rng.integers(16, 513, size=SERVE_STREAM)
     bs = rng.choice(SERVE_BS, size=SERVE_STREAM)
     nrhs = rng.integers(1, 5, size=SERVE_STREAM)
     arrivals = np.cumsum(rng.exponential(1.0, size=SERVE_STREAM))
     reqs = [{"side": int(side[i]), "bs": int(bs[i]), "nrhs": int(nrhs[i]),
-             "seed": (seed << 32) + i, "rhs_seed": (seed << 32) + i} for i in range(n_sample)]
-    return reqs, arrivals[:n_sample] - arrivals[0]
+             "seed": (seed << 32) + i, "rhs_seed": (seed << 32) + i} for i in range(SERVE_STREAM)]
+    return reqs, arrivals - arrivals[0]
 
 
-def plain_one_at_a_time(reqs, max_iter, use_jacobi=False):
-    """Each request's right-hand sides solved one after another by the plain
-    FP64 BiCGStab (x0 = 0, 1e-10), with point-Jacobi preconditioning when
-    use_jacobi (as gpu_bicgstab(A, true)).  Timed: the solve calls only
-    (matrix generation, solver set-up and its first graph build are
-    excluded).  Compulsory bytes per iteration of the fused solver: two FP64
+def plain_one_at_a_time(reqs, max_iter, stream, use_jacobi=False):
+    """Each request served alone by the plain FP64 BiCGStab (x0 = 0, 1e-10),
+    its right-hand sides one after another, with point-Jacobi preconditioning
+    when use_jacobi (as gpu_bicgstab(A, true)).  Timed per request, as the
+    served loop is charged: the generation of its FP64 matrix on the device
+    (bmfp_stencil_generate, HIP events) and the device time of each solve
+    (info.seconds: first kernel to the solution copy, host round trips
+    between iteration batches included).  Not timed: allocations, solver
+    set-up and graph capture, and the FP64 true-residual check each solve
+    runs after it has stopped.  Compulsory bytes per iteration: two FP64
     SpMVs (matrix, x read, y written) and 15 further FP64 vector passes (two
-    more for the inverse diagonal with Jacobi)."""
-    total, per_req, xs, iters, moved = 0.0, [], [], [], 0.0
+    more with the inverse diagonal)."""
+    total, per_req, xs, iters, moved, gen_s = 0.0, [], [], [], 0.0, 0.0
+    e0, e1 = bm.Event(), bm.Event()
     for r in reqs:
         A = bm.Matrix.stencil(r["side"], r["side"], r["bs"], 0, seed=r["seed"])
         n = A.y_len
@@ -226,15 +231,19 @@ def plain_one_at_a_time(reqs, max_iter, use_jacobi=False):
         B.fill(1, seed=r["rhs_seed"], stream_id=6)
         x = bm.DeviceArray(n)
         S = bm.BiCGStab(A, use_jacobi=use_jacobi)
-        S.solve(B.view(0, n), x, 1e-10, 2)            # graph build, untimed
-        t_req, xr = 0.0, []
+        S.solve(B.view(0, n), x, 1e-10, 2)            # graph capture, untimed
+        bm.synchronize()
+        e0.record(stream.handle)
+        A.regenerate(0, r["seed"], stream.handle)     # the request's matrix, as admission builds it
+        e1.record(stream.handle)
+        t_req = e1.elapsed_since(e0)
+        gen_s += t_req
+        xr = []
         for k in range(r["nrhs"]):
-            bm.synchronize()
-            t0 = time.perf_counter()
-            info = S.solve(B.view(k * n, n), x, 1e-10, max_iter)
-            t_req += time.perf_counter() - t0
+            info = S.solve(B.view(k * n, n), x, 1e-10, max_iter, stream=stream.handle)
             if info.status != bm.OK:
                 raise RuntimeError(f"plain FP64 BiCGStab failed on {r}: status {info.status}")
+            t_req += info.seconds
             iters.append(info.iterations)
             moved += info.iterations * (2.0 * (8.0 * A.nnzb * r["bs"] ** 2 + 4.0 * A.nnzb + 4.0 * (A.n_rows + 1)
                                                + 16.0 * n) + (17 if use_jacobi else 15) * 8.0 * n)
@@ -244,38 +253,46 @@ def plain_one_at_a_time(reqs, max_iter, use_jacobi=False):
         xs.append(np.stack(xr))
         S.destroy()
         A.destroy()
-    return total, np.array(per_req), xs, iters, moved
+    return total, np.array(per_req), xs, iters, moved, gen_s
 
 
-def request_bytes_moved(r, outer, inner, jacobi=False):
+SERVE_F16_MIN_BS = 6  # the server's default inner_f16_min_bs
+
+
+def request_bytes_moved(r, outer, inner, jacobi=False, f16_min_bs=SERVE_F16_MIN_BS):
     """Compulsory HBM bytes of one served request (right-hand sides share every
-    matrix read).  Refinement step 0 (x = 0): b read, x' and r32 written, no
-    matrix.  Later steps: the FP64 matrix once, x, d, b read and x', r32
-    written.  Inner iteration: the FP32 matrix twice and 68 bytes of FP32
-    vector traffic per right-hand side and row (v = A p: ri, p, v, r32 read,
-    p, v written; t = A s: ri, v read, s, t written; updates: s, p, d, t, r32
-    read, d, ri written), plus the FP32 inverse diagonal read three times per
-    row with Jacobi."""
+    matrix read), per row and right-hand side unless stated.  Refinement step
+    0 (x = 0): x' written, b read, r32 written (20 B), no matrix.  Later steps:
+    x' = x + d (x, d read, x' written) then the FP64 matrix once with x', b
+    read and r32 written (40 B).  Inner iteration: the FP32 matrix twice and
+    76 B of FP32 vectors (p = ri + beta (p - omega v): 16; v = A p: p, r32
+    read, v written; s = ri - alpha v: 12; t = A s: s read, t written; the
+    updates: s, p, d, t, r32 read, d, ri written), plus the FP32 inverse
+    diagonal read three times per row with Jacobi.  Requests with bs >=
+    f16_min_bs keep the inner matrix in FP16 with one FP32 scale per block
+    row."""
     side, bs, nr = r["side"], r["bs"], r["nrhs"]
     nodes = side * side
     nnzb = (3 * side - 2) ** 2
     n = nodes * bs
     idx = 4.0 * nnzb + 4.0 * (nodes + 1)
-    first = nr * n * (8 + 8 + 4)
-    resid = 8.0 * nnzb * bs * bs + idx + nr * n * (8 + 4 + 8 + 8 + 4)
-    inner_it = 2 * (4.0 * nnzb * bs * bs + idx) + nr * n * 68.0 + (12.0 * n if jacobi else 0.0)
+    first = nr * n * 20.0
+    resid = 8.0 * nnzb * bs * bs + idx + nr * n * 40.0
+    inner_vals = 2.0 * nnzb * bs * bs + 4.0 * nodes if (f16_min_bs and bs >= f16_min_bs) else 4.0 * nnzb * bs * bs
+    inner_it = 2 * (inner_vals + idx) + nr * n * 76.0 + (12.0 * n if jacobi else 0.0)
     return (first if outer > 0 else 0.0) + max(outer - 1, 0) * resid + inner * inner_it
 
 
 def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
-    """Replays the stream at `lam` requests/s in real time through bmfp_server.
-    Latency = completion seen by the host - scheduled arrival.  Like the plain
+    """Replays the requests at `lam` requests/s in real time through
+    bmfp_server.  Latency = completion seen by the host - scheduled arrival;
+    solves/s = requests / (last completion - first arrival).  Like the plain
     comparator's timing, solutions stay in device memory; with ref_x (an
     untimed accuracy pass) they are downloaded and compared."""
     S = bm.Server(**cfg)
     schedule = S.schedule()
     n = len(reqs)
-    arr = arrivals / lam
+    arr = arrivals[:n] / lam
     lat = np.zeros(n)
     ids = {}
     xs = {}
@@ -283,6 +300,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
     i = done = 0
     t0 = time.perf_counter()
     t_last = 0.0
+    next_log = 5.0
     while done < n:
         now = time.perf_counter() - t0
         while i < n and arr[i] <= now:
@@ -305,10 +323,14 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
             outer.append(res.outer_iterations)
             inner.append(res.inner_iterations)
             moved += request_bytes_moved(reqs[j], res.outer_iterations, res.inner_iterations,
-                                         bool(cfg.get("use_jacobi", 0)))
+                                         bool(cfg.get("use_jacobi", 0)),
+                                         cfg.get("inner_f16_min_bs", SERVE_F16_MIN_BS))
             if ref_x is not None:
                 xs[j] = x
             done += 1
+        if now > next_log:
+            log(f"[bench]   lambda={lam}: {done}/{n} done, {i - done} in the server, t={now:.1f} s")
+            next_log += 5.0
         if a == 0 and q == 0 and i < n:
             time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
     busy = S.stats()["busy_seconds"]
@@ -332,78 +354,81 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
     return out
 
 
-def serving_comparison(n_sample, lambdas, max_iter, cfg_overrides=None, plain=True):
-    reqs, arrivals = serving_stream(n_sample)
-    log(f"[bench] serving sample: {n_sample} requests, "
-        f"{sum(r['nrhs'] for r in reqs)} right-hand sides")
-    t0 = time.perf_counter()
-    if plain:
-        plain_s, plain_per, ref_x, plain_it, plain_bytes = plain_one_at_a_time(reqs, max_iter)
-        log(f"[bench] plain FP64 one at a time: {plain_s:.3f} s of solves "
-            f"({n_sample / plain_s:.1f} solves/s), wall {time.perf_counter() - t0:.1f} s")
-    else:
-        plain_s, plain_per, ref_x, plain_it, plain_bytes = float("nan"), np.array([np.nan]), None, [np.nan], 0.0
+def serving_comparison(n_requests, n_low, n_plain, lambdas, max_iter, stream, cfg_overrides=None, plain=True,
+                       jacobi=True):
+    """Config 4: the 20,000-request stream replayed in real time at each rate
+    (the first n_low requests at rates below 500/s, a steady-state prefix),
+    against the plain FP64 BiCGStab solving the first n_plain requests one at
+    a time (a stated sample; its solves/s is the comparison base)."""
+    reqs, arrivals = serving_stream()
+    log(f"[bench] serving stream: {len(reqs)} requests; replayed {n_requests} (lambda >= 500/s) and {n_low} "
+        f"(below); plain sample {n_plain}")
     cfg = {"memory_budget_bytes": int(200e9)}
     cfg.update(cfg_overrides or {})
-    runs = []
+    sample = reqs[:n_plain]
+    plain_res = None
     accuracy = None
-    if ref_x is not None:
-        # every request once more, all at t=0, solutions downloaded: results are
-        # bitwise those of the timed runs (independent of mix and schedule)
-        acc = serve_replay(reqs, arrivals, float("inf"), cfg, ref_x)
-        accuracy = {"max_err_vs_fp64": acc["max_err_vs_fp64"], "max_rel_residual": acc["max_rel_residual"],
-                    "reference": "plain FP64 GPU BiCGStab solutions (1e-10)",
-                    # speedup the served loop would reach moving its compulsory bytes at the
-                    # plain solver's rate (outer/inner counts are the same in every replay)
-                    "compulsory_bytes_ratio_plain_over_served": round(plain_bytes / (acc["compulsory_GB"] * 1e9), 3)}
-        log(f"[bench] served solutions vs plain FP64: max rel err {acc['max_err_vs_fp64']:.2e}, "
-            f"max rel residual {acc['max_rel_residual']:.2e}; compulsory bytes plain / served "
-            f"{accuracy['compulsory_bytes_ratio_plain_over_served']:.2f} (plain at {plain_bytes / plain_s / 1e9:.0f} GB/s)")
-    for k, lam in enumerate(lambdas):
-        r = serve_replay(reqs, arrivals, lam, cfg)
-        r["speedup_vs_plain"] = round(r["solves_per_s"] / (n_sample / plain_s), 3) if plain else None
-        vs = f"{r['speedup_vs_plain']}x plain" if plain else "no plain comparator"
-        log(f"[bench] serve lambda={lam}: {r['solves_per_s']} solves/s ({vs}), "
-            f"p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, busy {r['gpu_busy_frac']}")
-        runs.append(r)
-    jacobi = None
     if plain:
-        # the same stream with point-Jacobi preconditioning on both sides (the
-        # reference's driver solves with gpu_bicgstab(A, true)): plain FP64
-        # Jacobi BiCGStab one at a time against the served loop with Jacobi inner
-        # solves, saturated (the last lambda)
-        pj_s, pj_per, pj_x, pj_it, pj_bytes = plain_one_at_a_time(reqs, max_iter, use_jacobi=True)
-        log(f"[bench] plain FP64 + Jacobi one at a time: {pj_s:.3f} s of solves ({n_sample / pj_s:.1f} solves/s)")
-        cfg_j = dict(cfg, use_jacobi=1)
-        acc_j = serve_replay(reqs, arrivals, float("inf"), cfg_j, pj_x)
-        rj = serve_replay(reqs, arrivals, lambdas[-1], cfg_j)
-        rj["speedup_vs_plain_jacobi"] = round(rj["solves_per_s"] / (n_sample / pj_s), 3)
-        rj["speedup_vs_plain"] = round(rj["solves_per_s"] / (n_sample / plain_s), 3)
-        log(f"[bench] serve + Jacobi lambda={lambdas[-1]}: {rj['solves_per_s']} solves/s "
-            f"({rj['speedup_vs_plain_jacobi']}x plain + Jacobi, {rj['speedup_vs_plain']}x plain), "
-            f"p50 {rj['p50_ms']} ms, p99 {rj['p99_ms']} ms")
-        jacobi = {"plain_fp64_jacobi": {"solves_per_s": round(n_sample / pj_s, 2), "solve_seconds": round(pj_s, 4),
-                                        "GBs": round(pj_bytes / pj_s / 1e9, 1),
-                                        "p50_ms": round(float(np.percentile(pj_per, 50)) * 1e3, 3),
-                                        "p99_ms": round(float(np.percentile(pj_per, 99)) * 1e3, 3),
-                                        "mean_iterations": round(float(np.mean(pj_it)), 1)},
-                  "served_jacobi": rj,
-                  "accuracy": {"max_err_vs_plain_jacobi": acc_j["max_err_vs_fp64"],
-                               "max_rel_residual": acc_j["max_rel_residual"]}}
+        t0 = time.perf_counter()
+        plain_s, plain_per, ref_x, plain_it, plain_bytes, plain_gen = plain_one_at_a_time(sample, max_iter, stream)
+        log(f"[bench] plain FP64 one at a time: {plain_s:.3f} s ({plain_gen:.3f} s generating) "
+            f"({n_plain / plain_s:.1f} solves/s), wall {time.perf_counter() - t0:.1f} s")
+        plain_res = {"solves_per_s": round(n_plain / plain_s, 2), "requests": n_plain,
+                     "seconds": round(plain_s, 4), "generation_seconds": round(plain_gen, 4),
+                     "compulsory_GB": round(plain_bytes / 1e9, 2), "GBs": round(plain_bytes / plain_s / 1e9, 1),
+                     "p50_ms": round(float(np.percentile(plain_per, 50)) * 1e3, 3),
+                     "p99_ms": round(float(np.percentile(plain_per, 99)) * 1e3, 3),
+                     "mean_iterations": round(float(np.mean(plain_it)), 1),
+                     "timing": "per request: device matrix generation + solve device time (info.seconds) of "
+                               "each right-hand side, one request after another"}
+        # the sample once more through the server, all at t = 0, solutions
+        # downloaded: a request's result is bitwise the same in any mix
+        acc = serve_replay(sample, arrivals, float("inf"), cfg, ref_x)
+        accuracy = {"max_err_vs_fp64": acc["max_err_vs_fp64"], "max_rel_residual": acc["max_rel_residual"],
+                    "reference": f"plain FP64 GPU BiCGStab solutions (1e-10) of the first {n_plain} requests",
+                    "compulsory_bytes_ratio_plain_over_served": round(plain_bytes / (acc["compulsory_GB"] * 1e9), 3)}
+        log(f"[bench] served vs plain FP64 on the sample: max rel err {acc['max_err_vs_fp64']:.2e}, "
+            f"max rel residual {acc['max_rel_residual']:.2e}; compulsory bytes plain / served "
+            f"{accuracy['compulsory_bytes_ratio_plain_over_served']:.2f}")
+    runs = []
+    for lam in lambdas:
+        nr = n_requests if lam >= 500 else n_low
+        r = serve_replay(reqs[:nr], arrivals, lam, cfg)
+        r["speedup_vs_plain"] = round(r["solves_per_s"] / plain_res["solves_per_s"], 3) if plain else None
+        vs = f"{r['speedup_vs_plain']}x plain" if plain else "no plain comparator"
+        log(f"[bench] serve lambda={lam} ({nr} requests): {r['solves_per_s']} solves/s ({vs}), "
+            f"p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, busy {r['gpu_busy_frac']}, {r['busy_GBs']} GB/s")
+        runs.append(r)
+    jac = None
+    if plain and jacobi:
+        # point-Jacobi on both sides (the reference's driver solves with
+        # gpu_bicgstab(A, true)), on the plain sample, all at t = 0
+        pj_s, pj_per, pj_x, pj_it, pj_bytes, _ = plain_one_at_a_time(sample, max_iter, stream, use_jacobi=True)
+        acc_j = serve_replay(sample, arrivals, float("inf"), dict(cfg, use_jacobi=1), pj_x)
+        rj = serve_replay(sample, arrivals, float("inf"), dict(cfg, use_jacobi=1))
+        rj["max_err_vs_fp64"] = acc_j["max_err_vs_fp64"]
+        burst = serve_replay(sample, arrivals, float("inf"), cfg)
+        rj["speedup_vs_plain_jacobi"] = round(rj["solves_per_s"] / (n_plain / pj_s), 3)
+        log(f"[bench] the sample at once: served {burst['solves_per_s']} solves/s "
+            f"({burst['solves_per_s'] / plain_res['solves_per_s']:.3f}x plain); Jacobi on both sides: served "
+            f"{rj['solves_per_s']} vs plain {n_plain / pj_s:.1f} solves/s ({rj['speedup_vs_plain_jacobi']}x)")
+        jac = {"plain_fp64_jacobi": {"solves_per_s": round(n_plain / pj_s, 2), "seconds": round(pj_s, 4),
+                                     "mean_iterations": round(float(np.mean(pj_it)), 1)},
+               "served_jacobi_burst": {k: rj[k] for k in ("solves_per_s", "span_s", "p99_ms", "mean_inner",
+                                                          "max_err_vs_fp64", "max_rel_residual")},
+               "speedup_vs_plain_jacobi": rj["speedup_vs_plain_jacobi"],
+               "served_burst_no_jacobi": {k: burst[k] for k in ("solves_per_s", "span_s", "p99_ms", "mean_inner")},
+               "burst_speedup_vs_plain": round(burst["solves_per_s"] / plain_res["solves_per_s"], 3)}
     return {
-        "workload": f"first {n_sample} of a 20,000-request stream (seed 2): side U{{16..512}}, "
-                    f"bs in {{2,3,4,6,8}}, 1-4 RHS uniform[-1,1], tol 1e-10, diagonally dominant "
-                    f"9-point stencils; served by FP64 refinement around FP32 BiCGStab",
+        "workload": "20,000-request stream (seed 2): side U{16..512}, bs in {2,3,4,6,8}, 1-4 RHS uniform[-1,1], "
+                    "tol 1e-10, diagonally dominant 9-point stencils generated on the device at admission; "
+                    "served by FP64 refinement around FP32 BiCGStab",
+        "replayed": {"lambda_ge_500": n_requests, "lambda_lt_500": n_low, "plain_sample": n_plain},
         "config": {k: v for k, v in cfg.items()},
-        "plain_fp64": {"solves_per_s": round(n_sample / plain_s, 2), "solve_seconds": round(plain_s, 4),
-                       "compulsory_GB": round(plain_bytes / 1e9, 2), "GBs": round(plain_bytes / plain_s / 1e9, 1),
-                       "p50_ms": round(float(np.percentile(plain_per, 50)) * 1e3, 3),
-                       "p99_ms": round(float(np.percentile(plain_per, 99)) * 1e3, 3),
-                       "mean_iterations": round(float(np.mean(plain_it)), 1),
-                       "timing": "solve calls only, one request after another"} if plain else None,
+        "plain_fp64": plain_res,
         "served": runs,
         "accuracy": accuracy,
-        "jacobi": jacobi,
+        "jacobi": jac,
     }
 
 
@@ -477,8 +502,13 @@ def main():
     ap.add_argument("--no-solver", action="store_true",
                     help="skip the FP64 vs mixed-precision BiCGStab comparison")
     ap.add_argument("--solver-max-iter", type=int, default=20000)
-    ap.add_argument("--serve-requests", type=int, default=400,
-                    help="requests of the serving stream replayed (0 = skip the serving leg)")
+    ap.add_argument("--serve-requests", type=int, default=SERVE_STREAM,
+                    help="requests of the serving stream replayed at rates >= 500/s (0 = skip the serving leg)")
+    ap.add_argument("--serve-low-requests", type=int, default=4000,
+                    help="requests replayed at rates below 500/s (a steady-state prefix)")
+    ap.add_argument("--serve-plain-sample", type=int, default=400,
+                    help="requests solved one at a time by the plain FP64 BiCGStab (the comparison base)")
+    ap.add_argument("--serve-no-jacobi", action="store_true", help="skip the Jacobi-on-both-sides comparison")
     ap.add_argument("--serve-lambdas", default="200,1000,5000")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--no-csr-rows", action="store_true",
@@ -510,10 +540,11 @@ def main():
         print(json.dumps({"csr_shapes": csr_shape_comparison()}), flush=True)
         return
     if args.serve_only:
-        print(json.dumps(serving_comparison(args.serve_requests,
+        print(json.dumps(serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
                                             [float(v) for v in args.serve_lambdas.split(",")],
-                                            args.solver_max_iter, parse_overrides(args.serve_config),
-                                            plain=not args.serve_no_plain)), flush=True)
+                                            args.solver_max_iter, stream, parse_overrides(args.serve_config),
+                                            plain=not args.serve_no_plain, jacobi=not args.serve_no_jacobi)),
+              flush=True)
         return
 
     cases = []
@@ -640,9 +671,10 @@ def main():
     if not args.no_solver:
         out["solver"] = solver_comparison(stream, args.solver_max_iter)
     if args.serve_requests > 0:
-        out["serving"] = serving_comparison(args.serve_requests,
+        out["serving"] = serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
                                             [float(v) for v in args.serve_lambdas.split(",")],
-                                            args.solver_max_iter)
+                                            args.solver_max_iter, stream, plain=not args.serve_no_plain,
+                                            jacobi=not args.serve_no_jacobi)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         cb = cpu_baseline(args.cpu_seconds)

@@ -118,6 +118,7 @@ def _declare(L):
     L.bmfp_spmv_auto_variant.argtypes = [vp, ip]
     L.bmfp_jit_build.argtypes = [c_int, c_int, c_int, ctypes.POINTER(sz), dp]
     L.bmfp_csr_spmv_noalloc.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp]
+    L.bmfp_stencil_generate.argtypes = [vp, c_int, ctypes.c_uint64, vp]
     _declare_solver(L)
 
 
@@ -388,6 +389,10 @@ class Matrix:
                                     _ptr(rp), _ptr(cols), _ptr(vals)))
         return cls(h)
 
+    def regenerate(self, kind, seed, stream=None):
+        """bmfp_stencil_generate: new values in place (a stencil-made F64 row-major matrix)."""
+        check(lib().bmfp_stencil_generate(self.handle, kind, seed, stream))
+
     def convert(self, dtype, layout=None, stream=None):
         h = ctypes.c_void_p()
         check(lib().bmfp_matrix_convert(self.handle, dtype, self.layout if layout is None else layout,
@@ -552,7 +557,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget_bytes", ctypes.c_size_t), ("max_systems", ctypes.c_int),
                 ("iterations_per_round", ctypes.c_int), ("slot_groups", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int),
-                ("chunks_per_tile", ctypes.c_int), ("use_jacobi", ctypes.c_int)]
+                ("chunks_per_tile", ctypes.c_int), ("use_jacobi", ctypes.c_int),
+                ("inner_f16_min_bs", ctypes.c_int), ("inner_tol_f16", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):

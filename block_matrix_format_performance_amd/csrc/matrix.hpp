@@ -15,6 +15,7 @@ struct bmfp_matrix_t {
   int max_row_blocks = 0;      // longest (block) row
   double mean_row_blocks = 0;  // mean (block) row length
   int csr_group = 0;           // lanes per row chosen for BMFP_CSR_VECTOR
+  int stencil_nx = 0, stencil_ny = 0;  // grid of a bmfp_stencil_create matrix (0: other)
 
   std::shared_ptr<bmfp::DeviceBuffer> row_ptr;  // int[n_rows+1]  (shared by converted copies)
   std::shared_ptr<bmfp::DeviceBuffer> columns;  // int[nnzb]

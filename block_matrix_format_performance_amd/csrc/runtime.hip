@@ -569,6 +569,20 @@ extern "C" int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_co
   });
 }
 
+extern "C" int bmfp_stencil_generate(bmfp_matrix m, int kind, uint64_t seed, bmfp_stream stream)
+{
+  return bmfp::guarded("bmfp_stencil_generate", [&]() -> int {
+    if (!m) return fail(BMFP_ERR_INVALID, "matrix is NULL");
+    if (m->stencil_nx < 1 || m->dtype != BMFP_F64 || m->layout != BMFP_ROW_MAJOR)
+      return fail(BMFP_ERR_INVALID, "needs an F64 row-major matrix made by bmfp_stencil_create");
+    if (kind != 0 && kind != 1) return fail(BMFP_ERR_INVALID, "kind must be 0 or 1");
+    launch_gen_stencil(m->stencil_nx, m->stencil_ny, m->bs, kind, seed, m->row_ptr->as<int>(), m->columns->as<int>(),
+                       m->values.as<double>(), nullptr, static_cast<hipStream_t>(stream));
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  });
+}
+
 extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int kind, uint64_t seed, int dtype,
                                    int layout)
 {
@@ -590,6 +604,8 @@ extern "C" int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int
     m->nnzb = (int) nnzb;
     m->max_row_blocks = std::min(nx, 3) * std::min(ny, 3);
     m->mean_row_blocks = (double) nnzb / (double) nodes;
+    m->stencil_nx = nx;
+    m->stencil_ny = ny;
     BMFP_HIP(hipGetDevice(&m->device));
     std::vector<int> rp;
     stencil_row_ptr(nx, ny, rp);

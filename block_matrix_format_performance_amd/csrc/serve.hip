@@ -4,17 +4,26 @@
 // iterative refinement around FP32 BiCGStab.  Admitted requests advance
 // together through a per-request state machine on the device; the right-hand
 // sides of one request move in lockstep so that every block row of its
-// matrix is read once per SpMV for all of them:
+// matrix is read once per SpMV for all of them.  Each phase is one or two
+// slot kernels (an elementwise pass that stores the SpMV's operand, then the
+// SpMV, which gathers that one vector) and a finalize kernel:
 //
-//   RESID   (slot 0)  x' = x + d, r = b - A64 x', r32 = r              fin0: DONE | INNER_A | RESID
-//   INNER_A (slot 1)  p = ri + beta (p - omega v), v = A32 p            fin1: INNER_B | RESID
-//   INNER_B (slot 2)  s = ri - alpha v, t = A32 s                       fin2: INNER_C
-//   INNER_C (slot 3)  ri = s - omega t, d += alpha p + omega s          fin3: INNER_A | RESID
+//   RESID    slot X  x' = x + d                                          (elementwise)
+//            slot R  r = b - A64 x', r32 = r                             fin0: DONE | INNER_A | RESID
+//   INNER_A  slot P  p = ri + beta (p - omega v)                         (elementwise)
+//            slot A  v = A32 p                                           fin1: INNER_B | RESID
+//   INNER_B  slot S  s = ri - alpha v                                    (elementwise)
+//            slot B  t = A32 s                                           fin2: INNER_C
+//   INNER_C  slot C  ri = s - omega t, d += alpha p + omega s            fin3: INNER_A | RESID
+//
+// Forming p, s or x' at every gathered index instead (two or three vector
+// loads per gathered element) held these SpMVs at ~3.8 TB/s of compulsory
+// traffic; the stored operand costs one more vector pass per phase.
 //
 // The inner solve starts from ri = r32 with the shadow residual r32 itself
 // (it stays constant until the next RESID), p = v = d = 0: iteration 0 reads
-// r32 in place of ri and skips the zero terms, so no initialisation pass is
-// needed.  With x = 0, the first RESID is r = b and reads no matrix.
+// r32 in place of ri and p and skips the zero terms, so no initialisation
+// pass is needed.  With x = 0, the first RESID is r = b and reads no matrix.
 //
 // Per right-hand side k three bit masks select the work: `live` (not yet
 // converged or failed) for RESID, `act` (still inside the current inner
@@ -23,15 +32,15 @@
 // frozen until the others stop too, so its arithmetic is exactly that of
 // solving it alone.
 //
-// One batched iteration is eight kernels (slot0 fin0 slot1 fin1 slot2 fin2
-// slot3 fin3), captured iterations_per_round times in one HIP graph.  Slot
-// kernels give every wavefront its own tiles from a tile table (no workgroup
-// barriers); a tile is `passes` consecutive chunks of one request, a chunk is
-// 4 wavefront passes of 64/bs block rows.  Partial sums are kept per chunk and
-// the fin kernels (one wavefront per request) reduce a request's chunks in
-// chunk order, so the arithmetic of a request depends neither on the rest of
-// the batch nor on the tile schedule: results are bitwise identical alone, in
-// any mix, and under any schedule the autotuner picks.
+// One batched iteration is eleven kernels (X R fin0 P A fin1 S B fin2 C fin3),
+// captured iterations_per_round times in one HIP graph.  Slot kernels give
+// every wavefront its own tiles from a tile table (no workgroup barriers); a
+// tile is `passes` consecutive chunks of one request, a chunk is 4 wavefront
+// passes of 64/bs block rows.  Partial sums are kept per chunk and the fin
+// kernels (one wavefront per request) reduce a request's chunks in chunk
+// order, so the arithmetic of a request depends neither on the rest of the
+// batch nor on the tile schedule: results are bitwise identical alone, in any
+// mix, and under any schedule the autotuner picks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +59,7 @@ namespace bmfp {
 constexpr int SERVE_MAX_RHS = 4;
 constexpr int SERVE_PART = 2 * SERVE_MAX_RHS;  // partial sums per chunk
 constexpr int SERVE_CHUNK_PASSES = 4;          // wavefront passes per chunk
+constexpr int SERVE_F16_MIN_BS = 4;            // smallest block size with FP16 inner matrices
 
 enum : int { REQ_EMPTY = 0, REQ_RESID = 1, REQ_INNER_A = 2, REQ_INNER_B = 3, REQ_INNER_C = 4, REQ_DONE = 5 };
 
@@ -61,7 +71,9 @@ struct ReqDesc {
   const int *rp;
   const int *cols;
   const double *v64;
-  const float *v32;
+  const float *v32;     // FP32 inner matrix, or null
+  const __half *v16;    // FP16 inner matrix (one FP32 scale per block row in sc16), or null
+  const float *sc16;
   const float *dinv;  // Jacobi: 1 / diagonal entry per scalar row (FP32); null without
   const double *b;
   double *x[2];
@@ -93,8 +105,19 @@ __host__ __device__ constexpr int serve_chunk_brows(int bs) { return SERVE_CHUNK
 
 namespace {
 
-// Slot and finalize kernel SLOT serve phase REQ_RESID + SLOT.
-template <int SLOT> __device__ __forceinline__ bool slot_owns(int phase) { return phase == REQ_RESID + SLOT; }
+// Slot kernels, in launch order within a batched iteration, and the phase
+// each one serves; finalize kernel F serves phase REQ_RESID + F.
+enum : int { SLOT_X = 0, SLOT_R = 1, SLOT_P = 2, SLOT_A = 3, SLOT_S = 4, SLOT_B = 5, SLOT_C = 6, SERVE_SLOTS = 7 };
+__host__ __device__ constexpr int slot_phase(int slot)
+{
+  return slot <= SLOT_R ? REQ_RESID : slot <= SLOT_A ? REQ_INNER_A : slot <= SLOT_B ? REQ_INNER_B : REQ_INNER_C;
+}
+// slots that leave per-chunk partial sums for their phase's finalize kernel
+__host__ __device__ constexpr bool slot_reduces(int slot)
+{
+  return slot == SLOT_R || slot == SLOT_A || slot == SLOT_B || slot == SLOT_C;
+}
+template <int F> __device__ __forceinline__ bool fin_owns(int phase) { return phase == REQ_RESID + F; }
 
 using Acc = double[SERVE_MAX_RHS][2];
 
@@ -103,8 +126,9 @@ using Acc = double[SERVE_MAX_RHS][2];
 template <class T> using gptr = __attribute__((address_space(1))) T *;
 template <class T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>) p; }
 
-// Element `off` of an array: the byte offset fits 32 bits (guaranteed at
-// submission), so the access is a uniform base plus a 32-bit lane offset.
+// Element `off` of an array: the byte offset fits 32 bits (submission
+// refuses requests of 2^30 or more blocks or 2^27 or more rows times
+// right-hand sides), so the access is a uniform base plus a 32-bit lane offset.
 template <class T> __device__ __forceinline__ T ld(gptr<const T> base, uint32_t off)
 {
   return *(gptr<const T>) ((gptr<const char>) base + off * (uint32_t) sizeof(T));
@@ -152,22 +176,24 @@ template <class T, int NR> __device__ __forceinline__ void stv(gptr<T> base, uin
 }
 
 // N consecutive values at p (row r of a row-major block), in the widest loads
-// the alignment of a block row allows (up to 16 bytes).
+// the alignment of a block row allows (up to 16 bytes), copied out of
+// 32-bit lanes (so 2-byte elements load as whole words too).
 template <class T, int N> __device__ __forceinline__ void load_row(gptr<const T> p, T (&v)[N])
 {
-  constexpr int ALIGN = lowbit((int) sizeof(T) * N) < 16 ? lowbit((int) sizeof(T) * N) : 16;
-  constexpr int PER = ALIGN / (int) sizeof(T);
-  if constexpr (PER <= 1) {
+  constexpr int BYTES = (int) sizeof(T) * N;
+  constexpr int CH = lowbit(BYTES) < 16 ? lowbit(BYTES) : 16;
+  const gptr<const char> src = (gptr<const char>) p;
+  char *dst = reinterpret_cast<char *>(v);
+  if constexpr (CH >= 4) {
+    typedef unsigned int w_t __attribute__((ext_vector_type(CH / 4)));
+#pragma unroll
+    for (int c = 0; c < BYTES; c += CH) {
+      const w_t t = *(gptr<const w_t>) (src + c);
+      __builtin_memcpy(dst + c, &t, CH);
+    }
+  } else {
 #pragma unroll
     for (int c = 0; c < N; ++c) v[c] = p[c];
-  } else {
-    typedef T vec_t __attribute__((ext_vector_type(PER)));
-#pragma unroll
-    for (int c = 0; c < N; c += PER) {
-      const vec_t t = *(gptr<const vec_t>) (p + c);
-#pragma unroll
-      for (int e = 0; e < PER; ++e) v[c + e] = t[e];
-    }
   }
 }
 
@@ -203,25 +229,27 @@ __device__ __forceinline__ double wave_total(double v)
 // the same trip counts, so every exchange reads an active lane.  Vectors
 // interleave the right-hand sides: element i of k sits at i*NR + k.
 // Without use_matrix the product is 0 (no block is read).
-// gather(off, x[NR]) forms the operand at element offset off; own(off, m)
+// gather(off, x[NR]) loads the operand at element offset off; own(off, m)
 // loads what the epilogue needs of the owned row before the block loop;
 // epi(off, y[NR], m) finishes the row.  Every right-hand side is computed;
 // the epilogues keep the values the state machine must not change.
 // Blocks whose loads are issued together: as many as keep the batch's
-// registers (block row values + G gathered vectors of NR operands) within
-// about 40, up to a whole 9-block stencil row.
-template <class VT, class AT, int BS, int NR, int G> constexpr int spmv_batch()
+// registers (block row values + the NR gathered operands) within about 40,
+// up to a whole 9-block stencil row.
+template <class VT, class AT, int BS, int NR> constexpr int spmv_batch()
 {
-  constexpr int regs = BS * (int) sizeof(VT) / 4 + G * NR * (int) sizeof(AT) / 4;
+  // FP16 rows are sized as FP32 ones: they are converted before use
+  constexpr int regs = BS * ((int) sizeof(VT) < 4 ? 4 : (int) sizeof(VT)) / 4 + NR * (int) sizeof(AT) / 4;
   return 40 / regs > 9 ? 9 : (40 / regs < 1 ? 1 : 40 / regs);
 }
 
-template <class VT, class AT, int BS, int NR, int G, class OWNT, class GATHER, class OWN, class EPI>
+template <class VT, class AT, int BS, int NR, class OWNT, class GATHER, class OWN, class EPI>
 __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals, int br0, bool use_matrix,
-                                           const GATHER &gather, const OWN &own, const EPI &epi)
+                                           const GATHER &gather, const OWN &own, const EPI &epi,
+                                           gptr<const float> scale = nullptr)
 {
   constexpr int RPW = 64 / BS;
-  constexpr int U = spmv_batch<VT, AT, BS, NR, G>();
+  constexpr int U = spmv_batch<VT, AT, BS, NR>();
   const int lane = threadIdx.x & 63;
   const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
   const gptr<const int> rp = glb(D.rp), cols = glb(D.cols);
@@ -245,6 +273,8 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
     row_bounds(ps + 1, next_b, next_b1);
     OWNT mine;
     if (on) own(off, mine);
+    // FP16 values: the block row's scale multiplies the finished sums
+    const AT sc = (scale && on) ? (AT) ld(scale, (uint32_t) br) : (AT) 1;
     AT acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[k] = 0;
@@ -275,6 +305,9 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
 #pragma unroll
         for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[c], group_bcast<BS>(xe[k], c, base_lane), acc[k]);
     }
+    if (scale)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) acc[k] *= sc;
     if (on) epi(off, acc, mine);
   }
 }
@@ -291,172 +324,194 @@ template <int BS, int NR, class F> __device__ __forceinline__ void chunk_rows(co
   }
 }
 
-template <int NR> struct ResidOwn {
-  double xr[NR], b[NR], old[NR];
-};
-
-// x' = x + d (d counted as 0 where dval is clear), r = b - A64 x', r32 = r;
-// partials ||r||^2 (FP64) and ||r32||^2.  With x = 0 (step 0) the matrix is
-// not read: r = b.
+// x' = x + d (d counted as 0 where dval is clear; a finished right-hand side
+// keeps its iterate; x' = 0 at step 0).
 template <int BS, int NR>
-__device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+__device__ __forceinline__ void resid_x_chunk(const ReqDesc &D, const ReqState &S, int br0)
 {
   const int par = S.outer & 1;
   const unsigned live = S.live, dval = S.dval;
   const bool first = S.outer == 0;
-  const gptr<const double> xs = glb((const double *) D.x[par]), b = glb(D.b);
+  const gptr<const double> xs = glb((const double *) D.x[par]);
   const gptr<double> xn = glb(D.x[1 - par]);
   const gptr<const float> d = glb((const float *) D.d);
-  const gptr<float> r32 = glb(D.r32);
-  auto gather = [&](uint32_t o, double (&x)[NR]) {
+  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
+    double xw[NR];
     if (first) {
 #pragma unroll
-      for (int k = 0; k < NR; ++k) x[k] = 0.0;
-      return;
-    }
-    ldv<double, NR>(xs, o, x);
-    if (dval) {
-      float dd[NR];
-      ldv<float, NR>(d, o, dd);
+      for (int k = 0; k < NR; ++k) xw[k] = 0.0;
+    } else {
+      double xr[NR], old[NR];
+      ldv<double, NR>(xs, o, xr);
+      if (dval) {
+        float dd[NR];
+        ldv<float, NR>(d, o, dd);
 #pragma unroll
-      for (int k = 0; k < NR; ++k)
-        if ((dval >> k) & 1u) x[k] += (double) dd[k];
+        for (int k = 0; k < NR; ++k)
+          if ((dval >> k) & 1u) xr[k] += (double) dd[k];
+      }
+      if (live != (1u << NR) - 1u) ldv<double, NR>((gptr<const double>) xn, o, old);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) xw[k] = ((live >> k) & 1u) ? xr[k] : old[k];
     }
-  };
-  auto own = [&](uint32_t o, ResidOwn<NR> &m) {
-    gather(o, m.xr);
-    ldv<double, NR>(b, o, m.b);
-    if (live != (1u << NR) - 1u) ldv<double, NR>((gptr<const double>) xn, o, m.old);
-  };
-  chunk_spmv<double, double, BS, NR, 2, ResidOwn<NR>>(
-      D, glb(D.v64), br0, !first, gather, own, [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
-        double xw[NR];
+    stv<double, NR>(xn, o, xw);
+  });
+}
+
+template <int NR> struct ResidOwn {
+  double b[NR];
+};
+
+// r = b - A64 x', r32 = r; partials ||r||^2 (FP64) and ||r32||^2.  With x = 0
+// (step 0) the matrix is not read: r = b.
+template <int BS, int NR>
+__device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const int par = S.outer & 1;
+  const bool first = S.outer == 0;
+  const gptr<const double> xn = glb((const double *) D.x[1 - par]), b = glb(D.b);
+  const gptr<float> r32 = glb(D.r32);
+  chunk_spmv<double, double, BS, NR, ResidOwn<NR>>(
+      D, glb(D.v64), br0, !first, [&](uint32_t o, double (&x)[NR]) { ldv<double, NR>(xn, o, x); },
+      [&](uint32_t o, ResidOwn<NR> &m) { ldv<double, NR>(b, o, m.b); },
+      [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
         float rw[NR];
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-          // a finished right-hand side keeps its iterate
-          xw[k] = ((live >> k) & 1u) ? m.xr[k] : m.old[k];
           const double ri = m.b[k] - y[k];
           rw[k] = (float) ri;
           acc[k][0] += ri * ri;
           acc[k][1] += (double) rw[k] * (double) rw[k];
         }
-        stv<double, NR>(xn, o, xw);
         stv<float, NR>(r32, o, rw);
       });
 }
 
-template <int NR> struct AOwn {
-  float p[NR], rh[NR];
-};
-
-// p = ri + beta (p_prev - omega v_prev) (p = r32 at iteration 0), v = A32 p;
-// partial (r32, v).
+// p = ri + beta (p - omega v) for iterations >= 1 (iteration 0 uses p = r32,
+// read in place by the other phases).
 template <int BS, int NR>
-__device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+__device__ __forceinline__ void inner_p_chunk(const ReqDesc &D, const ReqState &S, int br0)
 {
-  const int it = S.inner_it;
+  if (S.inner_it == 0) return;
   float beta[NR], omega[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
     beta[k] = (float) S.k[k].beta;
     omega[k] = (float) S.k[k].omega;
   }
-  const gptr<const float> pp = glb((const float *) D.p[(it - 1) & 1]), vp = glb((const float *) D.v[(it - 1) & 1]);
-  const gptr<float> pc = glb(D.p[it & 1]), vc = glb(D.v[it & 1]);
-  const gptr<const float> ri = glb((const float *) D.ri), rh = glb((const float *) D.r32);
-  const gptr<const float> dinv = glb(D.dinv);
-  const bool jac = D.dinv != nullptr;
-  auto form_p = [&](uint32_t o, float (&x)[NR]) {
-    if (it == 0) {
-      ldv<float, NR>(rh, o, x);
-      return;
-    }
-    float rr[NR], p0[NR], v0[NR];
+  const gptr<float> p = glb(D.p[0]);
+  // the previous p: r32 after iteration 0, which stores none
+  const gptr<const float> pp = S.inner_it == 1 ? glb((const float *) D.r32) : (gptr<const float>) p;
+  const gptr<const float> v = glb((const float *) D.v[0]), ri = glb((const float *) D.ri);
+  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
+    float rr[NR], p0[NR], v0[NR], pn[NR];
     ldv<float, NR>(ri, o, rr);
     ldv<float, NR>(pp, o, p0);
-    ldv<float, NR>(vp, o, v0);
+    ldv<float, NR>(v, o, v0);
 #pragma unroll
-    for (int k = 0; k < NR; ++k) x[k] = p_update(rr[k], p0[k], v0[k], beta[k], omega[k]);
-  };
-  // the SpMV operand: p, or D^-1 p with Jacobi (right preconditioning)
+    for (int k = 0; k < NR; ++k) pn[k] = p_update(rr[k], p0[k], v0[k], beta[k], omega[k]);
+    stv<float, NR>(p, o, pn);
+  });
+}
+
+template <int NR> struct RhOwn {
+  float rh[NR];
+};
+
+// v = A32 q, q = p (r32 at iteration 0), or D^-1 p with Jacobi (right
+// preconditioning, scaled at the gathered index); partial (r32, v).
+template <int BS, int NR>
+__device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  const gptr<const float> rh = glb((const float *) D.r32);
+  const gptr<const float> q = S.inner_it == 0 ? rh : glb((const float *) D.p[0]);
+  const gptr<float> v = glb(D.v[0]);
+  const gptr<const float> dinv = glb(D.dinv);
+  const bool jac = D.dinv != nullptr;
   auto gather = [&](uint32_t o, float (&x)[NR]) {
-    form_p(o, x);
+    ldv<float, NR>(q, o, x);
     if (jac) {
       const float m = ld(dinv, o / NR);
 #pragma unroll
       for (int k = 0; k < NR; ++k) x[k] *= m;
     }
   };
-  auto own = [&](uint32_t o, AOwn<NR> &m) {
-    form_p(o, m.p);
-    if (it == 0) {
+  auto own = [&](uint32_t o, RhOwn<NR> &m) { ldv<float, NR>(rh, o, m.rh); };
+  auto epi = [&](uint32_t o, const float (&y)[NR], const RhOwn<NR> &m) {
+    stv<float, NR>(v, o, y);
 #pragma unroll
-      for (int k = 0; k < NR; ++k) m.rh[k] = m.p[k];
-    } else {
-      ldv<float, NR>(rh, o, m.rh);
-    }
+    for (int k = 0; k < NR; ++k) acc[k][0] += (double) m.rh[k] * (double) y[k];
   };
-  chunk_spmv<float, float, BS, NR, 3, AOwn<NR>>(D, glb(D.v32), br0, true, gather, own,
-                                                [&](uint32_t o, const float (&y)[NR], const AOwn<NR> &m) {
-                                                  stv<float, NR>(pc, o, m.p);
-                                                  stv<float, NR>(vc, o, y);
-#pragma unroll
-                                                  for (int k = 0; k < NR; ++k)
-                                                    acc[k][0] += (double) m.rh[k] * (double) y[k];
-                                                });
+  if constexpr (BS >= SERVE_F16_MIN_BS)
+    if (D.v16) {
+      chunk_spmv<__half, float, BS, NR, RhOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
+      return;
+    }
+  chunk_spmv<float, float, BS, NR, RhOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
-template <int NR> struct BOwn {
-  float s[NR];
-};
-
-// s = ri - alpha v (ri = r32 at iteration 0), t = A32 s; partials (t, s), (t, t).
+// s = ri - alpha v (ri = r32 at iteration 0).
 template <int BS, int NR>
-__device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+__device__ __forceinline__ void inner_s_chunk(const ReqDesc &D, const ReqState &S, int br0)
 {
   float alpha[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) alpha[k] = (float) S.k[k].alpha;
-  const int it = S.inner_it;
-  const gptr<const float> vcur = glb((const float *) D.v[it & 1]);
-  const gptr<const float> ri = glb((const float *) (it == 0 ? D.r32 : D.ri));
-  const gptr<float> s_out = glb(D.s), t_out = glb(D.t);
+  const gptr<const float> v = glb((const float *) D.v[0]);
+  const gptr<const float> ri = glb((const float *) (S.inner_it == 0 ? D.r32 : D.ri));
+  const gptr<float> s_out = glb(D.s);
+  chunk_rows<BS, NR>(D, br0, [&](uint32_t o) {
+    float rr[NR], vv[NR], sn[NR];
+    ldv<float, NR>(ri, o, rr);
+    ldv<float, NR>(v, o, vv);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) sn[k] = s_update(rr[k], vv[k], alpha[k]);
+    stv<float, NR>(s_out, o, sn);
+  });
+}
+
+template <int NR> struct SOwn {
+  float s[NR];
+};
+
+// t = A32 z, z = s or D^-1 s with Jacobi; partials (t, s), (t, t).
+template <int BS, int NR>
+__device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
+{
+  (void) S;
+  const gptr<const float> s_in = glb((const float *) D.s);
+  const gptr<float> t_out = glb(D.t);
   const gptr<const float> dinv = glb(D.dinv);
   const bool jac = D.dinv != nullptr;
-  auto form_s = [&](uint32_t o, float (&x)[NR]) {
-    float rr[NR], vv[NR];
-    ldv<float, NR>(ri, o, rr);
-    ldv<float, NR>(vcur, o, vv);
-#pragma unroll
-    for (int k = 0; k < NR; ++k) x[k] = s_update(rr[k], vv[k], alpha[k]);
-  };
-  // the SpMV operand: s, or D^-1 s with Jacobi
   auto gather = [&](uint32_t o, float (&x)[NR]) {
-    form_s(o, x);
+    ldv<float, NR>(s_in, o, x);
     if (jac) {
       const float m = ld(dinv, o / NR);
 #pragma unroll
       for (int k = 0; k < NR; ++k) x[k] *= m;
     }
   };
-  chunk_spmv<float, float, BS, NR, 2, BOwn<NR>>(D, glb(D.v32), br0, true, gather,
-                                                [&](uint32_t o, BOwn<NR> &m) { form_s(o, m.s); },
-                                                [&](uint32_t o, const float (&y)[NR], const BOwn<NR> &m) {
-                                                  stv<float, NR>(s_out, o, m.s);
-                                                  stv<float, NR>(t_out, o, y);
+  auto own = [&](uint32_t o, SOwn<NR> &m) { ldv<float, NR>(s_in, o, m.s); };
+  auto epi = [&](uint32_t o, const float (&y)[NR], const SOwn<NR> &m) {
+    stv<float, NR>(t_out, o, y);
 #pragma unroll
-                                                  for (int k = 0; k < NR; ++k) {
-                                                    acc[k][0] += (double) y[k] * (double) m.s[k];
-                                                    acc[k][1] += (double) y[k] * (double) y[k];
-                                                  }
-                                                });
+    for (int k = 0; k < NR; ++k) {
+      acc[k][0] += (double) y[k] * (double) m.s[k];
+      acc[k][1] += (double) y[k] * (double) y[k];
+    }
+  };
+  if constexpr (BS >= SERVE_F16_MIN_BS)
+    if (D.v16) {
+      chunk_spmv<__half, float, BS, NR, SOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
+      return;
+    }
+  chunk_spmv<float, float, BS, NR, SOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
 // ri = s - omega t, d += alpha p + omega s, or d += D^-1 (alpha p + omega s)
-// with Jacobi (d = 0 before iteration 0; a live right-hand side outside the
-// inner solve gets d = 0 at iteration 0 and keeps d afterwards); partials
-// (r32, ri), (ri, ri).
+// with Jacobi (p = r32 and d = 0 before at iteration 0; a live right-hand
+// side outside the inner solve gets d = 0 at iteration 0 and keeps d
+// afterwards); partials (r32, ri), (ri, ri).
 template <int BS, int NR>
 __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
@@ -468,8 +523,9 @@ __device__ __forceinline__ void inner_c_chunk(const ReqDesc &D, const ReqState &
     alpha[k] = (float) S.k[k].alpha;
     omega[k] = (float) S.k[k].omega;
   }
-  const gptr<const float> pcur = glb((const float *) D.p[it & 1]), s_in = glb((const float *) D.s),
-                          t_in = glb((const float *) D.t), rh = glb((const float *) D.r32);
+  const gptr<const float> rh = glb((const float *) D.r32);
+  const gptr<const float> pcur = it == 0 ? rh : glb((const float *) D.p[0]), s_in = glb((const float *) D.s),
+                          t_in = glb((const float *) D.t);
   const gptr<float> d = glb(D.d), ri = glb(D.ri);
   const gptr<const float> dinv = glb(D.dinv);
   const bool jac = D.dinv != nullptr;
@@ -529,15 +585,22 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
     const int chunk = T.chunk + c;
     const int br0 = chunk * serve_chunk_brows(BS);
     Acc acc = {};
-    if constexpr (SLOT == 0) {
+    if constexpr (SLOT == SLOT_X) {
+      resid_x_chunk<BS, NR>(D, S, br0);
+    } else if constexpr (SLOT == SLOT_R) {
       resid_chunk<BS, NR>(D, S, br0, acc);
-    } else if constexpr (SLOT == 1) {
+    } else if constexpr (SLOT == SLOT_P) {
+      inner_p_chunk<BS, NR>(D, S, br0);
+    } else if constexpr (SLOT == SLOT_A) {
       inner_a_chunk<BS, NR>(D, S, br0, acc);
-    } else if constexpr (SLOT == 2) {
+    } else if constexpr (SLOT == SLOT_S) {
+      inner_s_chunk<BS, NR>(D, S, br0);
+    } else if constexpr (SLOT == SLOT_B) {
       inner_b_chunk<BS, NR>(D, S, br0, acc);
     } else {
       inner_c_chunk<BS, NR>(D, S, br0, acc);
     }
+    if constexpr (!slot_reduces(SLOT)) continue;
     // per-chunk partial sums (0 for absent right-hand sides), lane j < 8 writes value j
     double mine = 0.0;
 #pragma unroll
@@ -567,13 +630,13 @@ __device__ __forceinline__ void slot_tile(const ReqDesc *__restrict__ rd, const 
   const TileDesc T = tiles[ti];
   const ReqState &S = rs[T.req];
   const int phase = S.phase;
-  if (!slot_owns<SLOT>(phase)) return;
+  if (phase != slot_phase(SLOT)) return;
   const ReqDesc &D = rd[T.req];
   BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
 }
 
 template <int SLOT>
-__global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
+__device__ __forceinline__ void slot_body(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
                                               const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
                                               double *__restrict__ part, int xcd_run, int persistent)
 {
@@ -604,29 +667,53 @@ __global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, co
   for (int ti = wave; ti < nt; ti += nw) slot_tile<SLOT>(rd, rs, tiles, part, ti);
 }
 
-// ---- finalize: one wavefront per request, lane k owns right-hand side k --------
+template <int SLOT>
+__global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
+                                              const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
+                                              double *__restrict__ part, int xcd_run, int persistent)
+{
+  slot_body<SLOT>(rd, rs, tiles, ntp, part, xcd_run, persistent);
+}
+
+// ---- finalize: one workgroup per request; after the reduction lane k of the
+// first wavefront owns right-hand side k.  SLOT here is the phase index (0
+// RESID, 1 INNER_A, 2 INNER_B, 3 INNER_C).
 template <int SLOT>
 __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, ReqState *__restrict__ rs,
                                              const int *__restrict__ list, const int *__restrict__ nlp,
                                              const double *__restrict__ part)
 {
-  const int w = (int) ((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  __shared__ double red[4][SERVE_PART];
+  const int w = (int) blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (w >= *nlp) return;
   const int q = list[w];
   ReqState &S = rs[q];
   const int phase = S.phase;
-  if (!slot_owns<SLOT>(phase)) return;
+  if (!fin_owns<SLOT>(phase)) return;
   const ReqDesc &D = rd[q];
+  // the request's chunk partials: thread t adds chunks t, t + 256, ... (8
+  // doubles each, 16-byte loads), then the waves' sums are added in order
   double a[SERVE_PART];
 #pragma unroll
   for (int j = 0; j < SERVE_PART; ++j) a[j] = 0.0;
-  for (int c = lane; c < D.n_chunks; c += 64) {
-    const double *pc = part + (int64_t) (D.chunk0 + c) * SERVE_PART;
+  for (int c = threadIdx.x; c < D.n_chunks; c += 256) {
+    const double2 *pc = reinterpret_cast<const double2 *>(part + (int64_t) (D.chunk0 + c) * SERVE_PART);
 #pragma unroll
-    for (int j = 0; j < SERVE_PART; ++j) a[j] += pc[j];
+    for (int j = 0; j < SERVE_PART / 2; ++j) {
+      const double2 t = pc[j];
+      a[2 * j] += t.x;
+      a[2 * j + 1] += t.y;
+    }
   }
 #pragma unroll
   for (int j = 0; j < SERVE_PART; ++j) a[j] = wave_sum(a[j]);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < SERVE_PART; ++j) red[wv][j] = a[j];
+  __syncthreads();
+  if (wv != 0) return;
+#pragma unroll
+  for (int j = 0; j < SERVE_PART; ++j) a[j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
   double a0 = 0.0, a1 = 0.0;
 #pragma unroll
   for (int k = 0; k < SERVE_MAX_RHS; ++k)
@@ -745,6 +832,26 @@ __global__ __launch_bounds__(256) void k_stencil_rp(int nx, int ny, int *__restr
 __global__ __launch_bounds__(256) void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n)
 {
   for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) b[i] = (float) a[i];
+}
+
+// FP16 inner matrix of a request (row-major blocks): each block row scaled by
+// one FP32 factor, its max |entry| mapped to 1.0 -- the rounding of
+// bmfp_matrix_convert to BMFP_F16.  One wavefront per block row.
+__global__ __launch_bounds__(256) void k_serve_f16(int n_brows, int bs, const int *__restrict__ rp,
+                                                   const double *__restrict__ v64, __half *__restrict__ v16,
+                                                   float *__restrict__ sc)
+{
+  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (br >= n_brows) return;
+  const int64_t bs2 = (int64_t) bs * bs, e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  double m = 0.0;
+  for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(v64[e]));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (m == 0.0) m = 1.0;
+  const double inv = 1.0 / m;
+  if (lane == 0) sc[br] = (float) m;
+  for (int64_t e = e0 + lane; e < e1; e += 64) v16[e] = __float2half_rn((float) (v64[e] * inv));
 }
 
 // Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64

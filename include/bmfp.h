@@ -160,6 +160,11 @@ int bmfp_csr_create(bmfp_matrix *out, int dtype, int n_rows, int n_cols,
  * a host round trip. */
 int bmfp_stencil_create(bmfp_matrix *out, int nx, int ny, int bs, int kind,
                         uint64_t seed, int dtype, int layout);
+/* Regenerates, in place and ordered on `stream`, the column indices and
+ * values of an F64 row-major matrix made by bmfp_stencil_create (same grid
+ * and block size) for another kind / seed: a request's matrix built on the
+ * device without an allocation (no reference counterpart). */
+int bmfp_stencil_generate(bmfp_matrix m, int kind, uint64_t seed, bmfp_stream stream);
 /* Device-side copy of a BCSR matrix into another storage type / layout
  * (indices shared by reference count).  Used for mixed precision. */
 int bmfp_matrix_convert(bmfp_matrix src, int dtype, int layout,
@@ -281,8 +286,8 @@ int bmfp_refine_destroy(bmfp_solver s);
 /* ---- serving: continuous-batching request loop (no reference counterpart) --
  * Requests -- an FP64 BCSR system with 1..4 right-hand sides -- queue on the
  * host and are admitted first-come first-served while their device footprint
- * (FP64 matrix + FP32 copy + 13 vectors per right-hand side) fits the memory
- * budget.  Admitted requests are set up on their own stream while earlier
+ * (FP64 matrix + FP32 copy, or FP16 copy and block-row scales, + 12 vectors
+ * per right-hand side) fits the memory budget.  Admitted requests are set up on their own stream while earlier
  * ones compute, then join the batch: each request is solved by FP64
  * iterative refinement around FP32 BiCGStab, its right-hand sides in lockstep
  * (one read of the matrix per SpMV for all of them), all requests advancing
@@ -306,6 +311,11 @@ typedef struct {
   int chunks_per_tile;        /* tile height in 256-row chunks; 0: tuned per bs        */
   int use_jacobi;             /* 1: point-Jacobi (1/diagonal, right) preconditioning of
                                  the inner solves, as gpu_bicgstab(A, true); 0: none   */
+  int inner_f16_min_bs;       /* requests with bs >= this (4, 6 or 8) keep their inner
+                                 matrix in FP16 with one FP32 scale per block row
+                                 instead of FP32; 0: FP32 for every request           */
+  double inner_tol_f16;       /* inner tolerance of FP16 requests; 0: 1e-3 for bs 8,
+                                 1e-4 below                                            */
 } bmfp_server_config;
 
 typedef struct {

@@ -117,7 +117,8 @@ def test_stencil_create_argument_errors():
 
 @pytest.mark.parametrize("bad", [dict(max_systems=0), dict(iterations_per_round=0), dict(slot_groups=-2),
                                  dict(chunks_per_tile=-2), dict(inner_tol=0.0), dict(max_outer=0),
-                                 dict(max_inner=0), dict(use_jacobi=2)])
+                                 dict(max_inner=0), dict(use_jacobi=2), dict(inner_f16_min_bs=3),
+                                 dict(inner_f16_min_bs=9), dict(inner_tol_f16=-1.0)])
 def test_server_config_validation(bad):
     with pytest.raises(bm.BmfpError) as e:
         bm.Server(**bad)
@@ -125,13 +126,30 @@ def test_server_config_validation(bad):
 
 
 def test_server_request_footprint():
-    # FP64 + FP32 matrix, the FP32 Jacobi inverse diagonal, 3 FP64 and 9 FP32 vectors over all
-    # right-hand sides, 256-B aligned pieces
+    # FP64 matrix, the inner copy (FP32 values, or FP16 values and one FP32
+    # scale per block row, whichever is larger), the FP32 Jacobi inverse
+    # diagonal, 3 FP64 and 9 FP32 vectors over all right-hand sides, 256-B
+    # aligned pieces
     def a256(b):
         return (b + 255) // 256 * 256
     for nx, ny, bs, nrhs in [(16, 16, 2, 1), (100, 37, 6, 3), (512, 512, 8, 4)]:
         nb, nnzb = nx * ny, (3 * nx - 2) * (3 * ny - 2)
         n, nv = nb * bs, nnzb * bs * bs
-        want = (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + a256(4 * nv) + a256(4 * n)
+        inner = max(a256(4 * nv), a256(2 * nv) + a256(4 * nb))
+        want = (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + inner + a256(4 * n)
                 + 3 * a256(8 * n * nrhs) + 9 * a256(4 * n * nrhs))
         assert bm.Server.request_bytes(bm.Server.stencil_request(nx, ny, bs, nrhs=nrhs)) == want
+
+
+def test_server_request_size_limits():
+    # block columns are read at 32-bit byte offsets: 2^30 - 1 blocks at most
+    # (an 11000 x 11000 bs 1 stencil has (3*11000-2)^2 = 1.09e9 > 2^30 blocks)
+    ok_side = 10920
+    assert (3 * ok_side - 2) ** 2 < 2 ** 30 <= (3 * 11000 - 2) ** 2
+    assert bm.Server.request_bytes(bm.Server.stencil_request(ok_side, ok_side, 1)) > 0
+    with pytest.raises(bm.BmfpError) as e:
+        bm.Server.request_bytes(bm.Server.stencil_request(11000, 11000, 1))
+    assert e.value.status == bm.ERR_INVALID
+    # rows x right-hand sides below 2^27 (interleaved FP64 vectors at 32-bit offsets)
+    with pytest.raises(bm.BmfpError):
+        bm.Server.request_bytes(bm.Server.stencil_request(4096, 4096, 2, nrhs=4))

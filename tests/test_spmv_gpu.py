@@ -416,6 +416,19 @@ def test_device_stencil_matches_host_generator(bs, kind, layout, dtype):
     assert np.array_equal(bm.spmv_host(D, x), bm.spmv_host(H, x))
 
 
+
+def test_stencil_regenerate_in_place():
+    # bmfp_stencil_generate rewrites a resident stencil matrix for another
+    # kind / seed: the values equal a fresh bmfp_stencil_create and the host
+    # generator's, bitwise
+    A = bm.Matrix.stencil(13, 9, 4, 0, seed=1)
+    A.regenerate(1, 77)
+    bm.synchronize()
+    m = oracle.gen_stencil9(13, 9, 4, 1, 77)
+    assert np.array_equal(A.download_values(), m.values)
+    with pytest.raises(bm.BmfpError):
+        A.convert(bm.F32).regenerate(0, 1)   # only F64 row-major stencil handles
+
 @pytest.mark.parametrize("bs", [5, 7, 9, 10, 13, 20, 40, 70])
 @pytest.mark.parametrize("dtype", [bm.F64, bm.F32])
 def test_bcsr_jit_kernel(bs, dtype):
