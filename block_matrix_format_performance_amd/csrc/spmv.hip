@@ -88,6 +88,31 @@ void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
       bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
+{
+  constexpr int BPW = 64 / (BS * (BS / VEC));
+  Grid g = make_grid((A.n_rows + BPW - 1) / BPW * 64);
+  k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+}
+
+// Column-major FP64 lane-group shapes (values per lane, blocks in flight per
+// lane), the fastest measured on HBM-resident 9-point stencil matrices
+// (~4.4e8 stored values, interleaved timings, % of 8 TB/s): bs 4 (2, 3)
+// 73-75, bs 6 (2, 9) 69-75, bs 8 (2, 9) 73-75, bs 12 (4, 3) 72-75,
+// bs 16 (4, 3) 75, bs 24 (12, 1) 73-77.  At bs 32 every shape stays near 50
+// (64 lanes hold one block row, so a wavefront has one x segment per block),
+// and the per-row lane-slice kernel is kept.
+template <int BS>
+constexpr DenseShape dgc_shape()
+{
+  if constexpr (BS == 4) return {2, 3};
+  else if constexpr (BS == 6 || BS == 8) return {2, 9};
+  else if constexpr (BS == 12 || BS == 16) return {4, 3};
+  else if constexpr (BS == 24) return {12, 1};
+  else return {0, 0};
+}
+
 template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
 int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)
 {
@@ -110,6 +135,9 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
       constexpr int RPW = 64 / (BS / D.vec);
       Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
       k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    } else if constexpr (LAYOUT == BMFP_COL_MAJOR && sizeof(VT) == 8 && dgc_shape<BS>().vec > 0) {
+      constexpr DenseShape G = dgc_shape<BS>();
+      launch_dgc<VT, AT, BS, G.vec, G.u, SCALED>(A, x, y, s);
     } else {
       constexpr int VEC = gpr_vec<VT>(BS);
       constexpr int G = pow2ceil(BS / VEC);
@@ -222,15 +250,18 @@ int auto_variant(const bmfp_matrix_t &A)
   // matrices (FP64, ~4.4e8 stored values): for bs 1-3 the wavefront's blocks
   // staged in LDS and split over each block row's lanes (bs 1: FP64 and FP32,
   // while rows average at most 9 blocks, the staging capacity; otherwise
-  // batched thread per row); 16-byte lane slices (row-major) or
-  // shuffle-shared x segments (column-major) from bs = 4 up.
+  // batched thread per row); 16-byte lane slices (row-major) from bs = 4 up;
+  // column-major: the column lane groups (FP64, bs 4-24, 69-77% of 8 TB/s
+  // against 64-73% for shuffle-shared x segments), shuffle-shared x segments
+  // otherwise.
   // block sizes without a precompiled instantiation: a kernel specialised by
   // hipRTC (F64 / F32), the runtime-bs thread-per-row kernel for F16
   if (!precompiled_bs(A.bs)) return A.dtype == BMFP_F16 ? BMFP_BCSR_THREAD_PER_ROW : BMFP_BCSR_JIT;
   if (A.bs == 1)  // staged for FP64 and FP32 (whole 4-byte words)
     return A.dtype != BMFP_F16 && A.mean_row_blocks <= 9.0 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_THREAD_PER_ROW;
   if (A.bs <= 3) return BMFP_BCSR_GROUP_PER_ROW;
-  return A.layout == BMFP_COL_MAJOR ? BMFP_BCSR_SHUFFLE_X : BMFP_BCSR_GROUP_PER_ROW;
+  if (A.layout == BMFP_ROW_MAJOR) return BMFP_BCSR_GROUP_PER_ROW;
+  return A.dtype == BMFP_F64 && A.bs <= 24 ? BMFP_BCSR_GROUP_PER_ROW : BMFP_BCSR_SHUFFLE_X;
 }
 
 int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s)

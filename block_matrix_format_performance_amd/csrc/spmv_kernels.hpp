@@ -350,6 +350,80 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 }
 
 // ============================================================================
+// Dense lane group per block row for column-major blocks, the counterpart of
+// k_bcsr_dgr: column c of a block is BS contiguous values, and lane l = c*H +
+// h of a block row's group of L = BS*H lanes (H = BS/VEC) owns its rows
+// [h*VEC, h*VEC + VEC), so the group reads every block as one contiguous range
+// and each lane needs a single x element per block.  Loads of U blocks are
+// issued together; each lane keeps VEC row partials over the block row; the
+// partials of a row set are summed over the BS columns by a fixed shuffle tree
+// (stride H), and lane h (c = 0) stores its VEC rows.  64/L block rows per
+// wavefront.
+// ============================================================================
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+__global__ __launch_bounds__(256) void k_bcsr_dgc(BcsrArgs<VT, AT> a)
+{
+  constexpr int H = BS / VEC, L = BS * H, BPW = 64 / L;
+  static_assert(BS % VEC == 0 && L <= 64, "bad group shape");
+  constexpr int ALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int q = lane / L, l = lane - q * L;
+  const int c = l / H, h = l - c * H;
+  const int64_t br = wave * BPW + q;
+  const bool on = lane < BPW * L && br < a.n_brows;
+  int b = 0, b1 = 0;
+  if (on) {
+    b = a.rp[br];
+    b1 = a.rp[br + 1];
+  }
+  const int off = c * BS + h * VEC;  // the lane's first element within a block
+  AT acc[VEC];
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) acc[i] = 0;
+  for (; b + U <= b1; b += U) {
+    int col[U];
+    VT v[U][VEC];
+    AT xv[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) col[k] = a.cols[b + k];
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN>(a.vals + (int64_t) (b + k) * (BS * BS) + off, v[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) xv[k] = a.x[(int64_t) col[k] * BS + c];
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) acc[i] = fmadd(cvt<AT>(v[k][i]), xv[k], acc[i]);
+  }
+  for (; U > 1 && b < b1; ++b) {
+    VT v[VEC];
+    load_n<VT, VEC, ALIGN>(a.vals + (int64_t) b * (BS * BS) + off, v);
+    const AT xe = a.x[(int64_t) a.cols[b] * BS + c];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc[i] = fmadd(cvt<AT>(v[i]), xe, acc[i]);
+  }
+  // column c += column c + s, s = 1, 2, 4, ... (fixed order; lanes of other
+  // groups and spare lanes read only themselves)
+#pragma unroll
+  for (int s2 = 1; s2 < BS; s2 *= 2) {
+    const bool take = c % (2 * s2) == 0 && c + s2 < BS;
+    const int src = (c + s2 < BS && lane + s2 * H < 64) ? lane + s2 * H : lane;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      const AT o = __shfl(acc[i], src);
+      if (take) acc[i] += o;
+    }
+  }
+  if (on && c == 0) {
+    AT sc = 1;
+    if constexpr (SCALED) sc = (AT) a.scale[br];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) a.y[br * BS + h * VEC + i] = SCALED ? acc[i] * sc : acc[i];
+  }
+}
+
+// ============================================================================
 // Lane group per block row with the blocks split across the group (bs 2, 3):
 // the BS lanes of a block row each take whole blocks b0+h, b0+h+BS, ... in
 // predicated batches of P (all index, value and x loads of a batch issued

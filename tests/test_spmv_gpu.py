@@ -307,14 +307,20 @@ def test_csr_merge_irregular_rows(name, dtype):
 
 
 def test_bcsr_auto_choice_small_blocks():
-    # bs 1-3: LDS-staged lane groups; bs 1 only while rows average <= 9 blocks
+    # bs 1-3: LDS-staged lane groups; bs 1 only while rows average <= 9 blocks;
+    # FP64 column-major bs 4-24: column lane groups, bs 32: shuffle-shared x
     for bs, layout, want in [(1, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW), (2, bm.COL_MAJOR, bm.BCSR_GROUP_PER_ROW),
                              (3, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW), (4, bm.ROW_MAJOR, bm.BCSR_GROUP_PER_ROW),
-                             (4, bm.COL_MAJOR, bm.BCSR_SHUFFLE_X)]:
+                             (4, bm.COL_MAJOR, bm.BCSR_GROUP_PER_ROW), (32, bm.COL_MAJOR, bm.BCSR_SHUFFLE_X)]:
         m, x = stencil_case(9, 7, bs, kind=0, seed=bs)
         vals = m.values if layout == bm.ROW_MAJOR else oracle.transpose_blocks(m)
         A = bm.Matrix.bcsr(m.row_ptr, m.columns, vals, m.n_rows, m.n_cols, bs, layout=layout)
         assert A.auto_variant() == want, (bs, layout)
+    # column lane groups are FP64 only: FP32 column-major shares x by shuffles
+    m, x = stencil_case(9, 7, 4, kind=0, seed=4)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, oracle.transpose_blocks(m), m.n_rows, m.n_cols, 4,
+                       dtype=bm.F32, layout=bm.COL_MAJOR)
+    assert A.auto_variant() == bm.BCSR_SHUFFLE_X
     m, x = stencil_case(9, 7, 1, kind=0, seed=1)
     A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 1, dtype=bm.F32)
     assert A.auto_variant() == bm.BCSR_GROUP_PER_ROW       # 4-byte blocks are staged too
