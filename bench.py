@@ -13,15 +13,21 @@ HIP events on the work stream, inputs already resident in HBM.
 
 Also reported: per-(bs, layout, family) GB/s (with the CSR expansion of each
 matrix under the three CSR families), the dominant kernel's roofline
-(HIP-event launch time; compulsory bytes per launch), and the CPU baseline:
-the oracle's single-threaded CSR SpMV (main.cpp:36-45 restated) on a bounded
-sample, timed on this host.
+(HIP-event launch time; compulsory bytes per launch; HBM traffic per launch
+read from the counter profile of that kernel's symbol under profiles/, null
+when the profile has no row for it), the solver and serving legs, and the
+CPU baseline: the oracle's single-threaded CSR SpMV (main.cpp:36-45
+restated) on every sweep matrix, CSR / BCSR SpMV and BiCGStab on the 256x256
+bs=2 system, and the FP64 BiCGStab on the 2048x1024 bs=4 system, run in a
+child process pinned to one core that never opens the GPU.
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -29,21 +35,139 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
+BS_SWEEP = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+CPU_CHILD_FLAG = "--cpu-child"
+
+
+def grid_side(bs, total_values):
+    return int(round(math.sqrt(total_values / (9.0 * bs * bs))))
+
+
+# ---- CPU baseline: a child process on one pinned core, never opening the GPU --
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_child(spec):
+    """The oracle's CPU path (test infrastructure, timed here only as the
+    baseline), single-threaded, pinned to spec["core"]:
+    - 256x256 bs=2 stencil: CSR SpMV (main.cpp:36-45 order), BCSR SpMV, mean
+      of 10 runs each (x uniform[0,1], the GPU's x), and the FP64 BiCGStab of
+      b = A*1 to 1e-8 (bicgstab.cu:388-513 restated), one full solve;
+    - 2048x1024 bs=4 convective system, b = A x*: the FP64 BiCGStab to 1e-10,
+      one full solve; its solution goes to spec["x_path"];
+    - every sweep matrix: CSR SpMV of its CSR expansion, mean of 3-10 runs
+      (as many as spec["seconds"] / #bs allows)."""
+    os.sched_setaffinity(0, {spec["core"]})
+    import oracle
+    L = oracle.lib()
+
+    def clog(msg):
+        print(f"[bench cpu] {msg}", file=sys.stderr, flush=True)
+
+    def mean_time(fn, reps):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    def csr_bytes(c):
+        return 12.0 * c.nnz + 4.0 * (c.n_rows + 1) + 16.0 * c.n_rows
+
+    out = {"cpu_model": cpu_model(), "core": spec["core"]}
+    # -- 256x256 bs=2 --------------------------------------------------------
+    m = oracle.gen_stencil9(256, 256, 2, 0, 0)
+    c = oracle.bcsr_to_csr(m)
+    x = oracle.rng_unit_vec(0, 4, c.n_cols)
+    y = np.empty(c.n_rows)
+    L.orc_csr_spmv(c.n_rows, c.row_ptr, c.columns, c.values, x, y)
+    t_csr = mean_time(lambda: L.orc_csr_spmv(c.n_rows, c.row_ptr, c.columns, c.values, x, y), 10)
+    L.orc_bcsr_spmv(m.n_rows, m.bs, 0, m.row_ptr, m.columns, m.values, x, y)
+    t_bcsr = mean_time(lambda: L.orc_bcsr_spmv(m.n_rows, m.bs, 0, m.row_ptr, m.columns, m.values, x, y), 10)
+    bcsr_bytes = 8.0 * m.nnzb * m.bs ** 2 + 4.0 * m.nnzb + 4.0 * (m.n_rows + 1) + 16.0 * c.n_rows
+    b = oracle.csr_spmv(c, np.ones(c.n_rows))
+    t0 = time.perf_counter()
+    xo, info = oracle.bicgstab(c, b, 1e-8, 10000)
+    t_solve = time.perf_counter() - t0
+    out["system_256x256_bs2"] = {
+        "csr_spmv_us": round(t_csr * 1e6, 2), "csr_GBs": round(csr_bytes(c) / t_csr / 1e9, 2),
+        "csr_GFLOPs": round(2.0 * c.nnz / t_csr / 1e9, 3),
+        "bcsr_spmv_us": round(t_bcsr * 1e6, 2), "bcsr_GBs": round(bcsr_bytes / t_bcsr / 1e9, 2),
+        "bicgstab_ms": round(t_solve * 1e3, 2), "iterations": info.iterations,
+        "max_abs_x_minus_1": float(np.max(np.abs(xo - 1.0)))}
+    clog(f"256x256 bs=2: CSR {t_csr*1e6:.0f} us, BCSR {t_bcsr*1e6:.0f} us, BiCGStab {t_solve*1e3:.0f} ms "
+         f"({info.iterations} it)")
+    del m, c, x, y, b, xo
+    # -- 2048x1024 bs=4 FP64 BiCGStab ------------------------------------------
+    m = oracle.gen_stencil9(2048, 1024, 4, 1, 0)
+    c = oracle.bcsr_to_csr(m)
+    del m
+    xs = oracle.rng_sym_vec(1, 5, c.n_rows)
+    b = oracle.csr_spmv(c, xs)
+    t0 = time.perf_counter()
+    xo, info = oracle.bicgstab(c, b, 1e-10, spec["max_iter"])
+    t_solve = time.perf_counter() - t0
+    np.save(spec["x_path"], xo)
+    out["system_2048x1024_bs4"] = {
+        "bicgstab_seconds": round(t_solve, 3), "iterations": info.iterations, "status": info.status,
+        "rel_residual": info.rel_residual,
+        "err_vs_xstar": float(np.linalg.norm(xo - xs) / np.linalg.norm(xs))}
+    clog(f"2048x1024 bs=4 FP64 BiCGStab: {info.iterations} it in {t_solve:.2f} s")
+    del c, xs, b, xo
+    # -- the sweep matrices, CSR ---------------------------------------------
+    per_bs = {}
+    budget = spec["seconds"] / max(len(spec["bs"]), 1)
+    for bs in spec["bs"]:
+        side = grid_side(bs, spec["total_values"])
+        m = oracle.gen_stencil9(side, side, bs, 0, 0)
+        c = oracle.bcsr_to_csr(m)
+        del m
+        x = oracle.rng_unit_vec(0, 4, c.n_cols)
+        y = np.empty(c.n_rows)
+        t1 = mean_time(lambda: L.orc_csr_spmv(c.n_rows, c.row_ptr, c.columns, c.values, x, y), 1)
+        reps = int(min(10, max(3, budget // max(t1, 1e-9))))
+        t = mean_time(lambda: L.orc_csr_spmv(c.n_rows, c.row_ptr, c.columns, c.values, x, y), reps)
+        per_bs[str(bs)] = {"us": round(t * 1e6, 1), "reps": reps, "csr_GBs": round(csr_bytes(c) / t / 1e9, 2),
+                           "GFLOPs": round(2.0 * c.nnz / t / 1e9, 3), "nnz": c.nnz}
+        clog(f"sweep bs={bs}: CSR SpMV {t*1e3:.1f} ms ({per_bs[str(bs)]['csr_GBs']} GB/s, mean of {reps})")
+        del c, x, y
+    out["sweep_csr"] = per_bs
+    return out
+
+
+if __name__ == "__main__" and len(sys.argv) > 2 and sys.argv[1] == CPU_CHILD_FLAG:
+    print(json.dumps(cpu_child(json.loads(sys.argv[2]))), flush=True)
+    sys.exit(0)
+
+
+def start_cpu_child(bs_list, total_values, seconds, max_iter):
+    """Starts the CPU leg on the last core this process may use and keeps
+    this process off that core.  Called before anything opens the GPU."""
+    avail = sorted(os.sched_getaffinity(0))
+    core = avail[-1]
+    if len(avail) > 1:
+        os.sched_setaffinity(0, set(avail[:-1]))
+    spec = {"core": core, "bs": bs_list, "total_values": total_values, "seconds": seconds, "max_iter": max_iter,
+            "x_path": os.path.join(tempfile.gettempdir(), f"bench_cpu_x_{os.getpid()}.npy")}
+    proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), CPU_CHILD_FLAG, json.dumps(spec)],
+                            stdout=subprocess.PIPE, text=True)
+    return proc, spec, len(avail)
+
+
 import block_matrix_format_performance_amd as bm  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled 2.0 / 1.0 by a 1.024 GB copy-kernel calibration) on the HBM-resident
-# sweep matrices; profiles/spmv_hbm_traffic_pmc.md gives the runs and commands.
-PMC_TRAFFIC_BYTES = {
-    ("hbm", 1, "thread_per_row"): 7.263e9,
-    ("hbm", 1, "group_per_row"): 7.259e9,
-    ("hbm", 2, "group_per_row"): 5.009e9,
-    ("hbm", 3, "group_per_row"): 4.197e9,
-    ("hbm", 4, "group_per_row"): 4.134e9,
-    ("hbm", 32, "group_per_row"): 3.639e9,
-}
-BS_SWEEP = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+# HBM bytes per dispatch of the SpMV kernels (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, scaled by a copy-kernel calibration), one table row per mangled
+# kernel symbol and sizing
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "spmv_hbm_traffic_pmc.md")
 FAMILY_NAMES = {
     bm.BCSR_THREAD_PER_ROW: "thread_per_row",
     bm.BCSR_GROUP_PER_ROW: "group_per_row",
@@ -57,8 +181,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def grid_side(bs, total_values):
-    return int(round(math.sqrt(total_values / (9.0 * bs * bs))))
+def profiled_traffic(symbol, sizing):
+    """HBM bytes per dispatch of `symbol` on the `sizing` sweep matrix from
+    the counter profile's table (rows `| symbol | sizing | bs | ... |
+    traffic bytes | ...`), or None when the profile has no such row."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            lines = f.readlines()
+    except OSError:
+        return None
+    header = None
+    for line in lines:
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if line.lstrip().startswith("| symbol"):
+            header = cells
+            continue
+        if header and len(cells) == len(header) and cells[0] == symbol and cells[1] == sizing:
+            return float(cells[header.index("traffic bytes")])
+    return None
 
 
 def build_case(bs, side, layouts):
@@ -96,48 +236,94 @@ def measure_copy(nbytes, reps, stream):
             "read_only": {"bytes_read": nbytes, "us": round(tr * 1e6, 2), "GB/s": round(nbytes / tr / 1e9, 1)}}
 
 
-def cpu_baseline(seconds):
-    """Oracle single-thread CSR SpMV on the bs=4 matrix of the literal sizing."""
-    import oracle
-    bs = 4
-    side = grid_side(bs, 4.4e7)
-    m = oracle.gen_stencil9(side, side, bs, 0, 0)
-    c = oracle.bcsr_to_csr(m)
-    x = oracle.rng_unit_vec(0, 4, c.n_cols) if c.n_cols < 2_000_000 else np.random.default_rng(0).uniform(0, 1, c.n_cols)
-    nbytes = 12.0 * c.nnz + 4.0 * (c.n_rows + 1) + 16.0 * c.n_rows
-    oracle.csr_spmv(c, x)  # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        oracle.csr_spmv(c, x)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    per = el / reps
-    return {"value": round(nbytes / per / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle CSR SpMV (main.cpp:36-45 order), bs=4 stencil {side}x{side} "
-                      f"(nnz={c.nnz}), {reps} reps in {el:.1f} s, {per*1e3:.2f} ms each",
-            "seconds_per_spmv": per}
+def cpu_baseline_result(proc, spec, host_cpus, step_bytes_by_bs, gpu_value, timeout):
+    """Collects the CPU leg and states it in the sweep's unit: the sweep
+    step's compulsory BCSR bytes over the summed CPU CSR SpMV times (the
+    same numerator as the GPU value)."""
+    try:
+        text, _ = proc.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        proc.communicate()
+        log("[bench] CPU leg did not finish in time")
+        return None, None
+    if proc.returncode != 0 or not text.strip():
+        log(f"[bench] CPU leg failed (exit {proc.returncode})")
+        return None, None
+    res = json.loads(text.strip().splitlines()[-1])
+    x_cpu = None
+    try:
+        x_cpu = np.load(spec["x_path"])
+        os.remove(spec["x_path"])
+    except OSError:
+        pass
+    sweep = res["sweep_csr"]
+    common = [bs for bs in step_bytes_by_bs if str(bs) in sweep]
+    t_cpu = sum(sweep[str(bs)]["us"] for bs in common) * 1e-6
+    value = sum(step_bytes_by_bs[bs] for bs in common) / t_cpu / 1e9 if t_cpu > 0 else None
+    out = {"value": round(value, 3) if value else None, "unit": "GB/s", "cores": 1, "kind": "port",
+           "sample": "oracle CSR SpMV (main.cpp:36-45 order) of the CSR expansion of every sweep matrix at the "
+                     f"{'hbm' if spec['total_values'] > 1e8 else 'literal'} sizing, mean of 3-10 runs each; "
+                     "value = the sweep step's compulsory BCSR bytes / summed CPU times; one pinned core "
+                     "(child process without the GPU)",
+           "cpu_model": res["cpu_model"], "host_cpus": host_cpus,
+           "gpu_over_cpu": round(gpu_value / value, 1) if value else None,
+           "per_bs_csr": sweep,
+           "system_256x256_bs2": res["system_256x256_bs2"],
+           "system_2048x1024_bs4": res["system_2048x1024_bs4"]}
+    return out, x_cpu
 
 
 def run_only(args, total_values, stream):
-    bs_s, lay_s, fam_s = args.only.split(":")
-    bs = int(bs_s)
-    layout = bm.ROW_MAJOR if lay_s == "row" else bm.COL_MAJOR
-    fam = {v: k for k, v in FAMILY_NAMES.items()}[fam_s]
-    side = grid_side(bs, total_values)
-    mats = build_case(bs, side, (layout,))
-    A = mats[layout]
-    x = bm.DeviceArray(A.x_len)
-    x.fill(0, seed=0, stream_id=4)
-    y = bm.DeviceArray(A.y_len)
-    nbytes, _ = A.traffic()
-    t = A.spmv_timed(x, y, fam, warmup=args.warmup, reps=args.steps, stream=stream.handle)
-    res = {"kernel": args.only, "side": side, "launches": args.warmup + args.steps, "us": round(t * 1e6, 2),
-           "algorithmic_bytes": nbytes, "GB/s": round(nbytes / t / 1e9, 1)}
+    """--only BS:LAYOUT:FAMILY[,BS:LAYOUT:FAMILY...]: each kernel on its sweep
+    matrix (built, timed --steps times, freed in turn), one JSON line each
+    with the kernel's symbol; then the copy yardstick."""
+    for item in args.only.split(","):
+        bs_s, lay_s, fam_s = item.split(":")
+        bs = int(bs_s)
+        layout = bm.ROW_MAJOR if lay_s == "row" else bm.COL_MAJOR
+        fam = {v: k for k, v in FAMILY_NAMES.items()}[fam_s]
+        side = grid_side(bs, total_values)
+        A = build_case(bs, side, (layout,))[layout]
+        x = bm.DeviceArray(A.x_len)
+        x.fill(0, seed=0, stream_id=4)
+        y = bm.DeviceArray(A.y_len)
+        nbytes, _ = A.traffic()
+        t = A.spmv_timed(x, y, fam, warmup=args.warmup, reps=args.steps, stream=stream.handle)
+        sym = A.kernel_symbol(fam) if fam != bm.BCSR_LIBRARY else None
+        print(json.dumps({"kernel": item, "symbol": sym, "side": side, "launches": args.warmup + args.steps,
+                          "us": round(t * 1e6, 2), "algorithmic_bytes": nbytes,
+                          "GB/s": round(nbytes / t / 1e9, 1)}), flush=True)
+        A.destroy()
+        x.free()
+        y.free()
     if args.copy_mb > 0:
-        res["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 5, stream)
-    print(json.dumps(res), flush=True)
+        print(json.dumps({"copy_yardstick": measure_copy(args.copy_mb * 1e6, 5, stream)}), flush=True)
+
+
+def config0_gpu(stream, reps):
+    """The 256x256 bs=2 stencil on the GPU: SpMV with the AUTO family (x
+    uniform[0,1]) and the FP64 BiCGStab of b = A*1 to 1e-8, x0 = 0."""
+    A = bm.Matrix.stencil(256, 256, 2, 0, seed=0)
+    n = A.y_len
+    x = bm.DeviceArray(n)
+    x.fill(0, seed=0, stream_id=4)
+    y = bm.DeviceArray(n)
+    nbytes, flops = A.traffic()
+    t = A.spmv_timed(x, y, bm.SPMV_AUTO, warmup=5, reps=reps, stream=stream.handle)
+    ones = bm.DeviceArray(n)
+    ones.fill(2, value=1.0)
+    b = bm.DeviceArray(n)
+    A.spmv(ones, b)
+    S = bm.BiCGStab(A)
+    S.solve(b, x, 1e-8, 2)                           # builds the iteration graph
+    info = S.solve(b, x, 1e-8, 10000)
+    err = float(np.max(np.abs(x.download() - 1.0)))
+    S.destroy()
+    A.destroy()
+    return {"spmv_us": round(t * 1e6, 2), "spmv_GBs": round(nbytes / t / 1e9, 1),
+            "spmv_GFLOPs": round(flops / t / 1e9, 2), "bicgstab_ms": round(info.seconds * 1e3, 3),
+            "iterations": info.iterations, "max_abs_x_minus_1": err}
 
 
 def solver_comparison(stream, max_iter):
@@ -173,7 +359,16 @@ def solver_comparison(stream, max_iter):
     log(f"[bench] FP64 BiCGStab: {info.iterations} it, {info.seconds*1e3:.1f} ms, "
         f"res {info.true_rel_residual:.2e}")
     S.destroy()
+    sols = {"fp64": h64}
     for name, inner, tol in (("mixed_fp32", bm.F32, 1e-5), ("mixed_fp16", bm.F16, 1e-3)):
+        # the one-time FP64 -> FP32 / FP16+scale copy the refinement makes of
+        # A, timed on its own (wall clock, allocation included)
+        bm.synchronize()
+        t0 = time.perf_counter()
+        C = A.convert(inner)
+        bm.synchronize()
+        t_copy = time.perf_counter() - t0
+        C.destroy()
         R = bm.Refine(A, inner)
         x = bm.DeviceArray(n * bs)
         R.solve(b, x, 1e-10, tol, 1, 2)             # builds the inner graphs
@@ -183,12 +378,17 @@ def solver_comparison(stream, max_iter):
                      "inner_iterations": ri.inner_iterations, "status": ri.status,
                      "escalations": ri.escalations, "rel_residual": ri.true_rel_residual,
                      "err_vs_fp64_solution": err(hx, h64), "err_vs_xstar": err(hx, h_xs),
-                     "time_vs_fp64": round(ri.seconds / info.seconds, 4)}
+                     "time_vs_fp64": round(ri.seconds / info.seconds, 4),
+                     "copy_seconds": round(t_copy, 5),
+                     "time_vs_fp64_with_copy": round((ri.seconds + t_copy) / info.seconds, 4),
+                     "inner_dtype": {bm.F32: "f32", bm.F16: "f16"}.get(ri.inner_dtype, ri.inner_dtype)}
+        sols[name] = hx
         log(f"[bench] {name}: {ri.outer_iterations} outer / {ri.inner_iterations} inner, "
-            f"{ri.seconds*1e3:.1f} ms ({ri.seconds / info.seconds:.3f}x FP64), res {ri.true_rel_residual:.2e}")
+            f"{ri.seconds*1e3:.1f} ms ({ri.seconds / info.seconds:.3f}x FP64; with the {t_copy*1e3:.2f} ms copy "
+            f"{(ri.seconds + t_copy) / info.seconds:.3f}x), res {ri.true_rel_residual:.2e}")
         R.destroy()
     A.destroy()
-    return res
+    return res, sols
 
 
 SERVE_STREAM = 20000
@@ -491,7 +691,8 @@ def main():
     ap.add_argument("--sizing", choices=("hbm", "literal"), default="hbm")
     ap.add_argument("--bs", default=",".join(map(str, BS_SWEEP)))
     ap.add_argument("--tune-reps", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="CPU time budget for the CPU leg's sweep SpMV timings (3-10 runs per matrix)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--families", default="all", help="all | auto")
     ap.add_argument("--only", default=None,
@@ -511,6 +712,7 @@ def main():
     ap.add_argument("--serve-no-jacobi", action="store_true", help="skip the Jacobi-on-both-sides comparison")
     ap.add_argument("--serve-lambdas", default="200,1000,5000")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--solver-only", action="store_true", help="run only the FP64 vs mixed-precision solver leg")
     ap.add_argument("--no-csr-rows", action="store_true",
                     help="skip the CSR rows (scalar / lane group / merge) of each bs in the sweep")
     ap.add_argument("--csr-only", action="store_true",
@@ -526,11 +728,15 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    total_values = 4.4e8 if args.sizing == "hbm" else 4.4e7
+    bs_list = [int(b) for b in args.bs.split(",")]
+    cpu = None
+    if not (args.no_cpu or args.only or args.csr_only or args.serve_only or args.solver_only):
+        cpu = start_cpu_child(bs_list, total_values, args.cpu_seconds, args.solver_max_iter)
+
     bm.init(0)
     dev = bm.device_info()
     log(f"[bench] device {dev['name']} CUs {dev['compute_units']} mem {dev['total_mem']/2**30:.0f} GiB")
-    total_values = 4.4e8 if args.sizing == "hbm" else 4.4e7
-    bs_list = [int(b) for b in args.bs.split(",")]
     stream = bm.Stream()
 
     if args.only:
@@ -538,6 +744,10 @@ def main():
         return
     if args.csr_only:
         print(json.dumps({"csr_shapes": csr_shape_comparison()}), flush=True)
+        return
+    if args.solver_only:
+        res, _ = solver_comparison(stream, args.solver_max_iter)
+        print(json.dumps({"solver": res}), flush=True)
         return
     if args.serve_only:
         print(json.dumps(serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
@@ -624,6 +834,7 @@ def main():
         per_kernel.append((t, c))
     dom_t, dom = max(per_kernel, key=lambda p: p[0])
     achieved = dom["bytes"] / dom_t / 1e9
+    dom_symbol = dom["A"].kernel_symbol(dom["family"])
 
     out = {
         "metric": "BCSR SpMV GB/s (% HBM peak); BiCGStab time to 1e-10 mixed vs FP64; p99 ms",
@@ -652,8 +863,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": PMC_TRAFFIC_BYTES.get((args.sizing, dom["bs"], dom["key"].split("/")[1])),
+            "traffic": profiled_traffic(dom_symbol, args.sizing),
             "kernel": f"bs={dom['bs']} {dom['key']}",
+            "symbol": dom_symbol,
             "launch_us": round(dom_t * 1e6, 2),
             "algorithmic_bytes": dom["bytes"],
         },
@@ -665,24 +877,34 @@ def main():
     }
     if args.copy_mb > 0:
         out["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 50, stream)
+    per_bs_bytes = [{"bs": c["bs"], "bytes": c["bytes"]} for c in cases]
     for c in cases:
         c["A"].destroy()
     cases.clear()
+    out["system_256x256_bs2"] = config0_gpu(stream, args.steps)
+    # the CPU leg runs beside the sweep only: the solver and serving legs make
+    # host round trips whose latency a busy host would lengthen
+    out["cpu_baseline"] = None
+    x_cpu = None
+    if cpu is not None:
+        log("[bench] waiting for the CPU leg ...")
+        t0 = time.perf_counter()
+        proc, spec, host_cpus = cpu
+        out["cpu_baseline"], x_cpu = cpu_baseline_result(proc, spec, host_cpus,
+                                                         {c["bs"]: c["bytes"] for c in per_bs_bytes}, value,
+                                                         timeout=1800)
+        log(f"[bench] CPU leg done ({time.perf_counter() - t0:.1f} s waited)")
     if not args.no_solver:
-        out["solver"] = solver_comparison(stream, args.solver_max_iter)
+        out["solver"], sols = solver_comparison(stream, args.solver_max_iter)
+        if x_cpu is not None:
+            # the GPU solutions against the CPU FP64 BiCGStab's
+            for name, xg in sols.items():
+                out["solver"][name]["err_vs_cpu_fp64"] = float(np.linalg.norm(xg - x_cpu) / np.linalg.norm(x_cpu))
     if args.serve_requests > 0:
         out["serving"] = serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
                                             [float(v) for v in args.serve_lambdas.split(",")],
                                             args.solver_max_iter, stream, plain=not args.serve_no_plain,
                                             jacobi=not args.serve_no_jacobi)
-    if not args.no_cpu:
-        log("[bench] CPU baseline ...")
-        cb = cpu_baseline(args.cpu_seconds)
-        cb.pop("seconds_per_spmv")
-        cb["host_cpus"] = os.cpu_count()
-        out["cpu_baseline"] = cb
-    else:
-        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 

@@ -116,6 +116,7 @@ def _declare(L):
     L.bmfp_spmv.argtypes = [vp, c_int, vp, vp, vp]
     L.bmfp_spmv_timed.argtypes = [vp, c_int, vp, vp, c_int, c_int, vp, dp]
     L.bmfp_spmv_auto_variant.argtypes = [vp, ip]
+    L.bmfp_spmv_kernel_symbol.argtypes = [vp, c_int, ctypes.c_char_p, sz]
     L.bmfp_jit_build.argtypes = [c_int, c_int, c_int, ctypes.POINTER(sz), dp]
     L.bmfp_csr_spmv_noalloc.argtypes = [c_int, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp]
     L.bmfp_stencil_generate.argtypes = [vp, c_int, ctypes.c_uint64, vp]
@@ -429,6 +430,12 @@ class Matrix:
         v = ctypes.c_int(0)
         check(lib().bmfp_spmv_auto_variant(self.handle, ctypes.byref(v)))
         return v.value
+
+    def kernel_symbol(self, variant=SPMV_AUTO):
+        """Mangled symbol of the kernel spmv(x, y, variant) launches."""
+        buf = ctypes.create_string_buffer(1024)
+        check(lib().bmfp_spmv_kernel_symbol(self.handle, variant, buf, len(buf)))
+        return buf.value.decode()
 
     def to_csr(self, dtype=F64, stream=None):
         """CSR expansion in device memory (csr_matrix_class(bcsr) entry order)."""

@@ -47,6 +47,9 @@ int launch_bcsr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hip
 // The hipRTC-specialised kernel for A's block size (F64 / F32).
 int launch_bcsr_jit(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
 int launch_csr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipStream_t s);
+// Host stub of the kernel launch_bcsr / launch_csr would run for `variant`
+// (the tile kernel of the merge path); nothing is launched.
+int spmv_kernel_fn(const bmfp_matrix_t &A, int variant, const void **fn);
 int launch_library(bmfp_matrix_t &A, const void *x, void *y, hipStream_t s);
 void destroy_library(bmfp_matrix_t &A);
 int auto_variant(const bmfp_matrix_t &A);

@@ -29,6 +29,14 @@ __global__ __launch_bounds__(256) void k_refine_first(const double *__restrict__
   write_partials<2>(part, acc, lds);
 }
 
+// Refinement step k >= 1, first half: x_new = x_old + d.
+__global__ __launch_bounds__(256) void k_refine_update(int64_t n, const double *__restrict__ x_old,
+                                                       const float *__restrict__ d, double *__restrict__ x_new)
+{
+  const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x_new[i] = x_old[i] + (double) d[i];
+}
+
 // Outer decision of the refinement loop, taken on the device: reduces the
 // residual kernel's partial sums (fixed order), forms ||r_k|| / ||b|| and
 // decides converged / non-finite / escalate FP16 -> FP32 (the residual
@@ -80,7 +88,9 @@ struct RefineState {
   DeviceBuffer part;                         // residual partial sums
   DeviceBuffer ctl;                          // RefineCtl: the outer decision
   RefineCtl *h_ctl = nullptr;                // pinned mirror
-  int np = 1;
+  int np = 1;                                // workgroups of the step-0 pass
+  int resid_groups = 1;                      // workgroups of the residual SpMV
+  DeviceBuffer partW;                        // their partial sums
   int64_t n = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_sync = nullptr;
@@ -98,11 +108,17 @@ struct RefineState {
 
 namespace {
 
+// The FP64 residual: x_new = x_old + d as a vector pass, then the SpMV of
+// the stored x_new with the split schedule's engine (on the 2048x1024 bs 4
+// system 30 + 500 us against 578 us forming x_old + d at every gathered
+// index), its per-workgroup partials summed by REDUCE_GROUPS workgroups.
 template <int BS, int LAYOUT>
 struct Resid {
-  using Engine = SolverEngine<double, double, BS, LAYOUT, false>;
+  using Engine = SplitEngine<double, double, BS, LAYOUT, false>;
   static int groups(const bmfp_matrix_t &A)
   {
+    if constexpr (one_tile<Engine>::value)
+      return (int) std::max<int64_t>(1, Engine::groups1(A.n_rows));
     const int64_t t = Engine::threads(A.n_rows);
     return (int) std::max<int64_t>(1, std::min<int64_t>((t + 255) / 256, SOLVER_MAX_GROUPS));
   }
@@ -116,8 +132,13 @@ struct Resid {
     M.scale = nullptr;
     const double *xo = R.x.as<double>() + (int64_t) xin * R.n;
     double *xn = R.x.as<double>() + (int64_t) (1 - xin) * R.n;
-    k_refine_residual<Engine><<<R.np, 256, 0, s>>>(M, b, xo, R.d32.as<float>(), xn, R.r32.as<float>(),
-                                                                 R.part.as<double>());
+    k_refine_update<<<(unsigned) ((R.n + 255) / 256), 256, 0, s>>>(R.n, xo, R.d32.as<float>(), xn);
+    BMFP_LAUNCHED();
+    k_refine_resid_split<Engine><<<R.resid_groups, 256, 0, s>>>(M, xn, b, R.r32.as<float>(),
+                                                                  R.partW.as<double>());
+    BMFP_LAUNCHED();
+    k_bicg_reduce<2><<<REDUCE_GROUPS, 256, 0, s>>>(R.partW.as<double>(), R.resid_groups, R.part.as<double>(),
+                                                   nullptr);
     BMFP_LAUNCHED();
     return BMFP_OK;
   }
@@ -129,7 +150,7 @@ bool pick(RefineState &R)
 #define BMFP_RESID_BS(B)                         \
   case B:                                        \
     R.resid = Resid<B, LAYOUT>::run;             \
-    R.np = Resid<B, LAYOUT>::groups(*R.A);       \
+    R.resid_groups = Resid<B, LAYOUT>::groups(*R.A); \
     return true;
   switch (R.A->bs) {
     BMFP_RESID_BS(1)
@@ -181,7 +202,9 @@ int refine_create(bmfp_solver_t &H, bmfp_matrix_t *A, int inner_dtype, int use_j
   BMFP_TRY(R->x.alloc(2 * n * sizeof(double)));
   BMFP_TRY(R->r32.alloc(n * sizeof(float)));
   BMFP_TRY(R->d32.alloc(n * sizeof(float)));
+  R->np = (int) std::max<int64_t>(1, std::min<int64_t>((R->n + 255) / 256, SOLVER_MAX_GROUPS));
   BMFP_TRY(R->part.alloc(2 * SOLVER_MAX_GROUPS * sizeof(double)));
+  BMFP_TRY(R->partW.alloc(2 * (size_t) R->resid_groups * sizeof(double)));
   BMFP_TRY(R->ctl.alloc(sizeof(RefineCtl)));
   BMFP_HIP(hipHostMalloc(reinterpret_cast<void **>(&R->h_ctl), sizeof(RefineCtl), hipHostMallocDefault));
   BMFP_HIP(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
@@ -231,8 +254,8 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
       BMFP_TRY(R.resid(R, b, xin, s));
       xin = 1 - xin;
     }
-    k_refine_decide<<<1, 256, 0, s>>>(R.part.as<double>(), R.np, k, R.dtype == BMFP_F16 ? 1 : 0,
-                                      R.ctl.as<RefineCtl>());
+    k_refine_decide<<<1, 256, 0, s>>>(R.part.as<double>(), k == 0 ? R.np : REDUCE_GROUPS, k,
+                                      R.dtype == BMFP_F16 ? 1 : 0, R.ctl.as<RefineCtl>());
     BMFP_LAUNCHED();
     BMFP_HIP(hipMemcpyAsync(R.h_ctl, R.ctl.ptr, sizeof(RefineCtl), hipMemcpyDeviceToHost, s));
     BMFP_HIP(hipStreamSynchronize(s));

@@ -128,6 +128,42 @@ __global__ __launch_bounds__(256) void k_convert(int n_brows, int bs, int src_la
   }
 }
 
+// Same layout, no scale: a flat pass, two values per thread (one 16-byte
+// load); nv values from a 256-byte aligned allocation.
+template <class OT>
+__global__ __launch_bounds__(256) void k_convert_flat(int64_t nv, const double *__restrict__ src,
+                                                      OT *__restrict__ dst)
+{
+  const int64_t i = ((int64_t) blockIdx.x * 256 + threadIdx.x) * 2;
+  if (i + 1 < nv) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 v = *reinterpret_cast<const d2 *>(src + i);
+    dst[i] = (OT) v.x;
+    dst[i + 1] = (OT) v.y;
+  } else if (i < nv) {
+    dst[i] = (OT) src[i];
+  }
+}
+
+// Same layout with the per-block-row FP16 scale: the row's max |entry|, then
+// every element scaled in place order (no index arithmetic).
+__global__ __launch_bounds__(256) void k_convert_scaled(int n_brows, int64_t bs2, const int *__restrict__ rp,
+                                                        const double *__restrict__ src, __half *__restrict__ dst,
+                                                        float *__restrict__ scale)
+{
+  const int64_t br = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (br >= n_brows) return;
+  const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  double m = 0.0;
+  for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(src[e]));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (m == 0.0) m = 1.0;
+  const double inv = 1.0 / m;
+  if (lane == 0) scale[br] = (float) m;
+  for (int64_t e = e0 + lane; e < e1; e += 64) dst[e] = __float2half_rn((float) (src[e] * inv));
+}
+
 }  // namespace bmfp
 
 using namespace bmfp;
@@ -473,6 +509,21 @@ int convert_values(bmfp_matrix_t &m, const double *src, int src_layout, hipStrea
   if (m.n_rows == 0) return BMFP_OK;
   const unsigned groups = (unsigned) (((int64_t) m.n_rows * 64 + 255) / 256);
   const int *rp = m.row_ptr->as<int>();
+  if (src_layout == m.layout && m.dtype != BMFP_F16) {
+    const unsigned flat = (unsigned) std::max<int64_t>((nv + 511) / 512, 1);
+    if (m.dtype == BMFP_F64)
+      k_convert_flat<double><<<flat, 256, 0, s>>>(nv, src, m.values.as<double>());
+    else
+      k_convert_flat<float><<<flat, 256, 0, s>>>(nv, src, m.values.as<float>());
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
+  if (src_layout == m.layout) {
+    k_convert_scaled<<<groups, 256, 0, s>>>(m.n_rows, (int64_t) m.bs * m.bs, rp, src, m.values.as<__half>(),
+                                             m.scale.as<float>());
+    BMFP_LAUNCHED();
+    return BMFP_OK;
+  }
   switch (m.dtype) {
   case BMFP_F64:
     k_convert<double, false><<<groups, 256, 0, s>>>(m.n_rows, m.bs, src_layout, m.layout, rp, src,
@@ -827,6 +878,21 @@ extern "C" int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *
     (void) hipEventDestroy(e1);
     if (st == BMFP_OK) *avg_seconds = ms * 1e-3 / reps;
     return st;
+  });
+}
+
+extern "C" int bmfp_spmv_kernel_symbol(bmfp_matrix A, int variant, char *buf, size_t len)
+{
+  return bmfp::guarded("bmfp_spmv_kernel_symbol", [&]() -> int {
+    if (!A || !buf || len == 0) return fail(BMFP_ERR_INVALID, "NULL argument");
+    const void *fn = nullptr;
+    BMFP_TRY(spmv_kernel_fn(*A, variant, &fn));
+    const char *name = hipKernelNameRefByPtr(fn, nullptr);
+    if (!name) return fail(BMFP_ERR_HIP, "hipKernelNameRefByPtr found no kernel name");
+    const size_t n = std::strlen(name);
+    if (n + 1 > len) return fail(BMFP_ERR_INVALID, "symbol of %zu bytes does not fit a %zu-byte buffer", n, len);
+    std::memcpy(buf, name, n + 1);
+    return BMFP_OK;
   });
 }
 

@@ -189,24 +189,6 @@ struct OperandS {
   }
 };
 
-// Refinement residual operand: xn_i = x_i + d_i (FP64 outer update).
-struct OperandXD {
-  const double *__restrict__ x;
-  const float *__restrict__ d;
-  __device__ __forceinline__ double raw(int64_t i) const { return x[i] + (double) d[i]; }
-  __device__ __forceinline__ double operator()(int64_t i) const { return raw(i); }
-  template <int N, int ALIGN>
-  __device__ __forceinline__ void load(int64_t i, double (&out)[N]) const
-  {
-    double xx[N];
-    float dd[N];
-    load_n<double, N, ALIGN>(x + i, xx);
-    load_n<float, N, cmin(ALIGN / 2 > 0 ? ALIGN / 2 : 1, 16)>(d + i, dd);
-#pragma unroll
-    for (int c = 0; c < N; ++c) out[c] = xx[c] + (double) dd[c];
-  }
-};
-
 // Plain vector operand (split schedule): the SpMV reads x as stored.
 template <class AT>
 struct OperandVec {
@@ -599,12 +581,13 @@ template <class VT, int BS> constexpr int rows_per_thread()
 }
 
 // RowsEngine: one thread per R consecutive scalar rows of a block row (row-
-// major blocks), R = rows_per_thread: its R block rows are one contiguous
-// load (16 bytes for FP16 bs 4 and FP32 bs 2, where a single row is 8), the
-// x segment one more; U blocks' loads issued together; no cross-lane sums.
-template <class VT, class AT, int BS, int U, bool SCALED>
+// major blocks), R = RR or rows_per_thread: its R block rows are one
+// contiguous load (16 bytes for FP16 bs 4 and FP32 bs 2, where a single row
+// is 8), the x segment one more; U blocks' loads issued together; no
+// cross-lane sums.
+template <class VT, class AT, int BS, int U, bool SCALED, int RR = 0>
 struct RowsEngine {
-  static constexpr int R = rows_per_thread<VT, BS>();
+  static constexpr int R = RR > 0 && BS % RR == 0 ? RR : rows_per_thread<VT, BS>();
   static constexpr int TPB = BS / R;  // threads per block row
   static int64_t threads(int64_t n_brows) { return n_brows * TPB; }
   static int64_t groups1(int64_t n_brows) { return (n_brows * TPB + 255) / 256; }
@@ -711,8 +694,11 @@ using SolverEngine = typename std::conditional<
 // per SpMV against 546 / 534 us with the dense lane groups), the dense lane
 // groups of the standalone FP64 kernel for 6+ columns, 16-byte row groups
 // for FP32 / FP16 blocks of 4-8 columns (FP16 bs 4: batches of 9 blocks,
-// 168 / 156 us against 178 / 168 with 3); the fused schedule's engines
-// otherwise.
+// 168 / 156 us against 178 / 168 with 3; FP32 bs 4: two rows, 32 bytes, per
+// thread in batches of 3 blocks, the FP32 refinement of the 2048x1024 bs 4
+// system 19.6 ms against 19.9 with one row in batches of 9, 21.4 with two
+// rows in batches of 9 and 24.2 with four rows); the fused schedule's
+// engines otherwise.
 template <class VT, int BS> constexpr int split_vec() { return dense_shape<VT, BS>().vec > 0 ? dense_shape<VT, BS>().vec : 1; }
 template <class VT, int BS> constexpr int split_u() { return dense_shape<VT, BS>().u > 0 ? dense_shape<VT, BS>().u : 1; }
 template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
@@ -722,7 +708,8 @@ using SplitEngine = typename std::conditional<
         (LAYOUT == BMFP_ROW_MAJOR && sizeof(VT) == 8 && BS >= 6 && dense_shape<VT, BS>().vec > 0),
         DenseEngine<VT, AT, BS, split_vec<VT, BS>(), split_u<VT, BS>(), SCALED>,
         typename std::conditional<(LAYOUT == BMFP_ROW_MAJOR && sizeof(VT) < 8 && BS >= 4 && BS <= 8),
-                                  RowsEngine<VT, AT, BS, (BS <= 4 ? 9 : 3), SCALED>,
+                                  RowsEngine<VT, AT, BS, (sizeof(VT) == 2 && BS <= 4 ? 9 : 3), SCALED,
+                                             (sizeof(VT) == 4 && BS == 4 ? 2 : 0)>,
                                   SolverEngine<VT, AT, BS, LAYOUT, SCALED>>::type>::type>::type;
 
 // Init: x = 0, r = r^ = b, p = v = 0 (both parities); partC = ((b,b), (b,b)).
@@ -1014,7 +1001,8 @@ __global__ __launch_bounds__(256) void k_bicg_sv(MatArgs<VT, AT> M, SolverVecs<A
   if (split_run<ENGINE>(M, op, epi, V.st)) write_partials<1>(V.partW, epi.part, lds);
 }
 
-// Sums the `count` per-workgroup partials of a split SpMV (K values each):
+// Sums the `count` per-workgroup partials of a split SpMV (K values each; st
+// may be null: no stop flag):
 // workgroup g of the REDUCE_GROUPS adds the contiguous slice g of them in a
 // fixed order and writes out[g*K + k]; the next kernel's head adds those
 // REDUCE_GROUPS values in order (V.npa / V.npb).  One workgroup alone took
@@ -1025,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_bicg_reduce(const double *__restrict__ 
                                                      double *__restrict__ out, const SolverState *st)
 {
   __shared__ double lds[2][4];
-  if (st->done) return;  // uniform: written only by earlier kernels
+  if (st && st->done) return;  // uniform: written only by earlier kernels
   const int per = (count + REDUCE_GROUPS - 1) / REDUCE_GROUPS;
   const int g0 = blockIdx.x * per, g1 = g0 + per < count ? g0 + per : count;
   double v[K];
@@ -1115,27 +1103,37 @@ __global__ __launch_bounds__(256) void k_jacobi(MatArgs<VT, AT> M, int bs, AT *_
 
 namespace bmfp {
 
-// Refinement residual: x_new = x_old + d (owned rows), r = b - A x_new in
-// FP64 (A x_new formed with x_old + d gathered on the fly), r32 = (float) r;
-// partial sums of (r, r) and (b, b).
-template <class ENGINE>
-__global__ __launch_bounds__(256) void k_refine_residual(MatArgs<double, double> M, const double *__restrict__ b,
-                                                         const double *__restrict__ x_old,
-                                                         const float *__restrict__ d, double *__restrict__ x_new,
-                                                         float *__restrict__ r32, double *__restrict__ part)
-{
-  __shared__ double lds[2][4];
-  const OperandXD op{x_old, d};
-  double acc[2] = {0.0, 0.0};
-  ENGINE::run(M, op, [&](int64_t row, double y) {
-    x_new[row] = op.raw(row);
-    const double bi = b[row];
+// Refinement residual of a stored iterate, for the split SpMV engines:
+// r = b - A x in FP64, r32 = (float) r and the partial sums ((r, r), (b, b))
+// per workgroup (one-tile engines) or per persistent workgroup.
+struct EpiR {
+  const double *__restrict__ b;
+  float *__restrict__ r32;
+  double part[2] = {0.0, 0.0};
+  __device__ __forceinline__ double pre(int64_t row) const { return b[row]; }
+  __device__ __forceinline__ void operator()(int64_t row, double y, double bi)
+  {
     const double ri = bi - y;
     r32[row] = (float) ri;
-    acc[0] += ri * ri;
-    acc[1] += bi * bi;
-  });
-  write_partials<2>(part, acc, lds);
+    part[0] += ri * ri;
+    part[1] += bi * bi;
+  }
+  __device__ __forceinline__ void operator()(int64_t row, double y) { (*this)(row, y, b[row]); }
+};
+
+template <class ENGINE>
+__global__ __launch_bounds__(256) void k_refine_resid_split(MatArgs<double, double> M, const double *__restrict__ x,
+                                                            const double *__restrict__ b, float *__restrict__ r32,
+                                                            double *__restrict__ part)
+{
+  __shared__ double lds[2][4];
+  const OperandVec<double> op{x};
+  EpiR epi{b, r32};
+  if constexpr (one_tile<ENGINE>::value)
+    ENGINE::run1(M, op, epi);
+  else
+    ENGINE::run(M, op, epi);
+  write_partials<2>(part, epi.part, lds);
 }
 
 }  // namespace bmfp

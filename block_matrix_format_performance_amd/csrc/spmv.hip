@@ -32,6 +32,30 @@ struct Grid {
   int per;
 };
 
+// Kernel-symbol queries (spmv_kernel_fn) run a launcher with tl_probe set: the
+// launch of the path's main kernel records its host stub and nothing runs.
+thread_local bool tl_probe = false;
+thread_local const void *tl_probe_fn = nullptr;
+
+template <class... P, class... A>
+void launch(void (*k)(P...), unsigned groups, unsigned threads, hipStream_t s, A... a)
+{
+  if (tl_probe) {
+    tl_probe_fn = reinterpret_cast<const void *>(k);
+    return;
+  }
+  k<<<groups, threads, 0, s>>>(a...);
+}
+
+// A helper kernel of a multi-kernel path (merge-path bounds and fix-up): a
+// probe skips it without recording it.
+template <class... P, class... A>
+void launch_aux(void (*k)(P...), unsigned groups, unsigned threads, hipStream_t s, A... a)
+{
+  if (tl_probe) return;
+  k<<<groups, threads, 0, s>>>(a...);
+}
+
 Grid make_grid(int64_t threads)
 {
   int64_t g = (threads + 255) / 256;
@@ -84,8 +108,8 @@ void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
   constexpr int RPW = 64 / BS, WPG = stage_wpg<BS>();
   const int64_t waves = std::max<int64_t>((A.n_rows + RPW - 1) / RPW, 1);
   const Grid g = make_grid((waves + WPG - 1) / WPG * 256);  // one group per WPG wavefronts
-  k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG><<<g.groups, 64 * WPG, 0, s>>>(
-      bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+  launch(k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG>, g.groups, 64 * WPG, s,
+         bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
 template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
@@ -93,7 +117,7 @@ void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 {
   constexpr int BPW = 64 / (BS * (BS / VEC));
   Grid g = make_grid((A.n_rows + BPW - 1) / BPW * 64);
-  k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+  launch(k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
 // Column-major FP64 lane-group shapes (values per lane, blocks in flight per
@@ -120,7 +144,7 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
   switch (variant) {
   case BMFP_BCSR_THREAD_PER_ROW: {
     Grid g = make_grid(nb * BS);
-    k_bcsr_tpr<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    launch(k_bcsr_tpr<VT, AT, BS, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     break;
   }
   case BMFP_BCSR_GROUP_PER_ROW: {
@@ -130,11 +154,11 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
     } else if constexpr (BS == 2 || BS == 3) {
       constexpr int RPW = 64 / BS;
       Grid g = make_grid((nb + RPW - 1) / RPW * 64);
-      k_bcsr_split<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      launch(k_bcsr_split<VT, AT, BS, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     } else if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
       constexpr int RPW = 64 / (BS / D.vec);
       Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
-      k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      launch(k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     } else if constexpr (LAYOUT == BMFP_COL_MAJOR && sizeof(VT) == 8 && dgc_shape<BS>().vec > 0) {
       constexpr DenseShape G = dgc_shape<BS>();
       launch_dgc<VT, AT, BS, G.vec, G.u, SCALED>(A, x, y, s);
@@ -142,14 +166,14 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
       constexpr int VEC = gpr_vec<VT>(BS);
       constexpr int G = pow2ceil(BS / VEC);
       Grid g = make_grid(nb * BS * G);
-      k_bcsr_gpr<VT, AT, BS, VEC, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      launch(k_bcsr_gpr<VT, AT, BS, VEC, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     }
     break;
   }
   case BMFP_BCSR_SHUFFLE_X: {
     constexpr int RPW = 64 / BS;
     Grid g = make_grid((nb + RPW - 1) / RPW * 64);
-    k_bcsr_shfx<VT, AT, BS, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+    launch(k_bcsr_shfx<VT, AT, BS, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     break;
   }
   case BMFP_BCSR_BLOCK_ROW_GROUP: {
@@ -158,7 +182,7 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
     else {
       constexpr int GL = bgr_lanes(BS);
       Grid g = make_grid(nb * GL);
-      k_bcsr_bgr<VT, AT, BS, GL, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      launch(k_bcsr_bgr<VT, AT, BS, GL, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     }
     break;
   }
@@ -175,7 +199,7 @@ int launch_runtime_bs(const bmfp_matrix_t &A, int variant, const void *x, void *
   if (variant != BMFP_BCSR_THREAD_PER_ROW)
     return fail(BMFP_ERR_UNSUPPORTED, "block size %d has no precompiled kernel for variant %d", A.bs, variant);
   Grid g = make_grid((int64_t) A.n_rows * A.bs);
-  k_bcsr_tpr_rt<VT, AT, LAYOUT, SCALED><<<g.groups, 256, 0, s>>>(bcsr_args<VT, AT, SCALED>(A, x, y, g.per), A.bs);
+  launch(k_bcsr_tpr_rt<VT, AT, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per), A.bs);
   BMFP_LAUNCHED();
   return BMFP_OK;
 }
@@ -313,7 +337,7 @@ void launch_csr_stage(const bmfp_matrix_t &A, CsrArgs<VT, AT> a, hipStream_t s)
   const int64_t waves = std::max<int64_t>((A.n_rows + RPW - 1) / RPW, 1);
   const Grid g = make_grid((waves + WPG - 1) / WPG * 256);  // one group per WPG wavefronts
   a.xcd_per = g.per;
-  k_csr_stage<VT, AT, G, WPG><<<g.groups, 64 * WPG, 0, s>>>(a);
+  launch(k_csr_stage<VT, AT, G, WPG>, g.groups, 64 * WPG, s, a);
 }
 
 template <class VT, class AT>
@@ -332,7 +356,7 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
     } else {
       Grid g = make_grid(A.n_rows);
       a.xcd_per = g.per;
-      k_csr_scalar<VT, AT><<<g.groups, 256, 0, s>>>(a);
+      launch(k_csr_scalar<VT, AT>, g.groups, 256, s, a);
     }
   } else if (variant == BMFP_CSR_VECTOR && csr_stage_group(A.mean_row_blocks, A.max_row_blocks, 2) > 0) {
     switch (csr_stage_group(A.mean_row_blocks, A.max_row_blocks, 2)) {
@@ -346,12 +370,12 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
     Grid g = make_grid((int64_t) A.n_rows * G);
     a.xcd_per = g.per;
     switch (G) {
-    case 2: k_csr_vector<VT, AT, 2><<<g.groups, 256, 0, s>>>(a); break;
-    case 4: k_csr_vector<VT, AT, 4><<<g.groups, 256, 0, s>>>(a); break;
-    case 8: k_csr_vector<VT, AT, 8><<<g.groups, 256, 0, s>>>(a); break;
-    case 16: k_csr_vector<VT, AT, 16><<<g.groups, 256, 0, s>>>(a); break;
-    case 32: k_csr_vector<VT, AT, 32><<<g.groups, 256, 0, s>>>(a); break;
-    default: k_csr_vector<VT, AT, 64><<<g.groups, 256, 0, s>>>(a); break;
+    case 2: launch(k_csr_vector<VT, AT, 2>, g.groups, 256, s, a); break;
+    case 4: launch(k_csr_vector<VT, AT, 4>, g.groups, 256, s, a); break;
+    case 8: launch(k_csr_vector<VT, AT, 8>, g.groups, 256, s, a); break;
+    case 16: launch(k_csr_vector<VT, AT, 16>, g.groups, 256, s, a); break;
+    case 32: launch(k_csr_vector<VT, AT, 32>, g.groups, 256, s, a); break;
+    default: launch(k_csr_vector<VT, AT, 64>, g.groups, 256, s, a); break;
     }
   } else if (variant == BMFP_CSR_MERGE) {
     const int64_t waves = merge_waves(A.n_rows, A.nnzb);
@@ -360,12 +384,13 @@ int launch_csr_typed(const bmfp_matrix_t &A, int variant, const void *x, void *y
     int *carry_row = A.merge_carry.as<int>();
     AT *carry_val = reinterpret_cast<AT *>(A.merge_carry.as<char>() + (size_t) waves * 8);
     int *bounds = reinterpret_cast<int *>(A.merge_carry.as<char>() + (size_t) waves * 16);
-    k_csr_merge_bounds<MERGE_TILE><<<(unsigned) ((waves + 1 + 255) / 256), 256, 0, s>>>(A.n_rows, A.nnzb, waves, a.rp, bounds);
+    launch_aux(k_csr_merge_bounds<MERGE_TILE>, (unsigned) ((waves + 1 + 255) / 256), 256, s, A.n_rows, A.nnzb,
+               waves, a.rp, bounds);
     BMFP_LAUNCHED();
-    k_csr_merge<VT, AT><<<(unsigned) ((waves + 3) / 4), 256, 0, s>>>(a, A.nnzb, waves, bounds, carry_row, carry_val);
+    launch(k_csr_merge<VT, AT>, (unsigned) ((waves + 3) / 4), 256, s, a, A.nnzb, waves, bounds, carry_row, carry_val);
     BMFP_LAUNCHED();
-    k_csr_merge_fixup<AT><<<(unsigned) ((waves + 255) / 256), 256, 0, s>>>(waves, A.n_rows, carry_row, carry_val,
-                                                                          a.y);
+    launch_aux(k_csr_merge_fixup<AT>, (unsigned) ((waves + 255) / 256), 256, s, waves, A.n_rows, carry_row,
+               carry_val, a.y);
   } else {
     return fail(BMFP_ERR_INVALID, "variant %d is not a CSR kernel family", variant);
   }
@@ -381,6 +406,23 @@ int launch_csr(const bmfp_matrix_t &A, int variant, const void *x, void *y, hipS
   case BMFP_F32: return launch_csr_typed<float, float>(A, variant, x, y, s);
   default: return fail(BMFP_ERR_UNSUPPORTED, "CSR SpMV supports F64 and F32 storage");
   }
+}
+
+int spmv_kernel_fn(const bmfp_matrix_t &A, int variant, const void **fn)
+{
+  if (variant == BMFP_SPMV_AUTO) variant = auto_variant(A);
+  if (variant == BMFP_BCSR_LIBRARY || variant == BMFP_BCSR_JIT)
+    return fail(BMFP_ERR_UNSUPPORTED, "variant %d runs library or run-time compiled code", variant);
+  const bool csr_family = variant == BMFP_CSR_SCALAR || variant == BMFP_CSR_VECTOR || variant == BMFP_CSR_MERGE;
+  if (A.is_bcsr == csr_family) return fail(BMFP_ERR_INVALID, "variant %d does not match the matrix format", variant);
+  tl_probe = true;
+  tl_probe_fn = nullptr;
+  const int st = A.is_bcsr ? launch_bcsr(A, variant, nullptr, nullptr, nullptr)
+                           : launch_csr(A, variant, nullptr, nullptr, nullptr);
+  tl_probe = false;
+  *fn = tl_probe_fn;
+  if (st == BMFP_OK && !*fn) return fail(BMFP_ERR_INVALID, "the matrix has no rows: no kernel runs");
+  return st;
 }
 
 // ---- rocSPARSE bsrmv: the library baseline row (reference: cuSPARSE) -------

@@ -233,6 +233,13 @@ int bmfp_spmv_timed(bmfp_matrix A, int variant, const void *x, void *y,
 int bmfp_jit_build(int bs, int dtype, int layout, size_t *code_bytes, double *seconds);
 /* The family BMFP_SPMV_AUTO resolves to for A. */
 int bmfp_spmv_auto_variant(bmfp_matrix A, int *variant);
+/* Writes into buf (len bytes) the code-object symbol of the kernel that
+ * bmfp_spmv(A, variant, ...) launches, mangled as `rocprofv3 -M` prints it
+ * (for the merge path, its tile kernel); launches nothing.  No counterpart
+ * in the reference: measurement support, so that counter profiles can be
+ * matched to the kernel a run picked.  BMFP_ERR_UNSUPPORTED for the library
+ * and run-time compiled families; BMFP_ERR_INVALID when A has no rows. */
+int bmfp_spmv_kernel_symbol(bmfp_matrix A, int variant, char *buf, size_t len);
 
 /* ---- BiCGStab ------------------------------------------------------------
  * Replaces gpu_bicgstab<T,C> (bicgstab.h:12-47, bicgstab.cu:23-526).  The

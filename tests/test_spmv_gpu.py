@@ -306,6 +306,36 @@ def test_csr_merge_irregular_rows(name, dtype):
     check_csr(c, x, bm.CSR_MERGE, dtype=dtype)
 
 
+def test_kernel_symbol_names_the_launched_kernel():
+    # bmfp_spmv_kernel_symbol resolves the family the same way bmfp_spmv does
+    m, x = stencil_case(9, 7, 4, kind=0, seed=4)
+    R = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, 4)
+    C = bm.Matrix.bcsr(m.row_ptr, m.columns, oracle.transpose_blocks(m), m.n_rows, m.n_cols, 4,
+                       layout=bm.COL_MAJOR)
+    assert "k_bcsr_dgr" in R.kernel_symbol(bm.BCSR_GROUP_PER_ROW)
+    assert "k_bcsr_dgc" in C.kernel_symbol(bm.BCSR_GROUP_PER_ROW)
+    assert "k_bcsr_shfx" in C.kernel_symbol(bm.BCSR_SHUFFLE_X)
+    assert "k_bcsr_tpr" in R.kernel_symbol(bm.BCSR_THREAD_PER_ROW)
+    assert R.kernel_symbol(bm.SPMV_AUTO) == R.kernel_symbol(R.auto_variant())
+    assert R.kernel_symbol(bm.BCSR_GROUP_PER_ROW) != C.kernel_symbol(bm.BCSR_GROUP_PER_ROW)
+    c = oracle.bcsr_to_csr(m)
+    A = bm.Matrix.csr(c.row_ptr, c.columns, c.values, c.n_rows, c.n_cols)
+    sym = A.kernel_symbol(bm.CSR_MERGE)
+    assert "k_csr_merge" in sym and "bounds" not in sym and "fixup" not in sym
+    with pytest.raises(bm.BmfpError) as e:
+        R.kernel_symbol(bm.BCSR_LIBRARY)
+    assert e.value.status == bm.ERR_UNSUPPORTED
+    with pytest.raises(bm.BmfpError):
+        R.kernel_symbol(bm.CSR_MERGE)             # a CSR family on a BCSR matrix
+    # the query launched nothing: y keeps its contents
+    dx = bm.DeviceArray.from_host(x)
+    dy = bm.DeviceArray.from_host(np.full(R.y_len, 7.0))
+    R.kernel_symbol(bm.BCSR_GROUP_PER_ROW)
+    bm.synchronize()
+    assert np.all(dy.download() == 7.0)
+    del dx
+
+
 def test_bcsr_auto_choice_small_blocks():
     # bs 1-3: LDS-staged lane groups; bs 1 only while rows average <= 9 blocks;
     # FP64 column-major bs 4-24: column lane groups, bs 32: shuffle-shared x
