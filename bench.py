@@ -747,14 +747,19 @@ def main():
         return
     if args.solver_only:
         res, _ = solver_comparison(stream, args.solver_max_iter)
-        print(json.dumps({"solver": res}), flush=True)
+        out = {"solver": res}
+        if args.copy_mb > 0:                          # counter calibration
+            out["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 5, stream)
+        print(json.dumps(out), flush=True)
         return
     if args.serve_only:
-        print(json.dumps(serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
-                                            [float(v) for v in args.serve_lambdas.split(",")],
-                                            args.solver_max_iter, stream, parse_overrides(args.serve_config),
-                                            plain=not args.serve_no_plain, jacobi=not args.serve_no_jacobi)),
-              flush=True)
+        out = serving_comparison(args.serve_requests, args.serve_low_requests, args.serve_plain_sample,
+                                 [float(v) for v in args.serve_lambdas.split(",")],
+                                 args.solver_max_iter, stream, parse_overrides(args.serve_config),
+                                 plain=not args.serve_no_plain, jacobi=not args.serve_no_jacobi)
+        if args.copy_mb > 0:                          # counter calibration
+            out["copy_yardstick"] = measure_copy(args.copy_mb * 1e6, 5, stream)
+        print(json.dumps(out), flush=True)
         return
 
     cases = []

@@ -507,7 +507,8 @@ struct DenseEngine {
   // workgroups of a one-tile-per-wavefront launch
   static int64_t groups1(int64_t n_brows) { return ((n_brows * BS + RPW - 1) / RPW + 3) / 4; }
   template <class OP, class EPI>
-  __device__ __forceinline__ static void tile(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t wt)
+  __device__ __forceinline__ static void tile(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t wt,
+                                              const int *stop = nullptr)
   {
     constexpr int ALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
     constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * BS), lowbit((int) sizeof(AT) * VEC));
@@ -525,6 +526,15 @@ struct DenseEngine {
       b1 = M.rp[br + 1];
       if constexpr (SCALED) sc = (AT) M.scale[br];
       if (g == 0) pre = epi.pre(row);
+    }
+    if (stop) {
+      // a stopped solve's tile returns here; the flag is read by a vector
+      // load (relaxed, device scope: no scalar-cache wait in front of the
+      // bounds) issued beside the bounds, and an empty asm that reads all
+      // three keeps the compiler from sinking the loads past the test
+      const int dn = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(b), "v"(b1), "v"(dn));
+      if (__builtin_amdgcn_readfirstlane(dn)) return;
     }
     AT acc = 0;
     for (; b + U <= b1; b += U) {
@@ -556,11 +566,14 @@ struct DenseEngine {
       epi(row, acc, pre);
     }
   }
-  // one tile per wavefront (grid of groups1 workgroups)
+  // one tile per wavefront (grid of groups1 workgroups); false when *stop
+  // was set (the tile did nothing)
   template <class OP, class EPI>
-  __device__ __forceinline__ static void run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  __device__ __forceinline__ static bool run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi,
+                                              const int *stop = nullptr)
   {
-    tile(M, op, epi, ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6);
+    tile(M, op, epi, ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6, stop);
+    return !(stop && __builtin_amdgcn_readfirstlane(*stop));
   }
   template <class OP, class EPI>
   __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
@@ -592,7 +605,8 @@ struct RowsEngine {
   static int64_t threads(int64_t n_brows) { return n_brows * TPB; }
   static int64_t groups1(int64_t n_brows) { return (n_brows * TPB + 255) / 256; }
   template <class OP, class EPI>
-  __device__ __forceinline__ static void item(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t t)
+  __device__ __forceinline__ static void item(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, int64_t t,
+                                              const int *stop = nullptr)
   {
     constexpr int CC = BS % 8 == 0 ? 8 : BS;  // columns per step (bounds registers)
     constexpr int VALIGN = cmin(lowbit((int) sizeof(VT) * BS * R), 16);
@@ -607,6 +621,11 @@ struct RowsEngine {
     for (int i = 0; i < R; ++i) {
       pre[i] = epi.pre((int64_t) br * BS + r0 + i);
       acc[i] = 0;
+    }
+    if (stop) {  // the stop test, as in DenseEngine::tile
+      const int dn = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(b0), "v"(b1), "v"(dn));
+      if (__builtin_amdgcn_readfirstlane(dn)) return;
     }
     int b = b0;
     if constexpr (U > 1) {
@@ -652,10 +671,12 @@ struct RowsEngine {
     }
   }
   template <class OP, class EPI>
-  __device__ __forceinline__ static void run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
+  __device__ __forceinline__ static bool run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi,
+                                              const int *stop = nullptr)
   {
     const int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x;
-    if (t < (int64_t) M.n_brows * TPB) item(M, op, epi, t);
+    if (t < (int64_t) M.n_brows * TPB) item(M, op, epi, t, stop);
+    return !(stop && __builtin_amdgcn_readfirstlane(*stop));
   }
   template <class OP, class EPI>
   __device__ __forceinline__ static void run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi)
@@ -944,16 +965,16 @@ template <class E, class = void> struct one_tile : std::false_type {};
 template <class E> struct one_tile<E, decltype((void) &E::groups1)> : std::true_type {};
 
 // The stop flag: a one-tile launch has tens of thousands of workgroups, so
-// each wavefront tests the flag itself (a scalar load, no workgroup barrier)
-// and a stopped solve's launches end at once; persistent engines check it
-// once per workgroup.  The flag is written only by earlier kernels, so every
+// each wavefront tests the flag itself (a scalar load, no workgroup barrier),
+// after issuing its row bounds (the test's latency then overlaps them), and a
+// stopped solve's launches end at once; persistent engines check it once per
+// workgroup.  The flag is written only by earlier kernels, so every
 // wavefront of a workgroup takes the same branch.
 template <class ENGINE, class VT, class AT, class OP, class EPI>
 __device__ __forceinline__ bool split_run(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi, const SolverState *st)
 {
   if constexpr (one_tile<ENGINE>::value) {
-    if (__builtin_amdgcn_readfirstlane(st->done)) return false;
-    ENGINE::run1(M, op, epi);
+    return ENGINE::run1(M, op, epi, &st->done);
   } else {
     if (solver_stopped(st)) return false;
     ENGINE::run(M, op, epi);

@@ -112,12 +112,12 @@ void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
          bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
-template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED, bool NT>
 void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 {
   constexpr int BPW = 64 / (BS * (BS / VEC));
   Grid g = make_grid((A.n_rows + BPW - 1) / BPW * 64);
-  launch(k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+  launch(k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED, NT>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
 // Column-major FP64 lane-group shapes (values per lane, blocks in flight per
@@ -126,12 +126,15 @@ void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 // 73-75, bs 6 (2, 9) 69-75, bs 8 (2, 9) 73-75, bs 12 (4, 3) 72-75,
 // bs 16 (4, 3) 75, bs 24 (12, 1) 73-77.  At bs 32 every shape stays near 50
 // (64 lanes hold one block row, so a wavefront has one x segment per block),
-// and the per-row lane-slice kernel is kept.
+// and the per-row lane-slice kernel is kept.  bs 8 loads its values
+// non-temporal (73.3 -> 76.1 / 78.4 in paired runs); at the other sizes that
+// was slower (bs 12 -7, bs 16 -5, bs 24 -40 points).
 template <int BS>
 constexpr DenseShape dgc_shape()
 {
   if constexpr (BS == 4) return {2, 3};
-  else if constexpr (BS == 6 || BS == 8) return {2, 9};
+  else if constexpr (BS == 6) return {2, 9};
+  else if constexpr (BS == 8) return {2, 9, true};
   else if constexpr (BS == 12 || BS == 16) return {4, 3};
   else if constexpr (BS == 24) return {12, 1};
   else return {0, 0};
@@ -158,10 +161,11 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
     } else if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
       constexpr int RPW = 64 / (BS / D.vec);
       Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
-      launch(k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      launch(k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED, D.nt>, g.groups, 256, s,
+             bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     } else if constexpr (LAYOUT == BMFP_COL_MAJOR && sizeof(VT) == 8 && dgc_shape<BS>().vec > 0) {
       constexpr DenseShape G = dgc_shape<BS>();
-      launch_dgc<VT, AT, BS, G.vec, G.u, SCALED>(A, x, y, s);
+      launch_dgc<VT, AT, BS, G.vec, G.u, SCALED, G.nt>(A, x, y, s);
     } else {
       constexpr int VEC = gpr_vec<VT>(BS);
       constexpr int G = pow2ceil(BS / VEC);

@@ -51,8 +51,8 @@ constexpr int pow2ceil(int v) { int p = 1; while (p < v) p *= 2; return p; }
 
 // Loads N consecutive VT from p, whose byte address is known to be a
 // multiple of ALIGN, in the widest chunks (<= 16 bytes) that alignment and
-// size allow.
-template <class VT, int N, int ALIGN>
+// size allow; NT marks 8- and 16-byte chunks non-temporal (streamed data).
+template <class VT, int N, int ALIGN, bool NT = false>
 __device__ __forceinline__ void load_n(const VT *__restrict__ p, VT (&out)[N])
 {
   constexpr int BYTES = N * (int) sizeof(VT);
@@ -62,13 +62,15 @@ __device__ __forceinline__ void load_n(const VT *__restrict__ p, VT (&out)[N])
   if constexpr (CH == 16) {
 #pragma unroll
     for (int i = 0; i < BYTES / 16; ++i) {
-      u32x4 v = *reinterpret_cast<const u32x4 *>(src + 16 * i);
+      const u32x4 *q = reinterpret_cast<const u32x4 *>(src + 16 * i);
+      u32x4 v = NT ? __builtin_nontemporal_load(q) : *q;
       __builtin_memcpy(dst + 16 * i, &v, 16);
     }
   } else if constexpr (CH == 8) {
 #pragma unroll
     for (int i = 0; i < BYTES / 8; ++i) {
-      u32x2 v = *reinterpret_cast<const u32x2 *>(src + 8 * i);
+      const u32x2 *q = reinterpret_cast<const u32x2 *>(src + 8 * i);
+      u32x2 v = NT ? __builtin_nontemporal_load(q) : *q;
       __builtin_memcpy(dst + 8 * i, &v, 8);
     }
   } else {
@@ -261,10 +263,15 @@ __global__ __launch_bounds__(256) void k_bcsr_gpr(BcsrArgs<VT, AT> a)
 // fixed shuffle tree over the group.  Shapes per block size (FP64) from
 // timing on the HBM-resident stencils, dense_shape(): VEC = 2 with U = 9 (a
 // whole 9-block stencil row in one batch) is the fastest everywhere but at
-// bs 6 (U = 3) and bs 24 (VEC = 4, U = 1).
+// bs 6 (U = 3) and bs 24 (VEC = 4, U = 1).  Non-temporal value loads (the
+// values are streamed once; x and the indices keep the caches) raise the
+// 9-block shapes at bs 8-32 (paired runs, % of 8 TB/s: bs 8 73.6 -> 76.0 /
+// 78.1, bs 16 76.0 / 79.1 -> 81.3 / 83.5, bs 32 78.2 / 80.0 -> 84.9 / 87.0)
+// and lower bs 4, 6 and 24, which keep cached loads.
 // ============================================================================
 struct DenseShape {
   int vec, u;
+  bool nt = false;  // values loaded non-temporal
 };
 template <class VT, int BS> constexpr DenseShape dense_shape()
 {
@@ -272,11 +279,11 @@ template <class VT, int BS> constexpr DenseShape dense_shape()
   switch (BS) {
   case 4: return {2, 9};
   case 6: return {2, 3};
-  case 8: return {2, 9};
-  case 12: return {2, 9};
-  case 16: return {2, 9};
+  case 8: return {2, 9, true};
+  case 12: return {2, 9, true};
+  case 16: return {2, 9, true};
   case 24: return {4, 1};
-  case 32: return {2, 9};
+  case 32: return {2, 9, true};
   default: return {0, 0};
   }
 }
@@ -297,7 +304,7 @@ __device__ __forceinline__ AT group_tree_sum(AT acc, int lane, int lr, int g)
   return acc;
 }
 
-template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED, bool NT = false>
 __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 {
   constexpr int G = BS / VEC;
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 #pragma unroll
     for (int k = 0; k < U; ++k) col[k] = a.cols[b + k];
 #pragma unroll
-    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN>(a.vals + ((int64_t) (b + k) * BS + r) * BS + g * VEC, v[k]);
+    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN, NT>(a.vals + ((int64_t) (b + k) * BS + r) * BS + g * VEC, v[k]);
 #pragma unroll
     for (int k = 0; k < U; ++k) load_n<AT, VEC, XALIGN>(a.x + (int64_t) col[k] * BS + g * VEC, xv[k]);
 #pragma unroll
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
   for (; U > 1 && b < b1; ++b) {
     VT v[VEC];
     AT xv[VEC];
-    load_n<VT, VEC, ALIGN>(a.vals + ((int64_t) b * BS + r) * BS + g * VEC, v);
+    load_n<VT, VEC, ALIGN, NT>(a.vals + ((int64_t) b * BS + r) * BS + g * VEC, v);
     load_n<AT, VEC, XALIGN>(a.x + (int64_t) a.cols[b] * BS + g * VEC, xv);
 #pragma unroll
     for (int c = 0; c < VEC; ++c) acc = fmadd(cvt<AT>(v[c]), xv[c], acc);
@@ -360,7 +367,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
 // (stride H), and lane h (c = 0) stores its VEC rows.  64/L block rows per
 // wavefront.
 // ============================================================================
-template <class VT, class AT, int BS, int VEC, int U, bool SCALED>
+template <class VT, class AT, int BS, int VEC, int U, bool SCALED, bool NT = false>
 __global__ __launch_bounds__(256) void k_bcsr_dgc(BcsrArgs<VT, AT> a)
 {
   constexpr int H = BS / VEC, L = BS * H, BPW = 64 / L;
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgc(BcsrArgs<VT, AT> a)
 #pragma unroll
     for (int k = 0; k < U; ++k) col[k] = a.cols[b + k];
 #pragma unroll
-    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN>(a.vals + (int64_t) (b + k) * (BS * BS) + off, v[k]);
+    for (int k = 0; k < U; ++k) load_n<VT, VEC, ALIGN, NT>(a.vals + (int64_t) (b + k) * (BS * BS) + off, v[k]);
 #pragma unroll
     for (int k = 0; k < U; ++k) xv[k] = a.x[(int64_t) col[k] * BS + c];
 #pragma unroll
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgc(BcsrArgs<VT, AT> a)
   }
   for (; U > 1 && b < b1; ++b) {
     VT v[VEC];
-    load_n<VT, VEC, ALIGN>(a.vals + (int64_t) b * (BS * BS) + off, v);
+    load_n<VT, VEC, ALIGN, NT>(a.vals + (int64_t) b * (BS * BS) + off, v);
     const AT xe = a.x[(int64_t) a.cols[b] * BS + c];
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = fmadd(cvt<AT>(v[i]), xe, acc[i]);
