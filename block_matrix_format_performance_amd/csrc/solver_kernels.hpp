@@ -718,8 +718,10 @@ using SolverEngine = typename std::conditional<
 // 168 / 156 us against 178 / 168 with 3; FP32 bs 4: two rows, 32 bytes, per
 // thread in batches of 3 blocks, the FP32 refinement of the 2048x1024 bs 4
 // system 19.6 ms against 19.9 with one row in batches of 9, 21.4 with two
-// rows in batches of 9 and 24.2 with four rows); the fused schedule's
-// engines otherwise.
+// rows in batches of 9 and 24.2 with four rows; staging a wavefront's 32
+// block rows in LDS first, so that each load instruction covers whole cache
+// lines, ran no faster in FP32 or FP16); the fused schedule's engines
+// otherwise.
 template <class VT, int BS> constexpr int split_vec() { return dense_shape<VT, BS>().vec > 0 ? dense_shape<VT, BS>().vec : 1; }
 template <class VT, int BS> constexpr int split_u() { return dense_shape<VT, BS>().u > 0 ? dense_shape<VT, BS>().u : 1; }
 template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
