@@ -48,6 +48,7 @@
 #include <cmath>
 #include <deque>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "matrix.hpp"
@@ -66,10 +67,12 @@ enum : int { REQ_EMPTY = 0, REQ_RESID = 1, REQ_INNER_A = 2, REQ_INNER_B = 3, REQ
 // Device view of one request.  Vectors interleave the nrhs right-hand sides:
 // element i of right-hand side k is at i*nrhs + k.
 struct ReqDesc {
-  int bs, n_brows, nrhs, chunk0, n_chunks, pad;
+  int bs, n_brows, nrhs, chunk0, n_chunks;
+  int sl_j;   // > 0: the inner matrix is pass-sliced with sl_j block slots per row; 0: BCSR order
   int64_t n;  // scalar rows of one right-hand side
   const int *rp;
   const int *cols;
+  const int *cols_sl;   // block columns of the pass-sliced inner matrix (sl_j > 0)
   const double *v64;
   const float *v32;     // FP32 inner matrix, or null
   const __half *v16;    // FP16 inner matrix (one FP32 scale per block row in sc16), or null
@@ -100,6 +103,16 @@ struct ReqState {
 struct TileDesc {
   int req, chunk, passes;
 };
+
+// The tile table lists the tiles of each served block size together.  The
+// counts array holds the table size, the finalize list size, then per block
+// size (1, 2, 3, 4, 6, 8, in that order) the first tile and the tile count.
+constexpr int SERVE_NBS = 6;
+__host__ __device__ constexpr int serve_bs_index(int bs)
+{
+  return bs <= 4 ? bs - 1 : bs == 6 ? 4 : 5;
+}
+constexpr int SERVE_COUNTS = 2 + 2 * SERVE_NBS;
 
 __host__ __device__ constexpr int serve_chunk_brows(int bs) { return SERVE_CHUNK_PASSES * (64 / bs); }
 
@@ -243,13 +256,13 @@ template <class VT, class AT, int BS, int NR> constexpr int spmv_batch()
   return 40 / regs > 9 ? 9 : (40 / regs < 1 ? 1 : 40 / regs);
 }
 
-template <class VT, class AT, int BS, int NR, class OWNT, class GATHER, class OWN, class EPI>
+template <class VT, class AT, int BS, int NR, class OWNT, int UB = 0, class GATHER, class OWN, class EPI>
 __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals, int br0, bool use_matrix,
                                            const GATHER &gather, const OWN &own, const EPI &epi,
                                            gptr<const float> scale = nullptr)
 {
   constexpr int RPW = 64 / BS;
-  constexpr int U = spmv_batch<VT, AT, BS, NR>();
+  constexpr int U = UB > 0 ? UB : spmv_batch<VT, AT, BS, NR>();
   const int lane = threadIdx.x & 63;
   const int lbr = lane / BS, r = lane - lbr * BS, base_lane = lbr * BS;
   const gptr<const int> rp = glb(D.rp), cols = glb(D.cols);
@@ -304,6 +317,74 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
       for (int k = 0; k < NR; ++k)
 #pragma unroll
         for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[c], group_bcast<BS>(xe[k], c, base_lane), acc[k]);
+    }
+    if (scale)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) acc[k] *= sc;
+    if (on) epi(off, acc, mine);
+  }
+}
+
+// Pass-sliced inner matrices.  A pass is the RPW = 64/BS consecutive block
+// rows one wavefront covers; the inner (FP32 or FP16) copy stores pass P's
+// blocks slot by slot: block j of the pass's row lbr is sliced block
+// (P*J + j)*RPW + lbr, J = the longest row rounded up to a multiple of 3, and
+// rows shorter than J are padded with zero blocks (column: the row's last
+// block column, 0 for an empty row).  Row r of that block then starts at
+// element ((P*J + j)*RPW*BS + lane)*BS for lane = lbr*BS + r: every load of a
+// wavefront reads one contiguous range (RPW*BS*BS values per slot) instead of
+// RPW separate blocks, and every lane of the wavefront runs the same J
+// iterations.  Products and their order are those of the BCSR path (a zero
+// block adds 0 * x).
+template <class VT, class AT, int BS, int NR, class OWNT, class GATHER, class OWN, class EPI>
+__device__ __forceinline__ void chunk_spmv_sl(const ReqDesc &D, gptr<const VT> vals, int br0, const GATHER &gather,
+                                              const OWN &own, const EPI &epi, gptr<const float> scale = nullptr)
+{
+  constexpr int RPW = 64 / BS, LPW = RPW * BS, U = 3;
+  const int lane = threadIdx.x & 63;
+  const int ln = lane < LPW ? lane : LPW - 1;  // spare lanes repeat the last row's loads
+  const int lbr = ln / BS, r = ln - lbr * BS, base_lane = lbr * BS;
+  const gptr<const int> cs = glb(D.cols_sl);
+  const int J = D.sl_j;  // a multiple of U
+  // passes of this chunk that hold rows (uniform)
+  const int passes = min(SERVE_CHUNK_PASSES, (D.n_brows - br0 + RPW - 1) / RPW);
+  // operand offsets of slots q0..q0+U-1 of the chunk's slot sequence (pass
+  // by pass): loaded one batch ahead, so each batch waits on one round trip
+  auto load_xo = [&](int q0, uint32_t (&xo)[U]) {
+    const int ps = q0 / J, j0 = q0 - ps * J;
+    const int64_t sb = ((int64_t) (br0 / RPW + ps) * J + j0) * RPW + lbr;
+#pragma unroll
+    for (int u = 0; u < U; ++u) xo[u] = (uint32_t) (ld(cs, (uint32_t) (sb + u * RPW)) * BS + r) * NR;
+  };
+  uint32_t xo[U];
+  if (passes > 0) load_xo(0, xo);
+#pragma unroll 1
+  for (int ps = 0; ps < passes; ++ps) {
+    const int br = br0 + ps * RPW + lbr;
+    const bool on = lane < LPW && br < D.n_brows;
+    const uint32_t off = (uint32_t) (br * BS + r) * NR;
+    OWNT mine;
+    if (on) own(off, mine);
+    const AT sc = (scale && on) ? (AT) ld(scale, (uint32_t) br) : (AT) 1;
+    AT acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0;
+    const int64_t s0 = (int64_t) (br0 / RPW + ps) * J * RPW;  // the pass's first sliced block
+    for (int j = 0; j < J; j += U) {
+      VT v[U][BS];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load_row<VT, BS>(vals + ((s0 + (int64_t) (j + u) * RPW) * BS + ln) * BS, v[u]);
+      AT xe[U][NR];
+#pragma unroll
+      for (int u = 0; u < U; ++u) gather(xo[u], xe[u]);
+      const int qn = ps * J + j + U;  // the next batch, in this pass or the next
+      if (qn < passes * J) load_xo(qn, xo);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+          for (int c = 0; c < BS; ++c) acc[k] = fmadd((AT) v[u][c], group_bcast<BS>(xe[u][k], c, base_lane), acc[k]);
     }
     if (scale)
 #pragma unroll
@@ -371,7 +452,7 @@ __device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S,
   const bool first = S.outer == 0;
   const gptr<const double> xn = glb((const double *) D.x[1 - par]), b = glb(D.b);
   const gptr<float> r32 = glb(D.r32);
-  chunk_spmv<double, double, BS, NR, ResidOwn<NR>>(
+  chunk_spmv<double, double, BS, NR, ResidOwn<NR>, (BS >= 6 ? 3 : 0)>(
       D, glb(D.v64), br0, !first, [&](uint32_t o, double (&x)[NR]) { ldv<double, NR>(xn, o, x); },
       [&](uint32_t o, ResidOwn<NR> &m) { ldv<double, NR>(b, o, m.b); },
       [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
@@ -444,10 +525,12 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
   };
   if constexpr (BS >= SERVE_F16_MIN_BS)
     if (D.v16) {
-      chunk_spmv<__half, float, BS, NR, RhOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
+      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, RhOwn<NR>>(D, glb(D.v16), br0, gather, own, epi, glb(D.sc16));
+      else chunk_spmv<__half, float, BS, NR, RhOwn<NR>, 1>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       return;
     }
-  chunk_spmv<float, float, BS, NR, RhOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
+  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, RhOwn<NR>>(D, glb(D.v32), br0, gather, own, epi);
+  else chunk_spmv<float, float, BS, NR, RhOwn<NR>, 1>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
 // s = ri - alpha v (ri = r32 at iteration 0).
@@ -502,10 +585,12 @@ __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &
   };
   if constexpr (BS >= SERVE_F16_MIN_BS)
     if (D.v16) {
-      chunk_spmv<__half, float, BS, NR, SOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
+      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, SOwn<NR>>(D, glb(D.v16), br0, gather, own, epi, glb(D.sc16));
+      else chunk_spmv<__half, float, BS, NR, SOwn<NR>, 1>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       return;
     }
-  chunk_spmv<float, float, BS, NR, SOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
+  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, SOwn<NR>>(D, glb(D.v32), br0, gather, own, epi);
+  else chunk_spmv<float, float, BS, NR, SOwn<NR>, 1>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
 // ri = s - omega t, d += alpha p + omega s, or d += D^-1 (alpha p + omega s)
@@ -623,7 +708,14 @@ __device__ __forceinline__ void run_tile(const ReqDesc &D, const ReqState &S, in
 // same operand rows, share one L2 while every XCD still sees tiles from the
 // whole table.  With 0 the tiles go to wavefronts in table order.  Grids are
 // multiples of 8 workgroups when K > 0.
-template <int SLOT>
+// The SpMV slots (R, A, B) run as one kernel per block size (BSK > 0) over
+// that block size's range of the tile table: a kernel holding every block
+// size's paths needs the registers of their union (about 1.4x the largest
+// one), which would hold all of them to 2-3 waves per SIMD.  The elementwise
+// slots (BSK = 0) walk the whole table.
+template <int SLOT> constexpr bool slot_per_bs() { return SLOT == SLOT_R || SLOT == SLOT_A || SLOT == SLOT_B; }
+
+template <int SLOT, int BSK>
 __device__ __forceinline__ void slot_tile(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
                                           const TileDesc *__restrict__ tiles, double *__restrict__ part, int ti)
 {
@@ -632,15 +724,24 @@ __device__ __forceinline__ void slot_tile(const ReqDesc *__restrict__ rd, const 
   const int phase = S.phase;
   if (phase != slot_phase(SLOT)) return;
   const ReqDesc &D = rd[T.req];
-  BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+  if constexpr (BSK > 0) {
+    constexpr int BS = BSK;
+    BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part))));
+  } else {
+    BMFP_SERVE_BS_SWITCH(D.bs, BMFP_SERVE_NR_SWITCH(D.nrhs, (run_tile<SLOT, BS, NR>(D, S, phase, T, glb(part)))));
+  }
 }
 
-template <int SLOT>
+template <int SLOT, int BSK>
 __device__ __forceinline__ void slot_body(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
-                                              const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
-                                              double *__restrict__ part, int xcd_run, int persistent)
+                                          const TileDesc *__restrict__ tiles, const int *__restrict__ counts,
+                                          double *__restrict__ part, int xcd_run, int persistent)
 {
-  const int nt = *ntp;
+  int t0 = 0, nt = counts[0];
+  if constexpr (BSK > 0) {
+    t0 = counts[2 + 2 * serve_bs_index(BSK)];
+    nt = counts[3 + 2 * serve_bs_index(BSK)];
+  }
   const int wib = (int) (threadIdx.x >> 6);
   if (!persistent) {
     int64_t ti = (int64_t) blockIdx.x * 4 + wib;
@@ -648,7 +749,7 @@ __device__ __forceinline__ void slot_body(const ReqDesc *__restrict__ rd, const 
       const int64_t l = (int64_t) (blockIdx.x >> 3) * 4 + wib;
       ti = (l / xcd_run * 8 + (blockIdx.x & 7)) * xcd_run + l % xcd_run;
     }
-    if (ti < nt) slot_tile<SLOT>(rd, rs, tiles, part, __builtin_amdgcn_readfirstlane((int) ti));
+    if (ti < nt) slot_tile<SLOT, BSK>(rd, rs, tiles, part, __builtin_amdgcn_readfirstlane(t0 + (int) ti));
     return;
   }
   if (xcd_run > 0) {
@@ -658,21 +759,21 @@ __device__ __forceinline__ void slot_body(const ReqDesc *__restrict__ rd, const 
     for (int j = lw;; j += nwx) {
       const int64_t ti64 = ((int64_t) (j / xcd_run) * 8 + xcd) * xcd_run + j % xcd_run;
       if (ti64 >= nt) break;
-      slot_tile<SLOT>(rd, rs, tiles, part, (int) ti64);
+      slot_tile<SLOT, BSK>(rd, rs, tiles, part, t0 + (int) ti64);
     }
     return;
   }
   const int wave = __builtin_amdgcn_readfirstlane((int) blockIdx.x * 4 + wib);
   const int nw = (int) gridDim.x * 4;
-  for (int ti = wave; ti < nt; ti += nw) slot_tile<SLOT>(rd, rs, tiles, part, ti);
+  for (int ti = wave; ti < nt; ti += nw) slot_tile<SLOT, BSK>(rd, rs, tiles, part, t0 + ti);
 }
 
-template <int SLOT>
+template <int SLOT, int BSK = 0>
 __global__ __launch_bounds__(256) void k_slot(const ReqDesc *__restrict__ rd, const ReqState *__restrict__ rs,
-                                              const TileDesc *__restrict__ tiles, const int *__restrict__ ntp,
+                                              const TileDesc *__restrict__ tiles, const int *__restrict__ counts,
                                               double *__restrict__ part, int xcd_run, int persistent)
 {
-  slot_body<SLOT>(rd, rs, tiles, ntp, part, xcd_run, persistent);
+  slot_body<SLOT, BSK>(rd, rs, tiles, counts, part, xcd_run, persistent);
 }
 
 // ---- finalize: one workgroup per request; after the reduction lane k of the
@@ -852,6 +953,49 @@ __global__ __launch_bounds__(256) void k_serve_f16(int n_brows, int bs, const in
   const double inv = 1.0 / m;
   if (lane == 0) sc[br] = (float) m;
   for (int64_t e = e0 + lane; e < e1; e += 64) v16[e] = __float2half_rn((float) (v64[e] * inv));
+}
+
+// Pass-sliced inner copy of a request's FP64 BCSR matrix (layout: see
+// chunk_spmv_sl), FP32 or FP16; one wavefront per block row of the
+// RPW-padded pass grid (rows past n_brows are all padding).  FP16 rows are
+// scaled by one FP32 factor each, the row's max |entry| mapped to 1.0: the
+// same rounding as k_serve_f16 and bmfp_matrix_convert to BMFP_F16.
+template <class OT>
+__global__ __launch_bounds__(256) void k_serve_slice(int n_brows, int bs, int sl_j, const int *__restrict__ rp,
+                                                     const int *__restrict__ cols, const double *__restrict__ v64,
+                                                     OT *__restrict__ out, int *__restrict__ cols_sl,
+                                                     float *__restrict__ sc)
+{
+  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int rpw = 64 / bs;
+  const int64_t n_pad = ((int64_t) n_brows + rpw - 1) / rpw * rpw;
+  if (br >= n_pad) return;
+  const int64_t bs2 = (int64_t) bs * bs;
+  const int b0 = br < n_brows ? rp[br] : 0, len = br < n_brows ? rp[br + 1] - b0 : 0;
+  const int64_t e0 = (int64_t) b0 * bs2, e1 = e0 + (int64_t) len * bs2;
+  double inv = 1.0;
+  if constexpr (sizeof(OT) == 2) {
+    double m = 0.0;
+    for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(v64[e]));
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (m == 0.0) m = 1.0;
+    inv = 1.0 / m;
+    if (lane == 0 && br < n_brows) sc[br] = (float) m;
+  }
+  const int64_t p = br / rpw, lbr = br - p * rpw;
+  auto slot_block = [&](int64_t j) { return (p * sl_j + j) * rpw + lbr; };
+  for (int64_t e = lane; e < (int64_t) sl_j * bs2; e += 64) {
+    const int64_t j = e / bs2, q = e - j * bs2;
+    OT o = (OT) 0.0f;
+    if (j < len) {
+      if constexpr (sizeof(OT) == 2) o = __float2half_rn((float) (v64[e0 + e] * inv));
+      else o = (OT) v64[e0 + e];
+    }
+    out[slot_block(j) * bs2 + q] = o;
+  }
+  const int pad = len > 0 ? cols[b0 + len - 1] : 0;
+  for (int j = lane; j < sl_j; j += 64) cols_sl[slot_block(j)] = j < len ? cols[b0 + j] : pad;
 }
 
 // Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64

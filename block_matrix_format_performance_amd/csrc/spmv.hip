@@ -122,20 +122,23 @@ void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 
 // Column-major FP64 lane-group shapes (values per lane, blocks in flight per
 // lane), the fastest measured on HBM-resident 9-point stencil matrices
-// (~4.4e8 stored values, interleaved timings, % of 8 TB/s): bs 4 (2, 3)
-// 73-75, bs 6 (2, 9) 69-75, bs 8 (2, 9) 73-75, bs 12 (4, 3) 72-75,
-// bs 16 (4, 3) 75, bs 24 (12, 1) 73-77.  At bs 32 every shape stays near 50
-// (64 lanes hold one block row, so a wavefront has one x segment per block),
-// and the per-row lane-slice kernel is kept.  bs 8 loads its values
-// non-temporal (73.3 -> 76.1 / 78.4 in paired runs); at the other sizes that
-// was slower (bs 12 -7, bs 16 -5, bs 24 -40 points).
+// (~4.4e8 stored values, interleaved timings, % of 8 TB/s): bs 4 (2, 4) 74.1
+// (2, 3: 73.5), bs 6 (3, 4) 71.8-71.9 (3, 3: 71.4; 2, 9: 69.0; one x element
+// and three 8-byte loads per lane, 5 block rows in 60 lanes of a wavefront),
+// bs 8 (2, 9) 78.1, bs 12 (6, 2) 73.5 (4, 3: 72.0), bs 16 (4, 3) 75.2,
+// bs 24 (12, 1) 75.3.  At bs 32 every shape stays near 50 (64 lanes hold one
+// block row, so a wavefront has one x segment per block), and the per-row
+// lane-slice kernel is kept.  bs 8 loads its values non-temporal (73.3 ->
+// 76.1 / 78.4 in paired runs); at the other sizes that was slower (bs 6
+// -6 to -26, bs 12 -7, bs 16 -5 to -30, bs 24 -40 points).
 template <int BS>
 constexpr DenseShape dgc_shape()
 {
-  if constexpr (BS == 4) return {2, 3};
-  else if constexpr (BS == 6) return {2, 9};
+  if constexpr (BS == 4) return {2, 4};
+  else if constexpr (BS == 6) return {3, 4};
   else if constexpr (BS == 8) return {2, 9, true};
-  else if constexpr (BS == 12 || BS == 16) return {4, 3};
+  else if constexpr (BS == 12) return {6, 2};
+  else if constexpr (BS == 16) return {4, 3};
   else if constexpr (BS == 24) return {12, 1};
   else return {0, 0};
 }

@@ -84,6 +84,50 @@ def test_largest_stream_request_matches_oracle():
     check_against_oracle(512, 512, 8, 0, (2 << 32) + 7, (2 << 32) + 7, 4, res, x)
 
 
+def diag_dominant_bcsr(n_rows, bs, lens, seed):
+    """Random BCSR matrix with the given row lengths (diagonal block
+    included): off-diagonal entries uniform[-1,1], each diagonal entry
+    the row's sum of |off-diagonal entries| + 1."""
+    rng = np.random.default_rng(seed)
+    rp, cols = [0], []
+    for i, ln in enumerate(lens):
+        others = rng.choice(np.delete(np.arange(n_rows), i), ln - 1, replace=False)
+        cols.extend(sorted(others.tolist() + [i]))
+        rp.append(len(cols))
+    rp, cols = np.array(rp, np.int32), np.array(cols, np.int32)
+    vals = rng.uniform(-1, 1, len(cols) * bs * bs).reshape(-1, bs, bs)
+    for i in range(n_rows):
+        blocks = range(rp[i], rp[i + 1])
+        d = next(b for b in blocks if cols[b] == i)
+        offsum = sum(np.abs(vals[b]).sum(axis=1) for b in blocks) - np.abs(np.diag(vals[d]))
+        vals[d][np.diag_indices(bs)] = offsum + 1.0
+    return rp, cols, vals.reshape(-1)
+
+
+# explicit requests: rows of 5-9 blocks keep the pass-sliced inner copy (its
+# padding stays within bounds), one 60-block row among single-block rows
+# sends the inner copy back to BCSR order; FP32 (bs 2) and FP16 (bs 6) inner
+@pytest.mark.parametrize("bs,shape", [(2, "ragged"), (2, "skewed"), (6, "ragged"), (6, "skewed")])
+def test_explicit_ragged_request_matches_oracle(bs, shape):
+    n_rows = 300
+    rng = np.random.default_rng(bs)
+    lens = rng.integers(5, 10, n_rows) if shape == "ragged" else np.where(np.arange(n_rows) == 7, 60, 1)
+    rp, cols, vals = diag_dominant_bcsr(n_rows, bs, lens, seed=20 + bs)
+    n = n_rows * bs
+    B = oracle_rhs(9, 2, n)
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    S.submit(S.bcsr_request(rp, cols, vals, n_rows, bs, nrhs=2, b=B))
+    res, x = next(iter(S.drain().values()))
+    assert res.status == bm.OK, res.status
+    c = oracle.bcsr_to_csr(oracle.BCSR(n_rows, n_rows, bs, rp, cols, vals))
+    for k in range(2):
+        r = B[k] - oracle.csr_spmv(c, x[k])
+        assert np.linalg.norm(r) / np.linalg.norm(B[k]) < 1e-10
+        xo, info = oracle.bicgstab(c, B[k], 1e-12, 5000)
+        assert info.status == 0
+        assert np.linalg.norm(x[k] - xo) / np.linalg.norm(xo) < 1e-9
+
+
 def mix_requests():
     # uneven grids and block sizes, as a request stream would bring
     specs = [(17, 17, 2, 1), (31, 9, 3, 2), (12, 40, 4, 3), (25, 25, 6, 1), (9, 13, 8, 4), (50, 3, 1, 2),
