@@ -935,67 +935,118 @@ __global__ __launch_bounds__(256) void k_to_f32(const double *__restrict__ a, fl
   for (int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256) b[i] = (float) a[i];
 }
 
-// FP16 inner matrix of a request (row-major blocks): each block row scaled by
-// one FP32 factor, its max |entry| mapped to 1.0 -- the rounding of
-// bmfp_matrix_convert to BMFP_F16.  One wavefront per block row.
-__global__ __launch_bounds__(256) void k_serve_f16(int n_brows, int bs, const int *__restrict__ rp,
-                                                   const double *__restrict__ v64, __half *__restrict__ v16,
-                                                   float *__restrict__ sc)
+// FP16 inner matrices scale each block row by one FP32 factor: sc = the row's
+// max |entry| rounded to FP32 (1 for an all-zero row), values stored as
+// FP16(FP32(v / sc)) and multiplied by sc after the row's sums.
+//
+// The scale of every block row: one wavefront per row, one read of its
+// values (up to 9 per lane held in registers, longer rows in further passes).
+__global__ __launch_bounds__(256) void k_serve_rowmax(int n_brows, int bs, const int *__restrict__ rp,
+                                                      const double *__restrict__ v64, float *__restrict__ sc)
 {
   const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (br >= n_brows) return;
   const int64_t bs2 = (int64_t) bs * bs, e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
   double m = 0.0;
-  for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(v64[e]));
+  for (int64_t e = e0; e < e1; e += 9 * 64) {
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = e + k * 64 + lane < e1 ? v64[e + k * 64 + lane] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m = fmax(m, fabs(v[k]));
+  }
   for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if (m == 0.0) m = 1.0;
-  const double inv = 1.0 / m;
-  if (lane == 0) sc[br] = (float) m;
-  for (int64_t e = e0 + lane; e < e1; e += 64) v16[e] = __float2half_rn((float) (v64[e] * inv));
+  if (lane == 0) sc[br] = m == 0.0 ? 1.0f : (float) m;
+}
+
+template <class OT> __device__ __forceinline__ OT to_inner(double v, double inv);
+template <> __device__ __forceinline__ float to_inner<float>(double v, double) { return (float) v; }
+template <> __device__ __forceinline__ __half to_inner<__half>(double v, double inv)
+{
+  return __float2half_rn((float) (v * inv));
+}
+
+// FP16 inner matrix in BCSR order (the fallback for matrices whose rows are
+// too uneven to slice); sc from k_serve_rowmax.  One thread per value.
+__global__ __launch_bounds__(256) void k_serve_f16(int n_brows, int bs, const int *__restrict__ rp,
+                                                   const double *__restrict__ v64, const float *__restrict__ sc,
+                                                   __half *__restrict__ v16)
+{
+  const int64_t bs2 = (int64_t) bs * bs, nv = (int64_t) rp[n_brows] * bs2;
+  for (int64_t e = (int64_t) blockIdx.x * 256 + threadIdx.x; e < nv; e += (int64_t) gridDim.x * 256) {
+    // the block row of value e: binary search of the row pointers
+    const int b = (int) (e / bs2);
+    int lo = 0, hi = n_brows;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (rp[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    v16[e] = to_inner<__half>(v64[e], 1.0 / (double) sc[lo]);
+  }
+}
+
+// Stores the N values of v at p in the widest chunks (up to 16 bytes) the
+// row's size allows (p is aligned to the lowest set bit of N*sizeof(T)).
+template <class T, int N> __device__ __forceinline__ void store_words(T *p, const T (&v)[N])
+{
+  constexpr int BYTES = N * (int) sizeof(T);
+  constexpr int CH = lowbit(BYTES) < 16 ? lowbit(BYTES) : 16;
+  char *dst = reinterpret_cast<char *>(p);
+  const char *src = reinterpret_cast<const char *>(v);
+  if constexpr (CH >= 4) {
+    typedef unsigned int w_t __attribute__((ext_vector_type(CH / 4)));
+#pragma unroll
+    for (int c = 0; c < BYTES; c += CH) {
+      w_t t;
+      __builtin_memcpy(&t, src + c, CH);
+      *reinterpret_cast<w_t *>(dst + c) = t;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < N; ++c) p[c] = v[c];
+  }
 }
 
 // Pass-sliced inner copy of a request's FP64 BCSR matrix (layout: see
-// chunk_spmv_sl), FP32 or FP16; one wavefront per block row of the
-// RPW-padded pass grid (rows past n_brows are all padding).  FP16 rows are
-// scaled by one FP32 factor each, the row's max |entry| mapped to 1.0: the
-// same rounding as k_serve_f16 and bmfp_matrix_convert to BMFP_F16.
-template <class OT>
-__global__ __launch_bounds__(256) void k_serve_slice(int n_brows, int bs, int sl_j, const int *__restrict__ rp,
+// chunk_spmv_sl), FP32, or FP16 with the scales in sc.  One thread per row of
+// a sliced block (grid-stride; 32-bit index arithmetic, the sliced values
+// staying below 2^31): consecutive threads store consecutive rows, each row
+// read from its block in the FP64 copy, zero in padding blocks; the thread of
+// row 0 also writes the block's column (padding: the row's last block column,
+// 0 for an empty row).
+template <class OT, int BS>
+__global__ __launch_bounds__(256) void k_serve_slice(int n_brows, int sl_j, const int *__restrict__ rp,
                                                      const int *__restrict__ cols, const double *__restrict__ v64,
-                                                     OT *__restrict__ out, int *__restrict__ cols_sl,
-                                                     float *__restrict__ sc)
+                                                     const float *__restrict__ sc, OT *__restrict__ out,
+                                                     int *__restrict__ cols_sl)
 {
-  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const int rpw = 64 / bs;
-  const int64_t n_pad = ((int64_t) n_brows + rpw - 1) / rpw * rpw;
-  if (br >= n_pad) return;
-  const int64_t bs2 = (int64_t) bs * bs;
-  const int b0 = br < n_brows ? rp[br] : 0, len = br < n_brows ? rp[br + 1] - b0 : 0;
-  const int64_t e0 = (int64_t) b0 * bs2, e1 = e0 + (int64_t) len * bs2;
-  double inv = 1.0;
-  if constexpr (sizeof(OT) == 2) {
-    double m = 0.0;
-    for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(v64[e]));
-    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if (m == 0.0) m = 1.0;
-    inv = 1.0 / m;
-    if (lane == 0 && br < n_brows) sc[br] = (float) m;
-  }
-  const int64_t p = br / rpw, lbr = br - p * rpw;
-  auto slot_block = [&](int64_t j) { return (p * sl_j + j) * rpw + lbr; };
-  for (int64_t e = lane; e < (int64_t) sl_j * bs2; e += 64) {
-    const int64_t j = e / bs2, q = e - j * bs2;
-    OT o = (OT) 0.0f;
-    if (j < len) {
-      if constexpr (sizeof(OT) == 2) o = __float2half_rn((float) (v64[e0 + e] * inv));
-      else o = (OT) v64[e0 + e];
+  constexpr int RPW = 64 / BS, BS2 = BS * BS;
+  const int n_rows = ((n_brows + RPW - 1) / RPW * RPW * sl_j) * BS;  // rows of all sliced blocks
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < n_rows; t += gridDim.x * 256) {
+    // sliced block sb = (P*J + j)*RPW + lbr holds block j of block row P*RPW + lbr
+    const int sb = t / BS, r = t - sb * BS;
+    const int pj = sb / RPW, lbr = sb - pj * RPW, pp = pj / sl_j, j = pj - pp * sl_j, br = pp * RPW + lbr;
+    int b0 = 0, len = 0;
+    if (br < n_brows) {
+      b0 = rp[br];
+      len = rp[br + 1] - b0;
     }
-    out[slot_block(j) * bs2 + q] = o;
+    if (r == 0) cols_sl[sb] = j < len ? cols[b0 + j] : len > 0 ? cols[b0 + len - 1] : 0;
+    OT o[BS];
+    if (j < len) {
+      const double inv = sizeof(OT) == 2 ? 1.0 / (double) sc[br] : 1.0;
+      double v[BS];
+      load_n<double, BS, lowbit(8 * BS)>(v64 + ((int64_t) (b0 + j) * BS2 + r * BS), v);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) o[c] = to_inner<OT>(v[c], inv);
+    } else {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) o[c] = (OT) 0.0f;
+    }
+    store_words<OT, BS>(out + (int64_t) t * BS, o);
   }
-  const int pad = len > 0 ? cols[b0 + len - 1] : 0;
-  for (int j = lane; j < sl_j; j += 64) cols_sl[slot_block(j)] = j < len ? cols[b0 + j] : pad;
 }
 
 // Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64

@@ -213,7 +213,9 @@ __device__ __forceinline__ void store_staged_blocks(const double *blk, int nb, b
   }
 }
 
-// k_gen_stencil_rows_t with the stores coalesced, for bs 4, 6 and 8.  There a
+// k_gen_stencil_rows_t with the stores coalesced, for bs 4, 6 and 8; with
+// `scale`, also each block row's largest |entry| rounded to FP32 (1 for an
+// all-zero row), the scale of an FP16 copy.  There a
 // lane's 16-byte stores of its block row still leave every store instruction
 // spread over one cache line per row.  A wavefront holds NPW = 64/BS
 // consecutive nodes; for each neighbour s the lanes put their block rows in
@@ -225,7 +227,8 @@ __device__ __forceinline__ void store_staged_blocks(const double *blk, int nb, b
 template <int BS>
 __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int kind, uint64_t seed,
                                                             const int *__restrict__ rp, int *__restrict__ cols,
-                                                            double *__restrict__ vals, float *__restrict__ vals32)
+                                                            double *__restrict__ vals, float *__restrict__ vals32,
+                                                            float *__restrict__ scale)
 {
 #pragma clang fp contract(off)
   constexpr int NPW = 64 / BS, BS2 = BS * BS;
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
   double *blk = blk_s[wib];
   int nb = rp[i], diag_nb = 0;
   double dg[BS];
-  double sum = 0.0;
+  double sum = 0.0, mx = 0.0;  // mx: largest |stored entry| of the lane's row
   for (int s = 0; s < 9; s++) {
     const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
     const bool present = on && jy >= 0 && jy < ny && jx >= 0 && jx < nx;
@@ -277,7 +280,10 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
     } else {
       if (present) {
 #pragma unroll
-        for (int c = 0; c < BS; c++) blk[q * BS2 + r * BS + c] = v[c];
+        for (int c = 0; c < BS; c++) {
+          blk[q * BS2 + r * BS + c] = v[c];
+          mx = fmax(mx, fabs(v[c]));
+        }
       }
       wave_lds_sync();
       store_staged_blocks<BS>(blk, nb, present, vals, vals32, lane);
@@ -290,17 +296,26 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
     for (int c = 0; c < BS; c++) {
       if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
       blk[q * BS2 + r * BS + c] = dg[c];
+      mx = fmax(mx, fabs(dg[c]));
     }
   }
   wave_lds_sync();
   store_staged_blocks<BS>(blk, diag_nb, on, vals, vals32, lane);
+  if (scale) {
+    // the node's BS rows are lanes q*BS .. q*BS + BS - 1
+    double m = mx;
+#pragma unroll
+    for (int k = 1; k < BS; k++) m = fmax(m, __shfl(mx, (q * BS + k) & 63));
+    if (on && r == 0) scale[i] = m == 0.0 ? 1.0f : (float) m;
+  }
 }
 
 // The generator for any block size: the register-row kernel for the block
 // sizes the request loop serves (coalesced through LDS for 4, 6 and 8), the
-// generic one otherwise.
+// generic one otherwise.  `scale` (FP16 block-row scales) is written for
+// bs 4, 6 and 8 only.
 inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
-                               double *vals, float *vals32, hipStream_t s)
+                               double *vals, float *vals32, hipStream_t s, float *scale = nullptr)
 {
   const unsigned g = (unsigned) (((int64_t) nx * ny * bs + 255) / 256);
   auto waves4 = [&](int npw) { return (unsigned) (((int64_t) nx * ny + 4 * npw - 1) / (4 * npw)); };
@@ -308,9 +323,9 @@ inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, 
   case 1: k_gen_stencil_rows_t<1><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
   case 2: k_gen_stencil_rows_t<2><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
   case 3: k_gen_stencil_rows_t<3><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 4: k_gen_stencil_rows_c<4><<<waves4(16), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 6: k_gen_stencil_rows_c<6><<<waves4(10), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 8: k_gen_stencil_rows_c<8><<<waves4(8), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
+  case 4: k_gen_stencil_rows_c<4><<<waves4(16), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
+  case 6: k_gen_stencil_rows_c<6><<<waves4(10), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
+  case 8: k_gen_stencil_rows_c<8><<<waves4(8), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
   default: k_gen_stencil_rows<<<g, 256, 0, s>>>(nx, ny, bs, kind, seed, rp, cols, vals, vals32); break;
   }
 }
