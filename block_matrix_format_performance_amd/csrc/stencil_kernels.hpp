@@ -113,6 +113,18 @@ template <class T, int N> __device__ __forceinline__ void store_row(T *p, const 
   }
 }
 
+// An FP32 copy written beside the FP64 blocks in the pass-sliced order of the
+// request loop's inner matrices: block j of node i is sliced block
+// ((i / RPW) * j_slots + j) * RPW + i % RPW (RPW = 64 / BS nodes per pass),
+// rows are padded to j_slots blocks with zero blocks whose column is the
+// row's last one, and the nodes past the grid in the last pass get zero
+// blocks and column 0.  vals == null: no such copy.
+struct SlicedOut {
+  float *vals = nullptr;
+  int *cols = nullptr;
+  int j_slots = 0;
+};
+
 // k_gen_stencil_rows for a compile-time block size: the same draws, the same
 // arithmetic and the same order of the diagonal row sum, with each block row
 // held in registers and written by up-to-16-byte stores (the generic kernel
@@ -121,25 +133,46 @@ template <class T, int N> __device__ __forceinline__ void store_row(T *p, const 
 template <int BS>
 __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int kind, uint64_t seed,
                                                             const int *__restrict__ rp, int *__restrict__ cols,
-                                                            double *__restrict__ vals, float *__restrict__ vals32)
+                                                            double *__restrict__ vals, float *__restrict__ vals32,
+                                                            SlicedOut sl = SlicedOut{})
 {
 #pragma clang fp contract(off)
+  constexpr int RPW = 64 / BS;
+  const int64_t n_nodes = (int64_t) nx * ny;
   const int64_t t64 = (int64_t) blockIdx.x * 256 + threadIdx.x;
-  if (t64 >= (int64_t) nx * ny * BS) return;
+  const int64_t n_pad = sl.vals ? (n_nodes + RPW - 1) / RPW * RPW : n_nodes;
+  if (t64 >= n_pad * BS) return;
   const int t = (int) t64;
   const int i = t / BS;
   const int r = t - i * BS;
+  // the row of slot j of node i in the sliced copy
+  auto sl_row = [&](int j) { return sl.vals + (((int64_t) (i / RPW) * sl.j_slots + j) * RPW + i % RPW) * (BS * BS) + r * BS; };
+  auto sl_col = [&](int j) { return sl.cols + ((int64_t) (i / RPW) * sl.j_slots + j) * RPW + i % RPW; };
+  if (i >= n_nodes) {  // a node past the grid in the last pass: zero blocks
+    float z[BS] = {};
+    for (int j = 0; j < sl.j_slots; j++) {
+      store_row<float, BS>(sl_row(j), z);
+      if (r == 0) *sl_col(j) = 0;
+    }
+    return;
+  }
   const int iy = i / nx, ix = i - iy * nx;
   const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
   constexpr int64_t bs2 = (int64_t) BS * BS;
-  int64_t nb = rp[i];
+  const int64_t nb0 = rp[i];
+  int64_t nb = nb0;
   int64_t diag_at = 0;
+  int diag_j = 0, last_col = 0;
   double dg[BS];
   double sum = 0.0;
   for (int s = 0; s < 9; s++) {
     const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
     if (jy < 0 || jy >= ny || jx < 0 || jx >= nx) continue;
-    if (r == 0) cols[nb] = jy * nx + jx;
+    last_col = jy * nx + jx;
+    if (r == 0) {
+      cols[nb] = last_col;
+      if (sl.vals) *sl_col((int) (nb - nb0)) = last_col;
+    }
     const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) bs2;
     const int64_t at = nb * bs2 + (int64_t) r * BS;
     double v[BS];
@@ -161,15 +194,17 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
     }
     if (s == 4) {
       diag_at = at;
+      diag_j = (int) (nb - nb0);
 #pragma unroll
       for (int c = 0; c < BS; c++) dg[c] = v[c];
     } else {
       store_row<double, BS>(vals + at, v);
-      if (vals32) {
+      if (vals32 || sl.vals) {
         float f[BS];
 #pragma unroll
         for (int c = 0; c < BS; c++) f[c] = (float) v[c];
-        store_row<float, BS>(vals32 + at, f);
+        if (vals32) store_row<float, BS>(vals32 + at, f);
+        if (sl.vals) store_row<float, BS>(sl_row((int) (nb - nb0)), f);
       }
     }
     nb++;
@@ -178,11 +213,19 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
   for (int c = 0; c < BS; c++)
     if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
   store_row<double, BS>(vals + diag_at, dg);
-  if (vals32) {
+  if (vals32 || sl.vals) {
     float f[BS];
 #pragma unroll
     for (int c = 0; c < BS; c++) f[c] = (float) dg[c];
-    store_row<float, BS>(vals32 + diag_at, f);
+    if (vals32) store_row<float, BS>(vals32 + diag_at, f);
+    if (sl.vals) store_row<float, BS>(sl_row(diag_j), f);
+  }
+  if (sl.vals) {  // padding slots
+    float z[BS] = {};
+    for (int j = (int) (nb - nb0); j < sl.j_slots; j++) {
+      store_row<float, BS>(sl_row(j), z);
+      if (r == 0) *sl_col(j) = last_col;
+    }
   }
 }
 
@@ -228,7 +271,7 @@ template <int BS>
 __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int kind, uint64_t seed,
                                                             const int *__restrict__ rp, int *__restrict__ cols,
                                                             double *__restrict__ vals, float *__restrict__ vals32,
-                                                            float *__restrict__ scale)
+                                                            float *__restrict__ scale, SlicedOut sl = SlicedOut{})
 {
 #pragma clang fp contract(off)
   constexpr int NPW = 64 / BS, BS2 = BS * BS;
@@ -244,7 +287,13 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
   const int iy = i / nx, ix = i - iy * nx;
   const uint64_t ka = rng_key(seed, kind == 0 ? STREAM_DD : STREAM_K), kc = rng_key(seed, STREAM_C);
   double *blk = blk_s[wib];
-  int nb = rp[i], diag_nb = 0;
+  const int nb0 = rp[i];
+  int nb = nb0, diag_nb = 0, last_col = 0;
+  // the wavefront is one pass of the sliced copy: lane (q, r) writes row r of
+  // its node's blocks there (nodes past the grid: zero blocks)
+  const bool sl_on = sl.vals && q < NPW;
+  auto sl_row = [&](int j) { return sl.vals + (((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q) * BS2 + r * BS; };
+  auto sl_col = [&](int j) { return sl.cols + ((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q; };
   double dg[BS];
   double sum = 0.0, mx = 0.0;  // mx: largest |stored entry| of the lane's row
   for (int s = 0; s < 9; s++) {
@@ -254,7 +303,11 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
 #pragma unroll
     for (int c = 0; c < BS; c++) v[c] = 0.0;
     if (present) {
-      if (r == 0) cols[nb] = jy * nx + jx;
+      last_col = jy * nx + jx;
+      if (r == 0) {
+        cols[nb] = last_col;
+        if (sl_on) *sl_col(nb - nb0) = last_col;
+      }
       const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) BS2;
 #pragma unroll
       for (int c = 0; c < BS; c++) {
@@ -284,6 +337,12 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
           blk[q * BS2 + r * BS + c] = v[c];
           mx = fmax(mx, fabs(v[c]));
         }
+        if (sl_on) {
+          float f[BS];
+#pragma unroll
+          for (int c = 0; c < BS; c++) f[c] = (float) v[c];
+          store_row<float, BS>(sl_row(nb - nb0), f);
+        }
       }
       wave_lds_sync();
       store_staged_blocks<BS>(blk, nb, present, vals, vals32, lane);
@@ -301,6 +360,21 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
   }
   wave_lds_sync();
   store_staged_blocks<BS>(blk, diag_nb, on, vals, vals32, lane);
+  if (sl_on) {
+    float f[BS];
+    if (on) {
+#pragma unroll
+      for (int c = 0; c < BS; c++) f[c] = (float) dg[c];
+      store_row<float, BS>(sl_row(diag_nb - nb0), f);
+    }
+    // padding slots (all of them for a node past the grid)
+#pragma unroll
+    for (int c = 0; c < BS; c++) f[c] = 0.0f;
+    for (int j = on ? nb - nb0 : 0; j < sl.j_slots; j++) {
+      store_row<float, BS>(sl_row(j), f);
+      if (r == 0) *sl_col(j) = on ? last_col : 0;
+    }
+  }
   if (scale) {
     // the node's BS rows are lanes q*BS .. q*BS + BS - 1
     double m = mx;
@@ -313,19 +387,22 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
 // The generator for any block size: the register-row kernel for the block
 // sizes the request loop serves (coalesced through LDS for 4, 6 and 8), the
 // generic one otherwise.  `scale` (FP16 block-row scales) is written for
-// bs 4, 6 and 8 only.
+// bs 4, 6 and 8 only; `sl` (a pass-sliced FP32 copy) for bs 1, 2, 3, 4, 6, 8.
 inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
-                               double *vals, float *vals32, hipStream_t s, float *scale = nullptr)
+                               double *vals, float *vals32, hipStream_t s, float *scale = nullptr,
+                               SlicedOut sl = SlicedOut{})
 {
-  const unsigned g = (unsigned) (((int64_t) nx * ny * bs + 255) / 256);
+  // threads of the register-row kernels: the nodes of whole passes with a sliced copy
+  const int64_t nodes = sl.vals ? ((int64_t) nx * ny + 64 / bs - 1) / (64 / bs) * (64 / bs) : (int64_t) nx * ny;
+  const unsigned g = (unsigned) ((nodes * bs + 255) / 256);
   auto waves4 = [&](int npw) { return (unsigned) (((int64_t) nx * ny + 4 * npw - 1) / (4 * npw)); };
   switch (bs) {
-  case 1: k_gen_stencil_rows_t<1><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 2: k_gen_stencil_rows_t<2><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 3: k_gen_stencil_rows_t<3><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32); break;
-  case 4: k_gen_stencil_rows_c<4><<<waves4(16), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
-  case 6: k_gen_stencil_rows_c<6><<<waves4(10), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
-  case 8: k_gen_stencil_rows_c<8><<<waves4(8), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale); break;
+  case 1: k_gen_stencil_rows_t<1><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, sl); break;
+  case 2: k_gen_stencil_rows_t<2><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, sl); break;
+  case 3: k_gen_stencil_rows_t<3><<<g, 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, sl); break;
+  case 4: k_gen_stencil_rows_c<4><<<waves4(16), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale, sl); break;
+  case 6: k_gen_stencil_rows_c<6><<<waves4(10), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale, sl); break;
+  case 8: k_gen_stencil_rows_c<8><<<waves4(8), 256, 0, s>>>(nx, ny, kind, seed, rp, cols, vals, vals32, scale, sl); break;
   default: k_gen_stencil_rows<<<g, 256, 0, s>>>(nx, ny, bs, kind, seed, rp, cols, vals, vals32); break;
   }
 }
