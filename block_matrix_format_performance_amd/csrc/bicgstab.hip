@@ -257,7 +257,7 @@ static int launch_iterations(bmfp_solver_t &S, int iters, hipStream_t s)
 }
 
 int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
-               bmfp_solve_info *info, bool true_residual)
+               bmfp_solve_info *info, bool true_residual, int first_batch, bool timed)
 {
   if (!(eps > 0.0) || max_iterations < 0) return fail(BMFP_ERR_INVALID, "eps must be > 0, max_iterations >= 0");
   if (S.n == 0) {
@@ -278,7 +278,8 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   // decrease so far predicts are left, plus the one whose first kernel sees
   // the stop (doubling while the residual has not decreased), launched as
   // graphs of 8, 4, 2 and 1 iterations.
-  int want = GRAPH_ITERS;
+  int want = first_batch > 0 ? std::min(first_batch, 8 * GRAPH_ITERS) : GRAPH_ITERS;
+  want = std::max(1, std::min(want, max_iterations + 1));
   for (;;) {
     BMFP_TRY(launch_iterations(S, want, s));
     BMFP_HIP(hipMemcpyAsync(S.h_state, S.state.ptr, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -297,10 +298,12 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   }
   const size_t vb = vec_size(S.A->dtype);
   BMFP_HIP(hipMemcpyAsync(x, S.vecs.ptr, (size_t) S.n * vb, hipMemcpyDeviceToDevice, s));
-  BMFP_HIP(hipEventRecord(S.ev1, s));
-  BMFP_HIP(hipEventSynchronize(S.ev1));
   float ms = 0.f;
-  BMFP_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  if (timed) {
+    BMFP_HIP(hipEventRecord(S.ev1, s));
+    BMFP_HIP(hipEventSynchronize(S.ev1));
+    BMFP_HIP(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+  }
   const SolverState &st = *S.h_state;
   bmfp_solve_info out{};
   out.iterations = st.iter;

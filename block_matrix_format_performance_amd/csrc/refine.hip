@@ -244,6 +244,7 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
   BMFP_HIP(hipMemcpyAsync(R.ctl.ptr, R.h_ctl, sizeof(RefineCtl), hipMemcpyHostToDevice, s));
   int xin = 0;
   int status = BMFP_ERR_NOT_CONVERGED;
+  int hint = 0;  // launch batch of the next inner solve: the previous one's iterations + 1
   for (int k = 0; k <= max_outer; k++) {
     if (k == 0) {
       // x_0 = 0: r = b without reading the matrix
@@ -278,15 +279,16 @@ int refine_run(RefineState &R, const double *b, double *x, double eps, double in
       out.escalations++;
     }
     bmfp_solve_info in{};
-    int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false);
+    int st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false, hint, false);
     out.inner_iterations += in.iterations;
     if ((st == BMFP_ERR_BREAKDOWN || st == BMFP_ERR_NONFINITE) && R.dtype == BMFP_F16) {
       BMFP_TRY(ensure_level(R, BMFP_F32));
       R.dtype = BMFP_F32;
       out.escalations++;
-      st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false);
+      st = solver_run(*R.inner[R.dtype], R.r32.ptr, R.d32.ptr, inner_tol, max_inner, s, &in, false, 0, false);
       out.inner_iterations += in.iterations;
     }
+    hint = in.iterations + 1;
     if (st == BMFP_ERR_NONFINITE) {
       status = st;
       break;

@@ -146,22 +146,33 @@ __global__ __launch_bounds__(256) void k_convert_flat(int64_t nv, const double *
 }
 
 // Same layout with the per-block-row FP16 scale: the row's max |entry|, then
-// every element scaled in place order (no index arithmetic).
+// every element scaled in place order (no index arithmetic).  A block row of
+// up to 576 values (9 blocks of 8x8) is read once, 9 values per lane loaded
+// together; longer rows are read a second time for the conversion.
 __global__ __launch_bounds__(256) void k_convert_scaled(int n_brows, int64_t bs2, const int *__restrict__ rp,
                                                         const double *__restrict__ src, __half *__restrict__ dst,
                                                         float *__restrict__ scale)
 {
+  constexpr int K = 9;
   const int64_t br = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (br >= n_brows) return;
   const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = e0 + lane + 64 * k < e1 ? src[e0 + lane + 64 * k] : 0.0;
   double m = 0.0;
-  for (int64_t e = e0 + lane; e < e1; e += 64) m = fmax(m, fabs(src[e]));
+#pragma unroll
+  for (int k = 0; k < K; ++k) m = fmax(m, fabs(v[k]));
+  for (int64_t e = e0 + lane + 64 * K; e < e1; e += 64) m = fmax(m, fabs(src[e]));
   for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   if (m == 0.0) m = 1.0;
   const double inv = 1.0 / m;
   if (lane == 0) scale[br] = (float) m;
-  for (int64_t e = e0 + lane; e < e1; e += 64) dst[e] = __float2half_rn((float) (src[e] * inv));
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (e0 + lane + 64 * k < e1) dst[e0 + lane + 64 * k] = __float2half_rn((float) (v[k] * inv));
+  for (int64_t e = e0 + lane + 64 * K; e < e1; e += 64) dst[e] = __float2half_rn((float) (src[e] * inv));
 }
 
 }  // namespace bmfp

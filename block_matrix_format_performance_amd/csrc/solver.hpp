@@ -34,6 +34,10 @@ struct bmfp_solver_t {
 namespace bmfp {
 size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi);
 int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi);
+// first_batch > 0: iterations launched before the first read of the solver
+// state (a caller's prediction, e.g. the previous solve's count + 1); 0: 8.
+// timed = false: no closing event synchronisation and no info->seconds (the
+// solution copy is still ordered on the solver's stream).
 int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_iterations, hipStream_t user,
-               bmfp_solve_info *info, bool true_residual = true);
+               bmfp_solve_info *info, bool true_residual = true, int first_batch = 0, bool timed = true);
 }  // namespace bmfp

@@ -512,3 +512,31 @@ def test_stream_yardsticks(n16):
     assert np.array_equal(sink.download(), [7.0, 7.0])
     assert L.bmfp_stream_read(src.ptr, 16 * n16 + 8, sink.ptr, None) == bm.ERR_INVALID
     assert L.bmfp_stream_copy(dst.ptr, src.ptr, 8, None) == bm.ERR_INVALID
+
+
+# Values written by a device kernel moments before (the stencil generator,
+# regenerating the matrix in place under new seeds) must be what every family
+# reads, the families with non-temporal value loads included; small grids
+# keep the fresh values inside the caches.
+@pytest.mark.parametrize("bs,layout", [(8, bm.ROW_MAJOR), (12, bm.ROW_MAJOR), (16, bm.ROW_MAJOR),
+                                       (32, bm.ROW_MAJOR), (8, bm.COL_MAJOR), (4, bm.ROW_MAJOR)])
+def test_spmv_reads_freshly_generated_values(bs, layout):
+    nx, ny = 24, 20
+    A = bm.Matrix.stencil(nx, ny, bs, 0, seed=0)
+    x = oracle.rng_unit_vec(5, 4, nx * ny * bs)
+    dx = bm.DeviceArray.from_host(x)
+    dy = bm.DeviceArray(nx * ny * bs)
+    for seed in (11, 12, 13, 14):
+        A.regenerate(0, seed)
+        M = A if layout == bm.ROW_MAJOR else A.convert(bm.F64, bm.COL_MAJOR)
+        for v in (bm.BCSR_GROUP_PER_ROW, bm.SPMV_AUTO, bm.BCSR_THREAD_PER_ROW):
+            M.spmv(dx, dy, v)
+            y = dy.download()
+            m = oracle.gen_stencil9(nx, ny, bs, 0, seed)
+            want = oracle.bcsr_spmv(m, x)
+            tol = F64_TOL * oracle.bcsr_absmv(m, x)
+            bad = np.nonzero(np.abs(y - want) > tol)[0]
+            assert bad.size == 0, (bs, layout, seed, v, bad[:5])
+        if M is not A:
+            M.destroy()
+    A.destroy()
