@@ -24,6 +24,8 @@ MatArgs<VT, AT> mat_args(const bmfp_matrix_t &A)
   m.cols = A.columns->as<int>();
   m.vals = A.values.as<VT>();
   m.scale = SCALED ? A.scale.as<float>() : nullptr;
+  m.cols_sl = A.sl_j > 0 ? A.cols_sl.as<int>() : nullptr;
+  m.sl_j = A.sl_j;
   return m;
 }
 
@@ -55,10 +57,18 @@ SolverVecs<AT> vec_args(const bmfp_solver_t &S, const void *b)
   return V;
 }
 
-template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
+// Block sizes with a pass-sliced split engine: FP32 / FP16 row-major 4, 6, 8
+// (the refinement's inner copies).
+template <class VT, int BS, int LAYOUT> constexpr bool sliceable()
+{
+  return sizeof(VT) < 8 && LAYOUT == BMFP_ROW_MAJOR && (BS == 4 || BS == 6 || BS == 8);
+}
+
+template <class VT, class AT, bool SCALED, int BS, int LAYOUT, bool SL = false>
 struct Ops {
   using Engine = SolverEngine<VT, AT, BS, LAYOUT, SCALED>;
-  using Split = SplitEngine<VT, AT, BS, LAYOUT, SCALED>;
+  using Split = typename std::conditional<SL, SlicedRowsEngine<VT, AT, BS, SCALED>,
+                                          SplitEngine<VT, AT, BS, LAYOUT, SCALED>>::type;
 
   static int groups(const bmfp_matrix_t &A, bool split)
   {
@@ -111,17 +121,36 @@ struct Ops {
   }
 };
 
+template <class VT, class AT, bool SCALED, int BS, int LAYOUT, bool SL>
+void assign_ops(bmfp_solver_t &S)
+{
+  using O = Ops<VT, AT, SCALED, BS, LAYOUT, SL>;
+  S.ops_init = O::init;
+  S.ops_iterate = O::iterate;
+  S.ops_jacobi = O::jacobi;
+  S.np = O::groups(*S.A, S.split);
+  S.spmv_groups = O::spmv_groups;
+}
+
+template <class VT, class AT, bool SCALED, int BS, int LAYOUT>
+bool pick_ops(bmfp_solver_t &S)
+{
+  if (S.A->sl_j > 0) {
+    if constexpr (sliceable<VT, BS, LAYOUT>()) {
+      assign_ops<VT, AT, SCALED, BS, LAYOUT, true>(S);
+      return true;
+    }
+    return false;
+  }
+  assign_ops<VT, AT, SCALED, BS, LAYOUT, false>(S);
+  return true;
+}
+
 template <class VT, class AT, bool SCALED, int LAYOUT>
 bool pick_bs(bmfp_solver_t &S)
 {
-#define BMFP_SOLVER_BS(B)                                        \
-  case B:                                                        \
-    S.ops_init = Ops<VT, AT, SCALED, B, LAYOUT>::init;           \
-    S.ops_iterate = Ops<VT, AT, SCALED, B, LAYOUT>::iterate;     \
-    S.ops_jacobi = Ops<VT, AT, SCALED, B, LAYOUT>::jacobi;       \
-    S.np = Ops<VT, AT, SCALED, B, LAYOUT>::groups(*S.A, S.split); \
-    S.spmv_groups = Ops<VT, AT, SCALED, B, LAYOUT>::spmv_groups;  \
-    return true;
+#define BMFP_SOLVER_BS(B) \
+  case B: return pick_ops<VT, AT, SCALED, B, LAYOUT>(S);
   switch (S.A->bs) {
     BMFP_SOLVER_BS(1)
     BMFP_SOLVER_BS(2)
@@ -179,6 +208,22 @@ int split_schedule(const bmfp_matrix_t &A)
 
 }  // namespace
 
+int sliced_pass_rows(int dtype, int bs)
+{
+  auto pick = [&](auto vt) -> int {
+    using VT = decltype(vt);
+    switch (bs) {
+    case 4: return sliced_rpw<VT, 4>();
+    case 6: return sliced_rpw<VT, 6>();
+    case 8: return sliced_rpw<VT, 8>();
+    default: return 0;
+    }
+  };
+  if (dtype == BMFP_F32) return pick(float{});
+  if (dtype == BMFP_F16) return pick(__half{});
+  return 0;
+}
+
 size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi)
 {
   const int64_t n = A.y_len();
@@ -193,9 +238,11 @@ int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi)
   S.A = A;
   S.use_jacobi = use_jacobi ? 1 : 0;
   S.n = A->y_len();
-  S.split = split_schedule(*A);
+  // a pass-sliced copy runs on the split schedule only, without Jacobi
+  S.split = split_schedule(*A) || A->sl_j > 0;
   if (A->n_rows != A->n_cols) return fail(BMFP_ERR_INVALID, "BiCGStab needs a square matrix");
   if (!A->is_bcsr) return fail(BMFP_ERR_UNSUPPORTED, "BiCGStab runs on BCSR handles (CSR: use bs = 1)");
+  if (A->sl_j > 0 && use_jacobi) return fail(BMFP_ERR_UNSUPPORTED, "no Jacobi preconditioning on a pass-sliced copy");
   bool ok = false;
   switch (A->dtype) {
   case BMFP_F64: ok = pick_layout<double, double, false>(S); break;
@@ -310,8 +357,8 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   out.status = st.status;
   out.rel_residual = st.norm_b2 > 0 ? std::sqrt(st.rr) / std::sqrt(st.norm_b2) : 0.0;
   out.seconds = ms * 1e-3;
-  // true residual ||b - A x|| / ||b|| (after the timed region)
-  if (true_residual) {
+  // true residual ||b - A x|| / ||b|| (after the timed region; BCSR-order values only)
+  if (true_residual && S.A->sl_j == 0) {
     void *y = S.vecs.as<char>() + (size_t) 8 * S.n * vb;  // t buffer
     BMFP_TRY(launch_bcsr(*S.A, auto_variant(*S.A), x, y, s));
     const int ng = std::min<int64_t>((S.n + 255) / 256, SOLVER_MAX_GROUPS);

@@ -22,6 +22,11 @@ struct bmfp_matrix_t {
   bmfp::DeviceBuffer values;                    // dtype[nnzb*bs*bs]
   bmfp::DeviceBuffer scale;                     // float[n_rows] (BMFP_F16 only)
   bmfp::DeviceBuffer merge_carry;               // per-wavefront carries of BMFP_CSR_MERGE (CSR only)
+  // Pass-sliced values (the solver's inner copies only): block j of block row
+  // P*sl_rpw + l at ((P*sl_j + j)*sl_rpw + l), rows padded with zero blocks to
+  // sl_j; its block columns in cols_sl.  sl_j = 0: values in BCSR order.
+  int sl_j = 0, sl_rpw = 0;
+  bmfp::DeviceBuffer cols_sl;
 
   // rocSPARSE state for BMFP_BCSR_LIBRARY, created on first use.
   void *lib_handle = nullptr;
@@ -56,6 +61,10 @@ int auto_variant(const bmfp_matrix_t &A);
 int csr_group_for(double mean_len);
 int csr_stage_group(double mean_len, double max_len, int lo);
 bool bcsr_supported(int variant, int dtype, int layout, int bs);
+// A pass-sliced FP32 / FP16 copy of the F64 row-major BCSR matrix src (layout
+// above, rpw block rows per pass, j_slots slots per row) into m, whose shape
+// fields the caller has filled; FP16 rows scaled by one FP32 factor each.
+int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s);
 // Device address of the allocation-free entry point's scratch on `device`
 // (the current device), resolved once per device.
 int noalloc_setup(int device, void **out);

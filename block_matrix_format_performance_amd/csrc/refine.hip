@@ -170,12 +170,40 @@ bool pick(RefineState &R)
 }
 
 // Creates the inner matrix copy and solver for `dtype` on first use.
+// The inner copy of the matrix in `dtype`: pass-sliced for the split
+// schedule's engine where one exists (row-major bs 4, 6, 8 without Jacobi,
+// at least 32 MB of inner values, padding within a quarter of the blocks),
+// else BCSR order.
 int ensure_level(RefineState &R, int dtype)
 {
   if (R.inner[dtype]) return BMFP_OK;
-  bmfp_matrix out = nullptr;
-  BMFP_TRY(bmfp_matrix_convert(R.A, dtype, R.A->layout, &out, R.stream));
-  R.low[dtype].reset(out);
+  const bmfp_matrix_t &A = *R.A;
+  const int rpw = sliced_pass_rows(dtype, A.bs);
+  const int J = (A.max_row_blocks + 2) / 3 * 3;
+  const int64_t nsb = ((int64_t) A.n_rows + std::max(rpw, 1) - 1) / std::max(rpw, 1) * rpw * J;
+  const bool slice = rpw > 0 && A.layout == BMFP_ROW_MAJOR && !R.use_jacobi && J > 0 &&
+                     (double) A.n_vals() * dtype_size(dtype) >= 32e6 && nsb <= (int64_t) A.nnzb + A.nnzb / 4;
+  if (slice) {
+    std::unique_ptr<bmfp_matrix_t> m(new bmfp_matrix_t);
+    m->is_bcsr = 1;
+    m->dtype = dtype;
+    m->layout = A.layout;
+    m->n_rows = A.n_rows;
+    m->n_cols = A.n_cols;
+    m->bs = A.bs;
+    m->nnzb = A.nnzb;
+    m->max_row_blocks = A.max_row_blocks;
+    m->mean_row_blocks = A.mean_row_blocks;
+    m->device = A.device;
+    m->row_ptr = A.row_ptr;
+    m->columns = A.columns;
+    BMFP_TRY(make_sliced_copy(A, dtype, rpw, J, *m, R.stream));
+    R.low[dtype].reset(m.release());
+  } else {
+    bmfp_matrix out = nullptr;
+    BMFP_TRY(bmfp_matrix_convert(R.A, dtype, R.A->layout, &out, R.stream));
+    R.low[dtype].reset(out);
+  }
   BMFP_HIP(hipStreamSynchronize(R.stream));
   std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
   BMFP_TRY(solver_create(*S, R.low[dtype].get(), R.use_jacobi));

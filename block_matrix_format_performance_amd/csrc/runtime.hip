@@ -1,6 +1,7 @@
 // runtime.hip -- C ABI: errors, device/memory/stream plumbing, matrix handles,
 // SpMV entry points.
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -173,6 +174,112 @@ __global__ __launch_bounds__(256) void k_convert_scaled(int n_brows, int64_t bs2
   for (int k = 0; k < K; ++k)
     if (e0 + lane + 64 * k < e1) dst[e0 + lane + 64 * k] = __float2half_rn((float) (v[k] * inv));
   for (int64_t e = e0 + lane + 64 * K; e < e1; e += 64) dst[e] = __float2half_rn((float) (src[e] * inv));
+}
+
+// FP16 block-row scales of a pass-sliced copy: the row's max |entry| rounded
+// to FP32 (1 for an all-zero row); one wavefront per block row, 9 values per
+// lane loaded together.
+__global__ __launch_bounds__(256) void k_row_scale(int n_brows, int64_t bs2, const int *__restrict__ rp,
+                                                   const double *__restrict__ src, float *__restrict__ scale)
+{
+  constexpr int K = 9;
+  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (br >= n_brows) return;
+  const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  double m = 0.0;
+  for (int64_t e = e0; e < e1; e += 64 * K) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = e + lane + 64 * k < e1 ? src[e + lane + 64 * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) m = fmax(m, fabs(v[k]));
+  }
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (lane == 0) scale[br] = m == 0.0 ? 1.0f : (float) m;
+}
+
+// Pass-sliced copy (layout: bmfp_matrix_t::sl_j): one thread per row of a
+// sliced block (grid-stride), consecutive threads storing consecutive rows;
+// each row read from its block in the FP64 source, zero in padding blocks;
+// the thread of row 0 writes the block's column (padding: the row's last
+// block column, 0 for an empty row).  FP16 values are FP16(FP32(v / scale)).
+template <class OT, int BS>
+__global__ __launch_bounds__(256) void k_slice_rows(int n_brows, int rpw, int sl_j, const int *__restrict__ rp,
+                                                    const int *__restrict__ cols, const double *__restrict__ src,
+                                                    const float *__restrict__ scale, OT *__restrict__ out,
+                                                    int *__restrict__ cols_sl)
+{
+  constexpr int64_t BS2 = BS * BS;
+  const int64_t n_rows = ((int64_t) n_brows + rpw - 1) / rpw * rpw * sl_j * BS;
+  for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < n_rows; t += (int64_t) gridDim.x * 256) {
+    const int64_t sb = t / BS, pj = sb / rpw, pp = pj / sl_j;
+    const int r = (int) (t - sb * BS), lbr = (int) (sb - pj * rpw), j = (int) (pj - pp * sl_j);
+    const int64_t br = pp * rpw + lbr;
+    int b0 = 0, len = 0;
+    if (br < n_brows) {
+      b0 = rp[br];
+      len = rp[br + 1] - b0;
+    }
+    if (r == 0) cols_sl[sb] = j < len ? cols[b0 + j] : len > 0 ? cols[b0 + len - 1] : 0;
+    OT o[BS];
+    if (j < len) {
+      double v[BS];
+      load_n<double, BS, lowbit(8 * BS)>(src + (((int64_t) b0 + j) * BS2 + (int64_t) r * BS), v);
+      if constexpr (sizeof(OT) == 2) {
+        const double inv = 1.0 / (double) scale[br];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) o[c] = __float2half_rn((float) (v[c] * inv));
+      } else {
+#pragma unroll
+        for (int c = 0; c < BS; ++c) o[c] = (OT) v[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) o[c] = (OT) 0.0f;
+    }
+    store_words<OT, BS>(out + t * BS, o);
+  }
+}
+
+int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s)
+{
+  if (src.dtype != BMFP_F64 || src.layout != BMFP_ROW_MAJOR || !src.is_bcsr || rpw <= 0 || j_slots <= 0 ||
+      (dtype != BMFP_F32 && dtype != BMFP_F16))
+    return fail(BMFP_ERR_INVALID, "pass-sliced copy of an F64 row-major BCSR matrix into F32 / F16");
+  const int bs = src.bs;
+  const int64_t nsb = ((int64_t) src.n_rows + rpw - 1) / rpw * rpw * j_slots;
+  m.sl_j = j_slots;
+  m.sl_rpw = rpw;
+  BMFP_TRY(m.values.alloc((size_t) std::max<int64_t>(nsb, 1) * bs * bs * dtype_size(dtype)));
+  BMFP_TRY(m.cols_sl.alloc(sizeof(int) * (size_t) std::max<int64_t>(nsb, 1)));
+  if (dtype == BMFP_F16) BMFP_TRY(m.scale.alloc(sizeof(float) * (size_t) std::max(src.n_rows, 1)));
+  if (src.n_rows == 0) return BMFP_OK;
+  const int *rp = src.row_ptr->as<int>(), *cols = src.columns->as<int>();
+  const double *v64 = src.values.as<double>();
+  if (dtype == BMFP_F16) {
+    k_row_scale<<<(unsigned) (((int64_t) src.n_rows * 64 + 255) / 256), 256, 0, s>>>(src.n_rows, (int64_t) bs * bs,
+                                                                                     rp, v64, m.scale.as<float>());
+    BMFP_LAUNCHED();
+  }
+  const unsigned g = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nsb * bs + 255) / 256, 1 << 20));
+  auto launch_bs = [&](auto bsc) {
+    constexpr int BS = decltype(bsc)::value;
+    if (dtype == BMFP_F16)
+      k_slice_rows<__half, BS><<<g, 256, 0, s>>>(src.n_rows, rpw, j_slots, rp, cols, v64, m.scale.as<float>(),
+                                                 m.values.as<__half>(), m.cols_sl.as<int>());
+    else
+      k_slice_rows<float, BS><<<g, 256, 0, s>>>(src.n_rows, rpw, j_slots, rp, cols, v64, nullptr,
+                                                m.values.as<float>(), m.cols_sl.as<int>());
+  };
+  switch (bs) {
+  case 4: launch_bs(std::integral_constant<int, 4>{}); break;
+  case 6: launch_bs(std::integral_constant<int, 6>{}); break;
+  case 8: launch_bs(std::integral_constant<int, 8>{}); break;
+  default: return fail(BMFP_ERR_UNSUPPORTED, "pass-sliced copies for bs 4, 6, 8 only");
+  }
+  BMFP_LAUNCHED();
+  return BMFP_OK;
 }
 
 }  // namespace bmfp

@@ -987,28 +987,6 @@ __global__ __launch_bounds__(256) void k_serve_f16(int n_brows, int bs, const in
   }
 }
 
-// Stores the N values of v at p in the widest chunks (up to 16 bytes) the
-// row's size allows (p is aligned to the lowest set bit of N*sizeof(T)).
-template <class T, int N> __device__ __forceinline__ void store_words(T *p, const T (&v)[N])
-{
-  constexpr int BYTES = N * (int) sizeof(T);
-  constexpr int CH = lowbit(BYTES) < 16 ? lowbit(BYTES) : 16;
-  char *dst = reinterpret_cast<char *>(p);
-  const char *src = reinterpret_cast<const char *>(v);
-  if constexpr (CH >= 4) {
-    typedef unsigned int w_t __attribute__((ext_vector_type(CH / 4)));
-#pragma unroll
-    for (int c = 0; c < BYTES; c += CH) {
-      w_t t;
-      __builtin_memcpy(&t, src + c, CH);
-      *reinterpret_cast<w_t *>(dst + c) = t;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < N; ++c) p[c] = v[c];
-  }
-}
-
 // Pass-sliced inner copy of a request's FP64 BCSR matrix (layout: see
 // chunk_spmv_sl), FP32, or FP16 with the scales in sc.  One thread per row of
 // a sliced block (grid-stride; 32-bit index arithmetic, the sliced values

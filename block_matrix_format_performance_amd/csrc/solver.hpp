@@ -33,6 +33,9 @@ struct bmfp_solver_t {
 
 namespace bmfp {
 size_t solver_workspace_bytes(const bmfp_matrix_t &A, int use_jacobi);
+// Block rows per pass of the solver's pass-sliced engine for (dtype, bs), 0 if
+// it has none (BMFP_F32 / BMFP_F16, bs 4, 6, 8).
+int sliced_pass_rows(int dtype, int bs);
 int solver_create(bmfp_solver_t &S, bmfp_matrix_t *A, int use_jacobi);
 // first_batch > 0: iterations launched before the first read of the solver
 // state (a caller's prediction, e.g. the previous solve's count + 1); 0: 8.

@@ -254,6 +254,8 @@ struct MatArgs {
   const int *__restrict__ cols;
   const VT *__restrict__ vals;
   const float *__restrict__ scale;
+  const int *__restrict__ cols_sl = nullptr;  // pass-sliced matrices: block columns
+  int sl_j = 0;                                // pass-sliced matrices: block slots per row
 };
 
 // Row engines: walk every scalar row of the matrix in a persistent grid and
@@ -684,6 +686,83 @@ struct RowsEngine {
     const int64_t total = (int64_t) M.n_brows * TPB;
     for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t) gridDim.x * 256)
       item(M, op, epi, t);
+  }
+};
+
+// Rows per thread and block rows per wavefront of SlicedRowsEngine (those of
+// the split schedule's RowsEngine: two rows per thread for FP32 bs 4).
+template <class VT, int BS> constexpr int sliced_rows()
+{
+  return sizeof(VT) == 4 && BS == 4 ? 2 : rows_per_thread<VT, BS>();
+}
+template <class VT, int BS> constexpr int sliced_rpw() { return 64 / (BS / sliced_rows<VT, BS>()); }
+
+// SlicedRowsEngine: RowsEngine over a pass-sliced copy (MatArgs::cols_sl,
+// sl_j).  One wavefront per pass of RPW block rows; lane (lbr, r0) reads R
+// rows of the pass's slot-j block of row lbr, so each load instruction of the
+// wavefront covers one contiguous range (RPW*BS*BS values per slot); every
+// lane runs the same sl_j slots in batches of 3 (sl_j is a multiple of 3;
+// padding blocks are zero).  Products and their order are those of RowsEngine.
+template <class VT, class AT, int BS, bool SCALED>
+struct SlicedRowsEngine {
+  static constexpr int R = sliced_rows<VT, BS>(), TPB = BS / R, RPW = 64 / TPB, LPW = RPW * TPB, U = 3;
+  static int64_t threads(int64_t n_brows) { return (n_brows + RPW - 1) / RPW * 64; }
+  static int64_t groups1(int64_t n_brows) { return (threads(n_brows) + 255) / 256; }
+  template <class OP, class EPI>
+  __device__ __forceinline__ static bool run1(const MatArgs<VT, AT> &M, const OP &op, EPI &&epi,
+                                              const int *stop = nullptr)
+  {
+    constexpr int VALIGN = cmin(lowbit((int) sizeof(VT) * BS * R), 16);
+    constexpr int CC = BS % 8 == 0 ? 8 : BS;
+    constexpr int XALIGN = cmin(lowbit((int) sizeof(AT) * CC), lowbit((int) sizeof(AT) * BS));
+    const int64_t pass = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ln = lane < LPW ? lane : LPW - 1;  // spare lanes repeat the last row's loads
+    const int lbr = ln / TPB, r0 = (ln - lbr * TPB) * R;
+    const int64_t br = pass * RPW + lbr;
+    const bool on = lane < LPW && br < M.n_brows;
+    const int J = M.sl_j;
+    const AT sc = (SCALED && on) ? (AT) M.scale[br] : (AT) 1;
+    AT pre[R], acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      pre[i] = on ? epi.pre(br * BS + r0 + i) : (AT) 0;
+      acc[i] = 0;
+    }
+    if (stop) {  // the stop test, as in RowsEngine::item
+      const int dn = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::"v"(dn));
+      if (__builtin_amdgcn_readfirstlane(dn)) return false;
+    }
+    if (pass * RPW < M.n_brows) {  // uniform: the wavefront's pass holds rows
+      const int64_t s0 = pass * J * RPW;
+      for (int j = 0; j < J; j += U) {
+        int col[U];
+        VT v[U][R * BS];
+        AT xv[U][BS];
+#pragma unroll
+        for (int k = 0; k < U; ++k) col[k] = M.cols_sl[s0 + (int64_t) (j + k) * RPW + lbr];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+          load_n<VT, R * BS, VALIGN>(M.vals + ((s0 + (int64_t) (j + k) * RPW + lbr) * BS + r0) * BS, v[k]);
+#pragma unroll
+        for (int k = 0; k < U; ++k) op.template load<BS, XALIGN>((int64_t) col[k] * BS, xv[k]);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) acc[i] = fmadd(cvt<AT>(v[k][i * BS + c]), xv[k][c], acc[i]);
+      }
+    }
+    if (on) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        if constexpr (SCALED) acc[i] *= sc;
+        epi(br * BS + r0 + i, acc[i], pre[i]);
+      }
+    }
+    return true;
   }
 };
 

@@ -79,6 +79,28 @@ __device__ __forceinline__ void load_n(const VT *__restrict__ p, VT (&out)[N])
   }
 }
 
+// Stores the N values of v at p in the widest chunks (up to 16 bytes) the
+// row's size allows (p is aligned to the lowest set bit of N*sizeof(T)).
+template <class T, int N> __device__ __forceinline__ void store_words(T *p, const T (&v)[N])
+{
+  constexpr int BYTES = N * (int) sizeof(T);
+  constexpr int CH = lowbit(BYTES) < 16 ? lowbit(BYTES) : 16;
+  char *dst = reinterpret_cast<char *>(p);
+  const char *src = reinterpret_cast<const char *>(v);
+  if constexpr (CH >= 4) {
+    typedef unsigned int w_t __attribute__((ext_vector_type(CH / 4)));
+#pragma unroll
+    for (int c = 0; c < BYTES; c += CH) {
+      w_t t;
+      __builtin_memcpy(&t, src + c, CH);
+      *reinterpret_cast<w_t *>(dst + c) = t;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < N; ++c) p[c] = v[c];
+  }
+}
+
 // Maps the hardware workgroup id so that each XCD (workgroups are dealt to
 // the 8 XCDs round robin) processes one contiguous run of tiles: neighbouring
 // block rows then share x segments, column indices and row pointers in the
