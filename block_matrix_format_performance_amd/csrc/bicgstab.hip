@@ -388,6 +388,10 @@ using namespace bmfp;
 
 bmfp_solver_t::~bmfp_solver_t()
 {
+  // A solve that returned early on an error can leave graph launches and the
+  // state copy into h_state queued: let them finish before the graphs, the
+  // pinned state and the stream go.
+  if (stream) (void) hipStreamSynchronize(stream);
   if (refine) refine_destroy(refine);
   for (hipGraphExec_t g : graphs)
     if (g) (void) hipGraphExecDestroy(g);
