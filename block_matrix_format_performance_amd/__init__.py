@@ -133,6 +133,7 @@ def _declare_solver(L):
     L.bmfp_refine_create.argtypes = [ctypes.POINTER(vp), vp, c_int, c_int]
     L.bmfp_refine_solve.argtypes = [vp, vp, vp, c_dbl, c_dbl, c_int, c_int, vp, ctypes.POINTER(SolveInfo)]
     L.bmfp_refine_destroy.argtypes = [vp]
+    L.bmfp_refine_copy_seconds.argtypes = [vp, c_int, ctypes.POINTER(c_dbl)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -532,6 +533,7 @@ class Refine:
         h = ctypes.c_void_p()
         check(lib().bmfp_refine_create(ctypes.byref(h), A64.handle, inner_dtype, 1 if use_jacobi else 0))
         self.handle = h
+        self.inner_dtype = inner_dtype
 
     def solve(self, b, x, eps=1e-10, inner_tol=1e-5, max_outer=50, max_inner=10000, stream=None):
         info = SolveInfo()
@@ -540,6 +542,13 @@ class Refine:
         if st not in SOLVE_STATUSES:
             check(st)
         return info
+
+    def copy_seconds(self, inner_dtype=None):
+        """Device time of the handle's copy of A in inner_dtype (default: the requested one)."""
+        t = ctypes.c_double(0)
+        check(lib().bmfp_refine_copy_seconds(self.handle, self.inner_dtype if inner_dtype is None else inner_dtype,
+                                             ctypes.byref(t)))
+        return t.value
 
     def solve_host(self, b, eps=1e-10, inner_tol=1e-5, max_outer=50, max_inner=10000):
         db = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))

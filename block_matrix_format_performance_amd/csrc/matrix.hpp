@@ -64,7 +64,10 @@ bool bcsr_supported(int variant, int dtype, int layout, int bs);
 // A pass-sliced FP32 / FP16 copy of the F64 row-major BCSR matrix src (layout
 // above, rpw block rows per pass, j_slots slots per row) into m, whose shape
 // fields the caller has filled; FP16 rows scaled by one FP32 factor each.
-int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s);
+// e0 / e1 (optional) are recorded on s around the copy's kernels, after its
+// allocations.
+int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s,
+                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Device address of the allocation-free entry point's scratch on `device`
 // (the current device), resolved once per device.
 int noalloc_setup(int device, void **out);

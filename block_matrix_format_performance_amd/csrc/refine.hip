@@ -95,6 +95,7 @@ struct RefineState {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_sync = nullptr;
   int (*resid)(const RefineState &, const double *b, int xin, hipStream_t) = nullptr;
+  double copy_seconds[3] = {};               // device time of each inner copy (by dtype)
   ~RefineState()
   {
     if (stream) (void) hipStreamSynchronize(stream);
@@ -197,13 +198,20 @@ int ensure_level(RefineState &R, int dtype)
     m->device = A.device;
     m->row_ptr = A.row_ptr;
     m->columns = A.columns;
-    BMFP_TRY(make_sliced_copy(A, dtype, rpw, J, *m, R.stream));
+    BMFP_TRY(make_sliced_copy(A, dtype, rpw, J, *m, R.stream, R.ev0, R.ev1));
     R.low[dtype].reset(m.release());
   } else {
+    // (the span includes the copy's allocation)
+    BMFP_HIP(hipEventRecord(R.ev0, R.stream));
     bmfp_matrix out = nullptr;
     BMFP_TRY(bmfp_matrix_convert(R.A, dtype, R.A->layout, &out, R.stream));
     R.low[dtype].reset(out);
+    BMFP_HIP(hipEventRecord(R.ev1, R.stream));
   }
+  BMFP_HIP(hipEventSynchronize(R.ev1));
+  float ms = 0.f;
+  BMFP_HIP(hipEventElapsedTime(&ms, R.ev0, R.ev1));
+  R.copy_seconds[dtype] = ms * 1e-3;
   BMFP_HIP(hipStreamSynchronize(R.stream));
   std::unique_ptr<bmfp_solver_t> S(new bmfp_solver_t);
   BMFP_TRY(solver_create(*S, R.low[dtype].get(), R.use_jacobi));
@@ -363,6 +371,16 @@ extern "C" int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, doub
                               static_cast<hipStream_t>(stream), info);
     if (st == BMFP_ERR_NOT_CONVERGED) set_error("refinement: max_outer (%d) reached", max_outer);
     return st;
+  });
+}
+
+extern "C" int bmfp_refine_copy_seconds(bmfp_solver s, int inner_dtype, double *seconds)
+{
+  return bmfp::guarded("bmfp_refine_copy_seconds", [&]() -> int {
+    if (!s || !s->refine || !seconds) return fail(BMFP_ERR_INVALID, "not a refinement handle");
+    if (inner_dtype != BMFP_F32 && inner_dtype != BMFP_F16) return fail(BMFP_ERR_INVALID, "bad inner_dtype");
+    *seconds = s->refine->copy_seconds[inner_dtype];
+    return BMFP_OK;
   });
 }
 

@@ -177,26 +177,27 @@ __global__ __launch_bounds__(256) void k_convert_scaled(int n_brows, int64_t bs2
 }
 
 // FP16 block-row scales of a pass-sliced copy: the row's max |entry| rounded
-// to FP32 (1 for an all-zero row); one wavefront per block row, 9 values per
-// lane loaded together.
+// to FP32 (1 for an all-zero row); 16 lanes per block row (4 rows per
+// wavefront), 9 values per lane loaded together.
 __global__ __launch_bounds__(256) void k_row_scale(int n_brows, int64_t bs2, const int *__restrict__ rp,
                                                    const double *__restrict__ src, float *__restrict__ scale)
 {
-  constexpr int K = 9;
-  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (br >= n_brows) return;
-  const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+  constexpr int K = 9, G = 16;
+  const int64_t br = ((int64_t) blockIdx.x * 256 + threadIdx.x) / G;
+  const int g = threadIdx.x & (G - 1);
   double m = 0.0;
-  for (int64_t e = e0; e < e1; e += 64 * K) {
-    double v[K];
+  if (br < n_brows) {
+    const int64_t e0 = (int64_t) rp[br] * bs2, e1 = (int64_t) rp[br + 1] * bs2;
+    for (int64_t e = e0 + g; e < e1; e += G * K) {
+      double v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = e + lane + 64 * k < e1 ? src[e + lane + 64 * k] : 0.0;
+      for (int k = 0; k < K; ++k) v[k] = e + G * k < e1 ? src[e + G * k] : 0.0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) m = fmax(m, fabs(v[k]));
+      for (int k = 0; k < K; ++k) m = fmax(m, fabs(v[k]));
+    }
   }
-  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if (lane == 0) scale[br] = m == 0.0 ? 1.0f : (float) m;
+  for (int off = G / 2; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (br < n_brows && g == 0) scale[br] = m == 0.0 ? 1.0f : (float) m;
 }
 
 // Pass-sliced copy (layout: bmfp_matrix_t::sl_j): one thread per row of a
@@ -242,7 +243,8 @@ __global__ __launch_bounds__(256) void k_slice_rows(int n_brows, int rpw, int sl
   }
 }
 
-int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s)
+int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, bmfp_matrix_t &m, hipStream_t s,
+                     hipEvent_t e0, hipEvent_t e1)
 {
   if (src.dtype != BMFP_F64 || src.layout != BMFP_ROW_MAJOR || !src.is_bcsr || rpw <= 0 || j_slots <= 0 ||
       (dtype != BMFP_F32 && dtype != BMFP_F16))
@@ -254,11 +256,15 @@ int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, 
   BMFP_TRY(m.values.alloc((size_t) std::max<int64_t>(nsb, 1) * bs * bs * dtype_size(dtype)));
   BMFP_TRY(m.cols_sl.alloc(sizeof(int) * (size_t) std::max<int64_t>(nsb, 1)));
   if (dtype == BMFP_F16) BMFP_TRY(m.scale.alloc(sizeof(float) * (size_t) std::max(src.n_rows, 1)));
-  if (src.n_rows == 0) return BMFP_OK;
+  if (e0) BMFP_HIP(hipEventRecord(e0, s));
+  if (src.n_rows == 0) {
+    if (e1) BMFP_HIP(hipEventRecord(e1, s));
+    return BMFP_OK;
+  }
   const int *rp = src.row_ptr->as<int>(), *cols = src.columns->as<int>();
   const double *v64 = src.values.as<double>();
   if (dtype == BMFP_F16) {
-    k_row_scale<<<(unsigned) (((int64_t) src.n_rows * 64 + 255) / 256), 256, 0, s>>>(src.n_rows, (int64_t) bs * bs,
+    k_row_scale<<<(unsigned) (((int64_t) src.n_rows * 16 + 255) / 256), 256, 0, s>>>(src.n_rows, (int64_t) bs * bs,
                                                                                      rp, v64, m.scale.as<float>());
     BMFP_LAUNCHED();
   }
@@ -279,6 +285,7 @@ int make_sliced_copy(const bmfp_matrix_t &src, int dtype, int rpw, int j_slots, 
   default: return fail(BMFP_ERR_UNSUPPORTED, "pass-sliced copies for bs 4, 6, 8 only");
   }
   BMFP_LAUNCHED();
+  if (e1) BMFP_HIP(hipEventRecord(e1, s));
   return BMFP_OK;
 }
 

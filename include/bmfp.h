@@ -288,6 +288,10 @@ int bmfp_refine_create(bmfp_solver *out, bmfp_matrix A64, int inner_dtype,
 int bmfp_refine_solve(bmfp_solver s, const double *b, double *x, double eps,
                       double inner_tol, int max_outer, int max_inner,
                       bmfp_stream stream, bmfp_solve_info *info);
+/* Device time, in seconds, of the copy of A in inner_dtype (BMFP_F32 or
+ * BMFP_F16) the handle made (0 if it has none): the one-time cost to add to
+ * its solves.  No reference counterpart: measurement support. */
+int bmfp_refine_copy_seconds(bmfp_solver s, int inner_dtype, double *seconds);
 int bmfp_refine_destroy(bmfp_solver s);
 
 /* ---- serving: continuous-batching request loop (no reference counterpart) --
