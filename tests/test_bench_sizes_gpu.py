@@ -9,7 +9,10 @@
 - The 2048x1024 bs=4 convective system (n = 8.39 M): the FP64 BiCGStab
   matches the oracle's BiCGStab (bicgstab.cu:388-513 restated) on the same
   right-hand side: same stop, iteration counts within 2, solutions within
-  1e-9 (relative 2-norm).
+  1e-9 (relative 2-norm).  FP64 refinement around FP32 and FP16+scale inner
+  solves (the benchmark's settings, on the pass-sliced inner copy) reaches a
+  true residual below 1e-10, recomputed by the oracle, and the oracle's
+  solution within 1e-9.
 - The 256x256 bs=2 system (b = A*1, eps 1e-8): the same comparison with the
   oracle's solve, and max |x - 1| bounded and equal to the oracle's.
 """
@@ -106,6 +109,23 @@ def test_config2_fp64_bicgstab_full_size(device):
     assert abs(info.iterations - oi.iterations) <= 2
     assert info.rel_residual < 1e-10 and info.true_rel_residual < 2e-10
     assert rel(x, xo) < 1e-9
+    # the benchmark's mixed-precision solves of the same system: FP64
+    # refinement around FP32 (inner tol 1e-5) and FP16+scale (1e-3) inner
+    # solves on the pass-sliced inner copy, true residual recomputed here
+    for inner, tol in ((bm.F32, 1e-5), (bm.F16, 1e-3)):
+        R = bm.Refine(A, inner)
+        assert R.copy_seconds() > 0
+        dx.upload(np.zeros(c.n_rows))
+        ri = R.solve(db, dx, 1e-10, tol, 50, 20000)
+        xr = dx.download()
+        r = b - oracle.csr_spmv(c, xr)
+        true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+        log(f"  refinement inner {inner}: {ri.outer_iterations} outer / {ri.inner_iterations} inner, "
+            f"true residual {true_rel:.3e}, rel(x, x_oracle) {rel(xr, xo):.3e}")
+        assert ri.status == bm.OK and ri.escalations == 0
+        assert true_rel < 1e-10 and abs(true_rel - ri.true_rel_residual) < 1e-11
+        assert rel(xr, xo) < 1e-9
+        R.destroy()
 
 
 def test_config0_solve_full_size(device):

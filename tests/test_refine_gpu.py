@@ -78,6 +78,37 @@ def test_refinement_block_sizes(bs):
     assert np.max(np.abs(x - 1.0)) < 1e-8
 
 
+# Sizes past the 32 MB of inner values at which the refinement makes a
+# pass-sliced inner copy (row-major bs 4, 6 and 8): the sliced split engine
+# runs the inner solves.  (bs, inner, grid side, inner tol)
+SLICED_CASES = [(6, bm.F32, 160, 1e-5), (8, bm.F32, 120, 1e-5),
+                (6, bm.F16, 232, 1e-3), (8, bm.F16, 176, 1e-3), (4, bm.F16, 360, 1e-3)]
+
+
+@pytest.mark.parametrize("bs,inner,side,inner_tol", SLICED_CASES)
+def test_refinement_sliced_inner_copy(bs, inner, side, inner_tol):
+    m = oracle.gen_stencil9(side, side, bs, 1, 7)
+    inner_bytes = m.nnzb * bs * bs * (4 if inner == bm.F32 else 2)
+    assert inner_bytes >= 32e6                      # the sliced copy's size floor
+    c = oracle.bcsr_to_csr(m)
+    xs = oracle.rng_sym_vec(3, 5, c.n_rows)
+    b = oracle.csr_spmv(c, xs)
+    A = bm.Matrix.bcsr(m.row_ptr, m.columns, m.values, m.n_rows, m.n_cols, bs)
+    R = bm.Refine(A, inner)
+    x, info = R.solve_host(b, eps=1e-10, inner_tol=inner_tol, max_outer=60)
+    assert info.status == bm.OK and info.escalations == 0, (info.status, info.outer_iterations)
+    r = b - oracle.csr_spmv(c, x)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert true_rel < 1e-10 and abs(true_rel - info.true_rel_residual) < 1e-11
+    xo, oi = oracle.bicgstab(c, b, 1e-10, 20000)
+    assert oi.status == 0
+    assert rel(x, xo) < 1e-9
+    # the same solve twice: bitwise equal (fixed-order reductions)
+    x2, i2 = R.solve_host(b, eps=1e-10, inner_tol=inner_tol, max_outer=60)
+    assert i2.inner_iterations == info.inner_iterations and np.array_equal(x, x2)
+    R.destroy()
+
+
 def test_fp16_stagnation_escalates_to_fp32():
     # 2x2 block with off-diagonal 1 - 1e-4: FP16 rounds it to 1 (singular),
     # FP32 keeps it; the FP16 inner solves cannot make progress.
