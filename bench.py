@@ -454,28 +454,44 @@ def plain_one_at_a_time(reqs, max_iter, stream, use_jacobi=False):
 SERVE_F16_MIN_BS = 6  # the server's default inner_f16_min_bs
 
 
-def request_bytes_moved(r, outer, inner, jacobi=False, f16_min_bs=SERVE_F16_MIN_BS):
+SERVE_SLOTS = "XRPASBC"
+
+
+def request_bytes_by_slot(r, outer, inner, jacobi=False, f16_min_bs=SERVE_F16_MIN_BS):
     """Compulsory HBM bytes of one served request (right-hand sides share every
-    matrix read), per row and right-hand side unless stated.  Refinement step
-    0 (x = 0): x' written, b read, r32 written (20 B), no matrix.  Later steps:
-    x' = x + d (x, d read, x' written) then the FP64 matrix once with x', b
-    read and r32 written (40 B).  Inner iteration: the FP32 matrix twice and
-    76 B of FP32 vectors (p = ri + beta (p - omega v): 16; v = A p: p, r32
-    read, v written; s = ri - alpha v: 12; t = A s: s read, t written; the
-    updates: s, p, d, t, r32 read, d, ri written), plus the FP32 inverse
-    diagonal read three times per row with Jacobi.  Requests with bs >=
-    f16_min_bs keep the inner matrix in FP16 with one FP32 scale per block
-    row."""
+    matrix read) by slot of the batched iteration, per row and right-hand side
+    unless stated.  Refinement step 0 (x = 0): X writes x' (8 B), R reads b
+    and writes r32 (12 B), no matrix.  Later steps: X x' = x + d (x, d read,
+    x' written: 20 B), then R reads the FP64 matrix once with x', b read and
+    r32 written (20 B).  Inner iteration: the inner matrix twice (A, B) and
+    76 B of FP32 vectors: P p = ri + beta (p - omega v): 16; A v = A p: p, r32
+    read, v written: 12; S s = ri - alpha v: 12; B t = A s: s read, t written:
+    8; C the updates: s, p, d, t, r32 read, d, ri written: 28; with Jacobi
+    the FP32 inverse diagonal read once per row in A, B and C.  Requests with
+    bs >= f16_min_bs keep the inner matrix in FP16 with one FP32 scale per
+    block row.  Matrix index bytes: block columns and row pointers."""
     side, bs, nr = r["side"], r["bs"], r["nrhs"]
     nodes = side * side
     nnzb = (3 * side - 2) ** 2
     n = nodes * bs
     idx = 4.0 * nnzb + 4.0 * (nodes + 1)
-    first = nr * n * 20.0
-    resid = 8.0 * nnzb * bs * bs + idx + nr * n * 40.0
+    steps = max(outer - 1, 0)
+    first = 1.0 if outer > 0 else 0.0
     inner_vals = 2.0 * nnzb * bs * bs + 4.0 * nodes if (f16_min_bs and bs >= f16_min_bs) else 4.0 * nnzb * bs * bs
-    inner_it = 2 * (inner_vals + idx) + nr * n * 76.0 + (12.0 * n if jacobi else 0.0)
-    return (first if outer > 0 else 0.0) + max(outer - 1, 0) * resid + inner * inner_it
+    jac = 4.0 * n if jacobi else 0.0
+    return {"X": first * nr * n * 8.0 + steps * nr * n * 20.0,
+            "R": first * nr * n * 12.0 + steps * (8.0 * nnzb * bs * bs + idx + nr * n * 20.0),
+            "P": inner * nr * n * 16.0,
+            "A": inner * (inner_vals + idx + nr * n * 12.0 + jac),
+            "S": inner * nr * n * 12.0,
+            "B": inner * (inner_vals + idx + nr * n * 8.0 + jac),
+            "C": inner * (nr * n * 28.0 + jac)}
+
+
+def request_bytes_moved(r, outer, inner, jacobi=False, f16_min_bs=SERVE_F16_MIN_BS):
+    """Compulsory HBM bytes of one served request: the sum over the slots of
+    request_bytes_by_slot."""
+    return sum(request_bytes_by_slot(r, outer, inner, jacobi, f16_min_bs).values())
 
 
 def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
@@ -492,6 +508,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
     ids = {}
     xs = {}
     worst_res, outer, inner, moved = 0.0, [], [], 0.0
+    by_slot = dict.fromkeys(SERVE_SLOTS, 0.0)
     i = done = 0
     t0 = time.perf_counter()
     t_last = 0.0
@@ -517,9 +534,11 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
             worst_res = max(worst_res, res.rel_residual)
             outer.append(res.outer_iterations)
             inner.append(res.inner_iterations)
-            moved += request_bytes_moved(reqs[j], res.outer_iterations, res.inner_iterations,
-                                         bool(cfg.get("use_jacobi", 0)),
-                                         cfg.get("inner_f16_min_bs", SERVE_F16_MIN_BS))
+            for k, v in request_bytes_by_slot(reqs[j], res.outer_iterations, res.inner_iterations,
+                                              bool(cfg.get("use_jacobi", 0)),
+                                              cfg.get("inner_f16_min_bs", SERVE_F16_MIN_BS)).items():
+                by_slot[k] += v
+                moved += v
             if ref_x is not None:
                 xs[j] = x
             done += 1
@@ -540,6 +559,7 @@ def serve_replay(reqs, arrivals, lam, cfg, ref_x=None):
            "gpu_busy_frac": round(busy / span, 3),
            "busy_GBs": round(moved / busy / 1e9, 1),
            "compulsory_GB": round(moved / 1e9, 2),
+           "compulsory_GB_by_slot": {k: round(v / 1e9, 2) for k, v in by_slot.items()},
            "max_rel_residual": worst_res,
            "mean_outer": round(float(np.mean(outer)), 2), "mean_inner": round(float(np.mean(inner)), 1),
            "schedule": {"slot_groups": schedule[0], "xcd_run": schedule[1], "chunks_per_tile": schedule[2]}}

@@ -69,6 +69,7 @@ enum : int { REQ_EMPTY = 0, REQ_RESID = 1, REQ_INNER_A = 2, REQ_INNER_B = 3, REQ
 struct ReqDesc {
   int bs, n_brows, nrhs, chunk0, n_chunks;
   int sl_j;   // > 0: the inner matrix is pass-sliced with sl_j block slots per row; 0: BCSR order
+  int sl64;   // 1: v64 is pass-sliced like the inner matrix (same slots and cols_sl); 0: BCSR order
   int64_t n;  // scalar rows of one right-hand side
   const int *rp;
   const int *cols;
@@ -336,16 +337,18 @@ __device__ __forceinline__ void chunk_spmv(const ReqDesc &D, gptr<const VT> vals
 // RPW separate blocks, and every lane of the wavefront runs the same J
 // iterations.  Products and their order are those of the BCSR path (a zero
 // block adds 0 * x).
+// Without use_matrix the product is 0 (no block is read).
 template <class VT, class AT, int BS, int NR, class OWNT, class GATHER, class OWN, class EPI>
-__device__ __forceinline__ void chunk_spmv_sl(const ReqDesc &D, gptr<const VT> vals, int br0, const GATHER &gather,
-                                              const OWN &own, const EPI &epi, gptr<const float> scale = nullptr)
+__device__ __forceinline__ void chunk_spmv_sl(const ReqDesc &D, gptr<const VT> vals, int br0, bool use_matrix,
+                                              const GATHER &gather, const OWN &own, const EPI &epi,
+                                              gptr<const float> scale = nullptr)
 {
   constexpr int RPW = 64 / BS, LPW = RPW * BS, U = 3;
   const int lane = threadIdx.x & 63;
   const int ln = lane < LPW ? lane : LPW - 1;  // spare lanes repeat the last row's loads
   const int lbr = ln / BS, r = ln - lbr * BS, base_lane = lbr * BS;
   const gptr<const int> cs = glb(D.cols_sl);
-  const int J = D.sl_j;  // a multiple of U
+  const int J = use_matrix ? D.sl_j : 0;  // a multiple of U
   // passes of this chunk that hold rows (uniform)
   const int passes = min(SERVE_CHUNK_PASSES, (D.n_brows - br0 + RPW - 1) / RPW);
   // operand offsets of slots q0..q0+U-1 of the chunk's slot sequence (pass
@@ -357,7 +360,7 @@ __device__ __forceinline__ void chunk_spmv_sl(const ReqDesc &D, gptr<const VT> v
     for (int u = 0; u < U; ++u) xo[u] = (uint32_t) (ld(cs, (uint32_t) (sb + u * RPW)) * BS + r) * NR;
   };
   uint32_t xo[U];
-  if (passes > 0) load_xo(0, xo);
+  if (passes > 0 && J > 0) load_xo(0, xo);
 #pragma unroll 1
   for (int ps = 0; ps < passes; ++ps) {
     const int br = br0 + ps * RPW + lbr;
@@ -444,7 +447,10 @@ template <int NR> struct ResidOwn {
 };
 
 // r = b - A64 x', r32 = r; partials ||r||^2 (FP64) and ||r32||^2.  With x = 0
-// (step 0) the matrix is not read: r = b.
+// (step 0) the matrix is not read: r = b.  A pass-sliced FP64 copy (stencil
+// requests) is read slot by slot; its products and their order are those of
+// the BCSR path, which reads one block at a time (batched BCSR loads beside
+// the sliced path cost the kernel a wave per SIMD at bs 6).
 template <int BS, int NR>
 __device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S, int br0, Acc &acc)
 {
@@ -452,20 +458,21 @@ __device__ __forceinline__ void resid_chunk(const ReqDesc &D, const ReqState &S,
   const bool first = S.outer == 0;
   const gptr<const double> xn = glb((const double *) D.x[1 - par]), b = glb(D.b);
   const gptr<float> r32 = glb(D.r32);
-  chunk_spmv<double, double, BS, NR, ResidOwn<NR>, (BS >= 6 ? 3 : 0)>(
-      D, glb(D.v64), br0, !first, [&](uint32_t o, double (&x)[NR]) { ldv<double, NR>(xn, o, x); },
-      [&](uint32_t o, ResidOwn<NR> &m) { ldv<double, NR>(b, o, m.b); },
-      [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
-        float rw[NR];
+  auto gather = [&](uint32_t o, double (&x)[NR]) { ldv<double, NR>(xn, o, x); };
+  auto own = [&](uint32_t o, ResidOwn<NR> &m) { ldv<double, NR>(b, o, m.b); };
+  auto epi = [&](uint32_t o, const double (&y)[NR], const ResidOwn<NR> &m) {
+    float rw[NR];
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-          const double ri = m.b[k] - y[k];
-          rw[k] = (float) ri;
-          acc[k][0] += ri * ri;
-          acc[k][1] += (double) rw[k] * (double) rw[k];
-        }
-        stv<float, NR>(r32, o, rw);
-      });
+    for (int k = 0; k < NR; ++k) {
+      const double ri = m.b[k] - y[k];
+      rw[k] = (float) ri;
+      acc[k][0] += ri * ri;
+      acc[k][1] += (double) rw[k] * (double) rw[k];
+    }
+    stv<float, NR>(r32, o, rw);
+  };
+  if (D.sl64) chunk_spmv_sl<double, double, BS, NR, ResidOwn<NR>>(D, glb(D.v64), br0, !first, gather, own, epi);
+  else chunk_spmv<double, double, BS, NR, ResidOwn<NR>, 1>(D, glb(D.v64), br0, !first, gather, own, epi);
 }
 
 // p = ri + beta (p - omega v) for iterations >= 1 (iteration 0 uses p = r32,
@@ -525,11 +532,11 @@ __device__ __forceinline__ void inner_a_chunk(const ReqDesc &D, const ReqState &
   };
   if constexpr (BS >= SERVE_F16_MIN_BS)
     if (D.v16) {
-      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, RhOwn<NR>>(D, glb(D.v16), br0, gather, own, epi, glb(D.sc16));
+      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, RhOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       else chunk_spmv<__half, float, BS, NR, RhOwn<NR>, 1>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       return;
     }
-  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, RhOwn<NR>>(D, glb(D.v32), br0, gather, own, epi);
+  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, RhOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
   else chunk_spmv<float, float, BS, NR, RhOwn<NR>, 1>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
@@ -585,11 +592,11 @@ __device__ __forceinline__ void inner_b_chunk(const ReqDesc &D, const ReqState &
   };
   if constexpr (BS >= SERVE_F16_MIN_BS)
     if (D.v16) {
-      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, SOwn<NR>>(D, glb(D.v16), br0, gather, own, epi, glb(D.sc16));
+      if (D.sl_j) chunk_spmv_sl<__half, float, BS, NR, SOwn<NR>>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       else chunk_spmv<__half, float, BS, NR, SOwn<NR>, 1>(D, glb(D.v16), br0, true, gather, own, epi, glb(D.sc16));
       return;
     }
-  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, SOwn<NR>>(D, glb(D.v32), br0, gather, own, epi);
+  if (D.sl_j) chunk_spmv_sl<float, float, BS, NR, SOwn<NR>>(D, glb(D.v32), br0, true, gather, own, epi);
   else chunk_spmv<float, float, BS, NR, SOwn<NR>, 1>(D, glb(D.v32), br0, true, gather, own, epi);
 }
 
@@ -1027,6 +1034,34 @@ __global__ __launch_bounds__(256) void k_serve_slice(int n_brows, int sl_j, cons
   }
 }
 
+// FP16 inner copy of a request whose FP64 values are already pass-sliced
+// (stencil requests; the block columns too): the same slots, each value
+// times its block row's 1/scale as in k_serve_slice.  One thread per row of
+// a sliced block; the rows of passes past the last block row are zeros.
+template <int BS>
+__global__ __launch_bounds__(256) void k_serve_sl_f16(int n_brows, int sl_j, const double *__restrict__ v64,
+                                                      const float *__restrict__ sc, __half *__restrict__ out)
+{
+  constexpr int RPW = 64 / BS;
+  const int n_rows = ((n_brows + RPW - 1) / RPW * RPW * sl_j) * BS;  // rows of all sliced blocks
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < n_rows; t += gridDim.x * 256) {
+    const int sb = t / BS;
+    const int pj = sb / RPW, lbr = sb - pj * RPW, br = pj / sl_j * RPW + lbr;
+    __half o[BS];
+    if (br < n_brows) {
+      const double inv = 1.0 / (double) sc[br];
+      double v[BS];
+      load_n<double, BS, lowbit(8 * BS)>(v64 + (int64_t) t * BS, v);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) o[c] = to_inner<__half>(v[c], inv);
+    } else {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) o[c] = (__half) 0.0f;
+    }
+    store_words<__half, BS>(out + (int64_t) t * BS, o);
+  }
+}
+
 // Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64
 // row-major blocks, 1 where the diagonal block is absent or |a_ii| < 1e-20
 // (the rule of the BiCGStab solver's k_jacobi), stored in FP32.
@@ -1045,6 +1080,30 @@ __global__ __launch_bounds__(256) void k_serve_dinv(int n_brows, int bs, const i
         found = true;
         break;
       }
+    dinv[row] = (float) ((!found || fabs(a) < 1e-20) ? 1.0 : 1.0 / a);
+  }
+}
+
+// The same from a pass-sliced FP64 copy: the row's slots in order, the first
+// whose column is the row's own (padding slots follow the real blocks).
+__global__ __launch_bounds__(256) void k_serve_dinv_sl(int n_brows, int bs, int sl_j, const int *__restrict__ cols_sl,
+                                                       const double *__restrict__ v64, float *__restrict__ dinv)
+{
+  const int rpw = 64 / bs;
+  const int64_t n = (int64_t) n_brows * bs;
+  for (int64_t row = (int64_t) blockIdx.x * 256 + threadIdx.x; row < n; row += (int64_t) gridDim.x * 256) {
+    const int br = (int) (row / bs), r = (int) (row - (int64_t) br * bs);
+    const int64_t s0 = (int64_t) (br / rpw) * sl_j * rpw + br % rpw;
+    double a = 0.0;
+    bool found = false;
+    for (int j = 0; j < sl_j; ++j) {
+      const int64_t sb = s0 + (int64_t) j * rpw;
+      if (cols_sl[sb] == br) {
+        a = v64[(sb * bs + r) * bs + r];
+        found = true;
+        break;
+      }
+    }
     dinv[row] = (float) ((!found || fabs(a) < 1e-20) ? 1.0 : 1.0 / a);
   }
 }

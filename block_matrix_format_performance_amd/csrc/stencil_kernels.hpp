@@ -113,17 +113,33 @@ template <class T, int N> __device__ __forceinline__ void store_row(T *p, const 
   }
 }
 
-// An FP32 copy written beside the FP64 blocks in the pass-sliced order of the
-// request loop's inner matrices: block j of node i is sliced block
-// ((i / RPW) * j_slots + j) * RPW + i % RPW (RPW = 64 / BS nodes per pass),
-// rows are padded to j_slots blocks with zero blocks whose column is the
-// row's last one, and the nodes past the grid in the last pass get zero
-// blocks and column 0.  vals == null: no such copy.
+// Copies in the pass-sliced order of the request loop's matrices: block j of
+// node i is sliced block ((i / RPW) * j_slots + j) * RPW + i % RPW (RPW =
+// 64 / BS nodes per pass), rows are padded to j_slots blocks with zero blocks
+// whose column is the row's last one, and the nodes past the grid in the
+// last pass get zero blocks and column 0.  vals: FP32 values, vals64: FP64
+// values, cols: the sliced block columns (written when either copy is).
 struct SlicedOut {
   float *vals = nullptr;
   int *cols = nullptr;
   int j_slots = 0;
+  double *vals64 = nullptr;
+  __host__ __device__ bool on() const { return vals != nullptr || vals64 != nullptr; }
 };
+
+// Row r (of BS values) of sliced slot `at` (the element offset of the row) in
+// the FP64 and FP32 sliced copies that are present.
+template <int BS>
+__device__ __forceinline__ void put_sliced_row(const SlicedOut &sl, int64_t at, const double (&v)[BS])
+{
+  if (sl.vals64) store_row<double, BS>(sl.vals64 + at, v);
+  if (sl.vals) {
+    float f[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) f[c] = (float) v[c];
+    store_row<float, BS>(sl.vals + at, f);
+  }
+}
 
 // k_gen_stencil_rows for a compile-time block size: the same draws, the same
 // arithmetic and the same order of the diagonal row sum, with each block row
@@ -140,18 +156,18 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
   constexpr int RPW = 64 / BS;
   const int64_t n_nodes = (int64_t) nx * ny;
   const int64_t t64 = (int64_t) blockIdx.x * 256 + threadIdx.x;
-  const int64_t n_pad = sl.vals ? (n_nodes + RPW - 1) / RPW * RPW : n_nodes;
+  const int64_t n_pad = sl.on() ? (n_nodes + RPW - 1) / RPW * RPW : n_nodes;
   if (t64 >= n_pad * BS) return;
   const int t = (int) t64;
   const int i = t / BS;
   const int r = t - i * BS;
-  // the row of slot j of node i in the sliced copy
-  auto sl_row = [&](int j) { return sl.vals + (((int64_t) (i / RPW) * sl.j_slots + j) * RPW + i % RPW) * (BS * BS) + r * BS; };
+  // the row of slot j of node i in the sliced copies
+  auto sl_row = [&](int j) { return (((int64_t) (i / RPW) * sl.j_slots + j) * RPW + i % RPW) * (BS * BS) + r * BS; };
   auto sl_col = [&](int j) { return sl.cols + ((int64_t) (i / RPW) * sl.j_slots + j) * RPW + i % RPW; };
   if (i >= n_nodes) {  // a node past the grid in the last pass: zero blocks
-    float z[BS] = {};
+    const double z[BS] = {};
     for (int j = 0; j < sl.j_slots; j++) {
-      store_row<float, BS>(sl_row(j), z);
+      put_sliced_row<BS>(sl, sl_row(j), z);
       if (r == 0) *sl_col(j) = 0;
     }
     return;
@@ -171,7 +187,7 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
     last_col = jy * nx + jx;
     if (r == 0) {
       cols[nb] = last_col;
-      if (sl.vals) *sl_col((int) (nb - nb0)) = last_col;
+      if (sl.on()) *sl_col((int) (nb - nb0)) = last_col;
     }
     const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) bs2;
     const int64_t at = nb * bs2 + (int64_t) r * BS;
@@ -198,32 +214,32 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_t(int nx, int ny, int 
 #pragma unroll
       for (int c = 0; c < BS; c++) dg[c] = v[c];
     } else {
-      store_row<double, BS>(vals + at, v);
-      if (vals32 || sl.vals) {
+      if (vals) store_row<double, BS>(vals + at, v);
+      if (vals32) {
         float f[BS];
 #pragma unroll
         for (int c = 0; c < BS; c++) f[c] = (float) v[c];
-        if (vals32) store_row<float, BS>(vals32 + at, f);
-        if (sl.vals) store_row<float, BS>(sl_row((int) (nb - nb0)), f);
+        store_row<float, BS>(vals32 + at, f);
       }
+      if (sl.on()) put_sliced_row<BS>(sl, sl_row((int) (nb - nb0)), v);
     }
     nb++;
   }
 #pragma unroll
   for (int c = 0; c < BS; c++)
     if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
-  store_row<double, BS>(vals + diag_at, dg);
-  if (vals32 || sl.vals) {
+  if (vals) store_row<double, BS>(vals + diag_at, dg);
+  if (vals32) {
     float f[BS];
 #pragma unroll
     for (int c = 0; c < BS; c++) f[c] = (float) dg[c];
-    if (vals32) store_row<float, BS>(vals32 + diag_at, f);
-    if (sl.vals) store_row<float, BS>(sl_row(diag_j), f);
+    store_row<float, BS>(vals32 + diag_at, f);
   }
-  if (sl.vals) {  // padding slots
-    float z[BS] = {};
+  if (sl.on()) {
+    put_sliced_row<BS>(sl, sl_row(diag_j), dg);
+    const double z[BS] = {};  // padding slots
     for (int j = (int) (nb - nb0); j < sl.j_slots; j++) {
-      store_row<float, BS>(sl_row(j), z);
+      put_sliced_row<BS>(sl, sl_row(j), z);
       if (r == 0) *sl_col(j) = last_col;
     }
   }
@@ -289,10 +305,11 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
   double *blk = blk_s[wib];
   const int nb0 = rp[i];
   int nb = nb0, diag_nb = 0, last_col = 0;
-  // the wavefront is one pass of the sliced copy: lane (q, r) writes row r of
-  // its node's blocks there (nodes past the grid: zero blocks)
-  const bool sl_on = sl.vals && q < NPW;
-  auto sl_row = [&](int j) { return sl.vals + (((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q) * BS2 + r * BS; };
+  // the wavefront is one pass of the sliced copies: lane (q, r) writes row r
+  // of its node's blocks there (nodes past the grid: zero blocks).  Without
+  // the BCSR FP64 output (vals null) no block is staged in LDS.
+  const bool sl_on = sl.on() && q < NPW;
+  auto sl_row = [&](int j) { return (((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q) * BS2 + r * BS; };
   auto sl_col = [&](int j) { return sl.cols + ((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q; };
   double dg[BS];
   double sum = 0.0, mx = 0.0;  // mx: largest |stored entry| of the lane's row
@@ -334,19 +351,16 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
       if (present) {
 #pragma unroll
         for (int c = 0; c < BS; c++) {
-          blk[q * BS2 + r * BS + c] = v[c];
+          if (vals) blk[q * BS2 + r * BS + c] = v[c];
           mx = fmax(mx, fabs(v[c]));
         }
-        if (sl_on) {
-          float f[BS];
-#pragma unroll
-          for (int c = 0; c < BS; c++) f[c] = (float) v[c];
-          store_row<float, BS>(sl_row(nb - nb0), f);
-        }
+        if (sl_on) put_sliced_row<BS>(sl, sl_row(nb - nb0), v);
       }
-      wave_lds_sync();
-      store_staged_blocks<BS>(blk, nb, present, vals, vals32, lane);
-      wave_lds_sync();  // the stores' LDS reads come before the next block's writes
+      if (vals) {  // (uniform)
+        wave_lds_sync();
+        store_staged_blocks<BS>(blk, nb, present, vals, vals32, lane);
+        wave_lds_sync();  // the stores' LDS reads come before the next block's writes
+      }
     }
     if (present) nb++;
   }
@@ -354,24 +368,20 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
 #pragma unroll
     for (int c = 0; c < BS; c++) {
       if (c == r) dg[c] = kind == 0 ? dg[c] + (sum + 1.0) : 1.02 * sum + 1.0;
-      blk[q * BS2 + r * BS + c] = dg[c];
+      if (vals) blk[q * BS2 + r * BS + c] = dg[c];
       mx = fmax(mx, fabs(dg[c]));
     }
   }
-  wave_lds_sync();
-  store_staged_blocks<BS>(blk, diag_nb, on, vals, vals32, lane);
+  if (vals) {
+    wave_lds_sync();
+    store_staged_blocks<BS>(blk, diag_nb, on, vals, vals32, lane);
+  }
   if (sl_on) {
-    float f[BS];
-    if (on) {
-#pragma unroll
-      for (int c = 0; c < BS; c++) f[c] = (float) dg[c];
-      store_row<float, BS>(sl_row(diag_nb - nb0), f);
-    }
+    if (on) put_sliced_row<BS>(sl, sl_row(diag_nb - nb0), dg);
     // padding slots (all of them for a node past the grid)
-#pragma unroll
-    for (int c = 0; c < BS; c++) f[c] = 0.0f;
+    const double z[BS] = {};
     for (int j = on ? nb - nb0 : 0; j < sl.j_slots; j++) {
-      store_row<float, BS>(sl_row(j), f);
+      put_sliced_row<BS>(sl, sl_row(j), z);
       if (r == 0) *sl_col(j) = on ? last_col : 0;
     }
   }
@@ -393,7 +403,7 @@ inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, 
                                SlicedOut sl = SlicedOut{})
 {
   // threads of the register-row kernels: the nodes of whole passes with a sliced copy
-  const int64_t nodes = sl.vals ? ((int64_t) nx * ny + 64 / bs - 1) / (64 / bs) * (64 / bs) : (int64_t) nx * ny;
+  const int64_t nodes = sl.on() ? ((int64_t) nx * ny + 64 / bs - 1) / (64 / bs) * (64 / bs) : (int64_t) nx * ny;
   const unsigned g = (unsigned) ((nodes * bs + 255) / 256);
   auto waves4 = [&](int npw) { return (unsigned) (((int64_t) nx * ny + 4 * npw - 1) / (4 * npw)); };
   switch (bs) {

@@ -126,7 +126,8 @@ def test_server_config_validation(bad):
 
 
 def test_server_request_footprint():
-    # FP64 matrix, the inner copy (FP32 values, or FP16 values and one FP32
+    # FP64 matrix (pass-sliced like the inner copy for a generated stencil
+    # whose inner copy is sliced), the inner copy (FP32 values, or FP16 values and one FP32
     # scale per block row, whichever is larger), the FP32 Jacobi inverse
     # diagonal, 3 FP64 and 9 FP32 vectors over all right-hand sides, 256-B
     # aligned pieces.  The inner copy is pass-sliced (9 block slots per row of
@@ -135,23 +136,26 @@ def test_server_request_footprint():
     def a256(b):
         return (b + 255) // 256 * 256
 
-    def want_bytes(nb, nnzb, bs, nrhs, jmax):
+    def want_bytes(nb, nnzb, bs, nrhs, jmax, stencil):
         n, nv = nb * bs, nnzb * bs * bs
         rpw = 64 // bs
         sb = -(-nb // rpw) * rpw * (-(-jmax // 3) * 3)
+        nv64 = nv
         if sb <= nnzb + nnzb // 4 + 576:
             nvs = sb * bs * bs
             inner = max(a256(4 * nvs), a256(2 * nvs) + a256(4 * nb)) + a256(4 * sb)
+            if stencil:         # generated stencils keep their FP64 values sliced too
+                nv64 = nvs
         else:
             inner = max(a256(4 * nv), a256(2 * nv) + a256(4 * nb))
-        return (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv) + inner + a256(4 * n)
+        return (a256(4 * (nb + 1)) + a256(4 * nnzb) + a256(8 * nv64) + inner + a256(4 * n)
                 + 3 * a256(8 * n * nrhs) + 9 * a256(4 * n * nrhs))
 
     for nx, ny, bs, nrhs in [(16, 16, 2, 1), (100, 37, 6, 3), (512, 512, 8, 4), (2, 40, 3, 2)]:
         nb, nnzb = nx * ny, (3 * nx - 2) * (3 * ny - 2)
         jmax = min(nx, 3) * min(ny, 3)
         got = bm.Server.request_bytes(bm.Server.stencil_request(nx, ny, bs, nrhs=nrhs))
-        assert got == want_bytes(nb, nnzb, bs, nrhs, jmax)
+        assert got == want_bytes(nb, nnzb, bs, nrhs, jmax, True)
     # an explicit matrix with one 60-block row keeps its inner copy in BCSR order
     n_rows, bs = 100, 2
     lens = [60 if i == 7 else 1 for i in range(n_rows)]
@@ -160,7 +164,14 @@ def test_server_request_footprint():
     vals = np.ones(len(cols) * bs * bs)
     got = bm.Server.request_bytes(bm.Server.bcsr_request(rp, cols, vals, n_rows, bs))
     nnzb = int(rp[-1])
-    assert got == want_bytes(n_rows, nnzb, bs, 1, 60)   # 7680 sliced blocks > 159 + 39 + 576
+    assert got == want_bytes(n_rows, nnzb, bs, 1, 60, False)   # 7680 sliced blocks > 159 + 39 + 576
+    # an explicit matrix with even rows: sliced inner copy, FP64 values in BCSR order
+    lens = [5] * n_rows
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([(np.arange(5) + i) % n_rows for i in range(n_rows)]).astype(np.int32)
+    vals = np.ones(len(cols) * bs * bs)
+    got = bm.Server.request_bytes(bm.Server.bcsr_request(rp, cols, vals, n_rows, bs))
+    assert got == want_bytes(n_rows, int(rp[-1]), bs, 1, 5, False)
 
 
 def test_server_request_size_limits():
