@@ -101,13 +101,29 @@ BcsrArgs<VT, AT> bcsr_args(const bmfp_matrix_t &A, const void *x, void *y, int p
 // 10.4 KB at bs 2, 14.4 KB at bs 3 in FP64), the fastest of 1, 2 and 4 on the
 // HBM-resident 9-point stencils.
 template <int BS> constexpr int stage_wpg() { return BS == 1 ? 4 : 2; }
+// XCD bands of k_bcsr_stage at bs 1 and 2: 128 wavefronts (8192 block rows
+// at bs 1, 4096 at bs 2).  On the HBM-resident 9-point stencils (a band ~1.2
+// grid rows) they cut the fetched bytes by 4.8% at bs 1 and 5.7% at bs 2,
+// at -0.4 and +0.9 points of HBM peak in paired sweeps (bs 3: -0.9, so it
+// keeps the hardware order); bands of 4 grid rows and more fetch less still
+// but run 1.5-3 points slower.
+template <int BS> constexpr int stage_band_waves() { return BS <= 2 ? 128 : 0; }
 
 template <class VT, class AT, int BS, int LAYOUT, bool SCALED>
 void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 {
-  constexpr int RPW = 64 / BS, WPG = stage_wpg<BS>();
+  constexpr int RPW = 64 / BS, WPG = stage_wpg<BS>(), BAND = stage_band_waves<BS>() / WPG;
   const int64_t waves = std::max<int64_t>((A.n_rows + RPW - 1) / RPW, 1);
-  const Grid g = make_grid((waves + WPG - 1) / WPG * 256);  // one group per WPG wavefronts
+  const int64_t groups = (waves + WPG - 1) / WPG;  // one group per WPG wavefronts
+  if (BAND > 0 && groups >= 8 * BAND && !xcd_remap_enabled()) {
+    // whole rounds of 8 bands; groups past the last block row exit at once
+    const int64_t gb = (groups + 8 * BAND - 1) / (8 * BAND) * (8 * BAND);
+    BcsrArgs<VT, AT> a = bcsr_args<VT, AT, SCALED>(A, x, y, 0);
+    a.band = BAND;
+    launch(k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG>, (unsigned) gb, 64 * WPG, s, a);
+    return;
+  }
+  const Grid g = make_grid(groups * 256);
   launch(k_bcsr_stage<VT, AT, BS, LAYOUT, SCALED, WPG>, g.groups, 64 * WPG, s,
          bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }

@@ -111,12 +111,24 @@ __device__ __forceinline__ int64_t tile_id(int per)
   if (per == 0) return i;
   return (i & 7) * (int64_t) per + (i >> 3);
 }
+// Banded order: XCD j (workgroup ids j, j + 8, ...) takes bands j, j + 8, ...
+// of `band` consecutive tiles, so the block rows of a band, and the x
+// segments their neighbouring rows share, stay in one L2 without the eight
+// XCDs streaming eight distant regions (launch a multiple of 8 * band
+// groups).  band = 0: tile_id(per).
+__device__ __forceinline__ int64_t tile_id(int per, int band)
+{
+  if (band == 0) return tile_id(per);
+  const int64_t l = (int64_t) (blockIdx.x >> 3), xcd = blockIdx.x & 7;
+  return (l / band * 8 + xcd) * band + l % band;
+}
 
 // ---- shared launch arguments --------------------------------------------------
 template <class VT, class AT>
 struct BcsrArgs {
   int n_brows;
   int xcd_per;
+  int band = 0;  // tiles per XCD band (tile_id(per, band)); 0: none
   const int *__restrict__ rp;
   const int *__restrict__ cols;
   const VT *__restrict__ vals;
@@ -640,7 +652,7 @@ __global__ __launch_bounds__(64 * WPG) void k_bcsr_stage(BcsrArgs<VT, AT> a)
   __shared__ __align__(16) char vals_s[WPG][staged_bytes(CAP * BB)];
   __shared__ __align__(16) char cols_s[WPG][staged_bytes(CAP * 4)];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
-  const int64_t r0_64 = (tile_id(a.xcd_per) * WPG + wib) * RPW;
+  const int64_t r0_64 = (tile_id(a.xcd_per, a.band) * WPG + wib) * RPW;
   if (r0_64 >= a.n_brows) return;  // the whole wavefront leaves; no barrier follows
   const int r0 = (int) r0_64, r1 = r0 + RPW < a.n_brows ? r0 + RPW : a.n_brows;
   const int B0 = a.rp[r0], B1 = a.rp[r1];
