@@ -16,7 +16,9 @@ bench = pytest.importorskip("bench")
 def test_traffic_lookup_matches_symbol_and_sizing():
     sym = "_ZN4bmfp12k_bcsr_stageIddLi1ELi1ELb0ELi4EEEvNS_8BcsrArgsIT_T0_EE"
     t = bench.profiled_traffic(sym, "hbm")
-    assert t is not None and 7.0e9 < t < 7.5e9
+    # the bs = 1 sweep matrix: 6,256,665,396 algorithmic bytes; the measured
+    # traffic is at least that and re-reads part of x
+    assert t is not None and 6256665396 <= t < 1.2 * 6256665396
     assert bench.profiled_traffic(sym, "literal") is None        # no row for that sizing
     assert bench.profiled_traffic("_ZN4bmfp3nopEv", "hbm") is None
 
