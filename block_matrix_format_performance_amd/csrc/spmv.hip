@@ -128,12 +128,39 @@ void launch_stage(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
          bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
 }
 
+// XCD bands of the dense lane-group kernels (tile_id(per, band)): about 8192
+// scalar rows per band, at bs 8 and 12, where paired sweeps on the
+// HBM-resident stencils measured row-major bs 12 74.4 -> 76.8% of HBM peak
+// and bs 8 +0.3 points in both layouts; at bs 4 (-0.6), 6 (0) and 16-32
+// (-0.3 to -0.9) the hardware order is kept.
+template <int BS> constexpr bool dense_banded() { return BS == 8 || BS == 12; }
+
+// Launches k with `groups` workgroups of 256 lanes covering rows_per_group
+// scalar rows each, dealt to XCDs in bands when dense_banded<BS>().
+template <class VT, class AT, bool SCALED, int BS, class K>
+void launch_dense(K k, const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s, int64_t groups,
+                  int64_t rows_per_group)
+{
+  const int band = (int) std::max<int64_t>(1, 8192 / rows_per_group);
+  if (dense_banded<BS>() && groups >= 8 * band && !xcd_remap_enabled()) {
+    // whole rounds of 8 bands; waves past the last row exit at once
+    const int64_t gb = (groups + 8 * band - 1) / (8 * band) * (8 * band);
+    BcsrArgs<VT, AT> a = bcsr_args<VT, AT, SCALED>(A, x, y, 0);
+    a.band = band;
+    launch(k, (unsigned) gb, 256, s, a);
+    return;
+  }
+  const Grid g = make_grid(groups * 256);
+  launch(k, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+}
+
 template <class VT, class AT, int BS, int VEC, int U, bool SCALED, bool NT>
 void launch_dgc(const bmfp_matrix_t &A, const void *x, void *y, hipStream_t s)
 {
   constexpr int BPW = 64 / (BS * (BS / VEC));
-  Grid g = make_grid((A.n_rows + BPW - 1) / BPW * 64);
-  launch(k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED, NT>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+  const int64_t waves = (A.n_rows + BPW - 1) / BPW;
+  launch_dense<VT, AT, SCALED, BS>(k_bcsr_dgc<VT, AT, BS, VEC, U, SCALED, NT>, A, x, y, s, (waves + 3) / 4,
+                                   4 * BPW * BS);
 }
 
 // Column-major FP64 lane-group shapes (values per lane, blocks in flight per
@@ -179,9 +206,9 @@ int launch_fixed(const bmfp_matrix_t &A, int variant, const void *x, void *y, hi
       launch(k_bcsr_split<VT, AT, BS, LAYOUT, SCALED>, g.groups, 256, s, bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
     } else if constexpr (LAYOUT == BMFP_ROW_MAJOR && D.vec > 0) {
       constexpr int RPW = 64 / (BS / D.vec);
-      Grid g = make_grid((nb * BS + RPW - 1) / RPW * 64);
-      launch(k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED, D.nt>, g.groups, 256, s,
-             bcsr_args<VT, AT, SCALED>(A, x, y, g.per));
+      const int64_t waves = (nb * BS + RPW - 1) / RPW;
+      launch_dense<VT, AT, SCALED, BS>(k_bcsr_dgr<VT, AT, BS, D.vec, D.u, SCALED, D.nt>, A, x, y, s,
+                                       (waves + 3) / 4, 4 * RPW);
     } else if constexpr (LAYOUT == BMFP_COL_MAJOR && sizeof(VT) == 8 && dgc_shape<BS>().vec > 0) {
       constexpr DenseShape G = dgc_shape<BS>();
       launch_dgc<VT, AT, BS, G.vec, G.u, SCALED, G.nt>(A, x, y, s);

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgr(BcsrArgs<VT, AT> a)
   constexpr int RPW = 64 / G;
   static_assert(BS % VEC == 0 && G <= 64, "bad group shape");
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int64_t wave = (tile_id(a.xcd_per, a.band) * 256 + threadIdx.x) >> 6;
   const int lr = lane / G, g = lane - lr * G;
   const int64_t row = wave * RPW + lr;
   const int64_t rows = (int64_t) a.n_brows * BS;
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void k_bcsr_dgc(BcsrArgs<VT, AT> a)
   static_assert(BS % VEC == 0 && L <= 64, "bad group shape");
   constexpr int ALIGN = cmin(lowbit((int) sizeof(VT) * BS), lowbit((int) sizeof(VT) * VEC));
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (tile_id(a.xcd_per) * 256 + threadIdx.x) >> 6;
+  const int64_t wave = (tile_id(a.xcd_per, a.band) * 256 + threadIdx.x) >> 6;
   const int q = lane / L, l = lane - q * L;
   const int c = l / H, h = l - c * H;
   const int64_t br = wave * BPW + q;

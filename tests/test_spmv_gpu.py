@@ -126,6 +126,18 @@ def test_bcsr_f64_xcd_sized_grid(bs):
             check_bcsr(m, x, layout, v)
 
 
+# grids past 8 XCD bands (a partial last round of bands included), so the
+# lane-group family takes its banded tile order: the staged kernel at bs 1
+# and 2 (8192 / 4096 block rows per band), the dense kernels at bs 8 and 12
+# (~8192 scalar rows per band)
+@pytest.mark.parametrize("bs,side", [(1, 270), (2, 190), (8, 95), (12, 76)])
+def test_bcsr_f64_banded_grid(bs, side):
+    m, x = stencil_case(side, side, bs, kind=1, seed=5)
+    for layout in (bm.ROW_MAJOR, bm.COL_MAJOR):
+        check_bcsr(m, x, layout, bm.BCSR_GROUP_PER_ROW)
+        check_bcsr(m, x, layout, bm.SPMV_AUTO)
+
+
 @pytest.mark.parametrize("bs", [5, 7, 40])
 def test_bcsr_runtime_block_size(bs):
     m, x = stencil_case(6, 5, bs, kind=0, seed=1)
