@@ -345,6 +345,10 @@ int solver_run(bmfp_solver_t &S, const void *b, void *x, double eps, int max_ite
   }
   const size_t vb = vec_size(S.A->dtype);
   BMFP_HIP(hipMemcpyAsync(x, S.vecs.ptr, (size_t) S.n * vb, hipMemcpyDeviceToDevice, s));
+  // the caller's later work on its stream reads x: order it after the copy
+  // (an untimed solve returns without waiting for it)
+  BMFP_HIP(hipEventRecord(S.ev_sync, s));
+  BMFP_HIP(hipStreamWaitEvent(user, S.ev_sync, 0));
   float ms = 0.f;
   if (timed) {
     BMFP_HIP(hipEventRecord(S.ev1, s));
