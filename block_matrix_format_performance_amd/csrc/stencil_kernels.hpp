@@ -395,9 +395,10 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
 }
 
 // The generator for any block size: the register-row kernel for the block
-// sizes the request loop serves (coalesced through LDS for 4, 6 and 8), the
-// generic one otherwise.  `scale` (FP16 block-row scales) is written for
-// bs 4, 6 and 8 only; `sl` (a pass-sliced FP32 copy) for bs 1, 2, 3, 4, 6, 8.
+// sizes the request loop serves (its BCSR-order FP64 stores coalesced through
+// LDS for 4, 6 and 8), the generic one otherwise.  `vals` null: no BCSR-order
+// values.  `scale` (FP16 block-row scales) is written for bs 4, 6 and 8 only;
+// `sl` (pass-sliced FP64 and / or FP32 copies) for bs 1, 2, 3, 4, 6, 8.
 inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
                                double *vals, float *vals32, hipStream_t s, float *scale = nullptr,
                                SlicedOut sl = SlicedOut{})
