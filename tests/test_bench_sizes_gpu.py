@@ -125,6 +125,12 @@ def test_config2_fp64_bicgstab_full_size(device):
         assert ri.status == bm.OK and ri.escalations == 0
         assert true_rel < 1e-10 and abs(true_rel - ri.true_rel_residual) < 1e-11
         assert rel(xr, xo) < 1e-9
+        # the same solve again: the same steps and bits (each inner solve's
+        # correction is ordered before the next residual reads it)
+        dx.upload(np.zeros(c.n_rows))
+        r2 = R.solve(db, dx, 1e-10, tol, 50, 20000)
+        assert (r2.outer_iterations, r2.inner_iterations) == (ri.outer_iterations, ri.inner_iterations)
+        assert np.array_equal(dx.download(), xr)
         R.destroy()
 
 
