@@ -690,10 +690,11 @@ struct RowsEngine {
 };
 
 // Rows per thread and block rows per wavefront of SlicedRowsEngine: one
-// 16-byte row load per block (FP16 bs 4: two 8-byte rows).  For FP32 bs 4
-// one row per thread (16 block rows per pass) beat two (RowsEngine's choice,
-// 32 per pass) and four (64 per pass): FP32 refinement of the 2048x1024
-// system 18.2 / 18.4 / 18.7 ms, same iterations and bits.
+// 16-byte row load per block (FP16 bs 4: two 8-byte rows).  Measured on the
+// refinement of the 2048x1024 bs 4 system (same iterations and bits): FP32
+// one row per thread (16 block rows per pass) 18.2 ms against two
+// (RowsEngine's choice) 18.4 and four 18.7; FP16 two rows 15.2 ms against
+// one 16.9 and four 16.8.
 template <class VT, int BS> constexpr int sliced_rows() { return rows_per_thread<VT, BS>(); }
 template <class VT, int BS> constexpr int sliced_rpw() { return 64 / (BS / sliced_rows<VT, BS>()); }
 
