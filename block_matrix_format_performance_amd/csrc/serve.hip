@@ -1034,34 +1034,6 @@ __global__ __launch_bounds__(256) void k_serve_slice(int n_brows, int sl_j, cons
   }
 }
 
-// FP16 inner copy of a request whose FP64 values are already pass-sliced
-// (stencil requests; the block columns too): the same slots, each value
-// times its block row's 1/scale as in k_serve_slice.  One thread per row of
-// a sliced block; the rows of passes past the last block row are zeros.
-template <int BS>
-__global__ __launch_bounds__(256) void k_serve_sl_f16(int n_brows, int sl_j, const double *__restrict__ v64,
-                                                      const float *__restrict__ sc, __half *__restrict__ out)
-{
-  constexpr int RPW = 64 / BS;
-  const int n_rows = ((n_brows + RPW - 1) / RPW * RPW * sl_j) * BS;  // rows of all sliced blocks
-  for (int t = blockIdx.x * 256 + threadIdx.x; t < n_rows; t += gridDim.x * 256) {
-    const int sb = t / BS;
-    const int pj = sb / RPW, lbr = sb - pj * RPW, br = pj / sl_j * RPW + lbr;
-    __half o[BS];
-    if (br < n_brows) {
-      const double inv = 1.0 / (double) sc[br];
-      double v[BS];
-      load_n<double, BS, lowbit(8 * BS)>(v64 + (int64_t) t * BS, v);
-#pragma unroll
-      for (int c = 0; c < BS; ++c) o[c] = to_inner<__half>(v[c], inv);
-    } else {
-#pragma unroll
-      for (int c = 0; c < BS; ++c) o[c] = (__half) 0.0f;
-    }
-    store_words<__half, BS>(out + (int64_t) t * BS, o);
-  }
-}
-
 // Jacobi preconditioner of a request: 1 / a_ii per scalar row from the FP64
 // row-major blocks, 1 where the diagonal block is absent or |a_ii| < 1e-20
 // (the rule of the BiCGStab solver's k_jacobi), stored in FP32.

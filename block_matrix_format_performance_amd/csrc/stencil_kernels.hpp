@@ -119,13 +119,28 @@ template <class T, int N> __device__ __forceinline__ void store_row(T *p, const 
 // whose column is the row's last one, and the nodes past the grid in the
 // last pass get zero blocks and column 0.  vals: FP32 values, vals64: FP64
 // values, cols: the sliced block columns (written when either copy is).
+// vals16 (bs 4, 6 and 8 only, beside vals64): FP16 values, each FP64 value v
+// of block row i stored as half(float(v * (1 / scale_i))) with scale_i the row's
+// largest |entry| rounded to FP32 (1 for an all-zero row).
 struct SlicedOut {
   float *vals = nullptr;
   int *cols = nullptr;
   int j_slots = 0;
   double *vals64 = nullptr;
+  __half *vals16 = nullptr;
   __host__ __device__ bool on() const { return vals != nullptr || vals64 != nullptr; }
 };
+
+// Row of BS FP16 values v[c] * inv at p (p a multiple of 2*BS bytes), in
+// 4- to 16-byte stores.
+template <int BS>
+__device__ __forceinline__ void put_half_row(__half *p, const double (&v)[BS], double inv)
+{
+  __half h[BS];
+#pragma unroll
+  for (int c = 0; c < BS; c++) h[c] = __float2half_rn((float) (v[c] * inv));
+  store_words<__half, BS>(p, h);
+}
 
 // Row r (of BS values) of sliced slot `at` (the element offset of the row) in
 // the FP64 and FP32 sliced copies that are present.
@@ -274,7 +289,8 @@ __device__ __forceinline__ void store_staged_blocks(const double *blk, int nb, b
 
 // k_gen_stencil_rows_t with the stores coalesced, for bs 4, 6 and 8; with
 // `scale`, also each block row's largest |entry| rounded to FP32 (1 for an
-// all-zero row), the scale of an FP16 copy.  There a
+// all-zero row), the scale of an FP16 copy, and with sl.vals16 that copy,
+// pass-sliced.  There a
 // lane's 16-byte stores of its block row still leave every store instruction
 // spread over one cache line per row.  A wavefront holds NPW = 64/BS
 // consecutive nodes; for each neighbour s the lanes put their block rows in
@@ -311,6 +327,25 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
   const bool sl_on = sl.on() && q < NPW;
   auto sl_row = [&](int j) { return (((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q) * BS2 + r * BS; };
   auto sl_col = [&](int j) { return sl.cols + ((int64_t) (i0 / NPW) * sl.j_slots + j) * NPW + q; };
+  // row r of the node's block toward neighbour s, before the diagonal shift
+  auto draw = [&](int s, double (&v)[BS]) {
+    const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) BS2;
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      if (kind == 0) {
+        v[c] = rng_sym(ka, base + (uint64_t) r * BS + c);
+      } else if (s == 4) {
+        v[c] = 0.0;
+      } else {
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        const double k = rng_unit(ka, base + (uint64_t) lo * BS + hi);
+        double cc = 0.0;
+        if (r < c) cc = rng_sym(kc, base + (uint64_t) r * BS + c);
+        else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * BS + r);
+        v[c] = -(k + 0.3 * cc);
+      }
+    }
+  };
   double dg[BS];
   double sum = 0.0, mx = 0.0;  // mx: largest |stored entry| of the lane's row
   for (int s = 0; s < 9; s++) {
@@ -325,23 +360,10 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
         cols[nb] = last_col;
         if (sl_on) *sl_col(nb - nb0) = last_col;
       }
-      const uint64_t base = ((uint64_t) i * 9u + (uint64_t) s) * (uint64_t) BS2;
+      draw(s, v);
 #pragma unroll
-      for (int c = 0; c < BS; c++) {
-        if (kind == 0) {
-          v[c] = rng_sym(ka, base + (uint64_t) r * BS + c);
-        } else if (s == 4) {
-          v[c] = 0.0;
-        } else {
-          const int lo = r < c ? r : c, hi = r < c ? c : r;
-          const double k = rng_unit(ka, base + (uint64_t) lo * BS + hi);
-          double cc = 0.0;
-          if (r < c) cc = rng_sym(kc, base + (uint64_t) r * BS + c);
-          else if (r > c) cc = -rng_sym(kc, base + (uint64_t) c * BS + r);
-          v[c] = -(k + 0.3 * cc);
-        }
+      for (int c = 0; c < BS; c++)
         if (!(s == 4 && c == r)) sum += fabs(v[c]);
-      }
     }
     if (s == 4) {
       diag_nb = nb;
@@ -385,12 +407,35 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
       if (r == 0) *sl_col(j) = on ? last_col : 0;
     }
   }
-  if (scale) {
+  if (scale || sl.vals16) {  // (uniform)
     // the node's BS rows are lanes q*BS .. q*BS + BS - 1
-    double m = mx;
+    double m = 0.0;
 #pragma unroll
-    for (int k = 1; k < BS; k++) m = fmax(m, __shfl(mx, (q * BS + k) & 63));
-    if (on && r == 0) scale[i] = m == 0.0 ? 1.0f : (float) m;
+    for (int k = 0; k < BS; k++) m = fmax(m, __shfl(mx, (q * BS + k) & 63));
+    const float sc = m == 0.0 ? 1.0f : (float) m;
+    if (scale && on && r == 0) scale[i] = sc;
+    if (sl.vals16 && q < NPW) {
+      // the FP16 copy needs the whole block row's scale first: the
+      // off-diagonal rows are drawn again (the same draws), the diagonal row
+      // is dg, then the padding slots (all of them past the grid) are zeros
+      const double inv = 1.0 / (double) sc;
+      int j = 0;
+      for (int s = 0; s < 9; s++) {
+        const int jy = iy + s / 3 - 1, jx = ix + s % 3 - 1;
+        if (!(on && jy >= 0 && jy < ny && jx >= 0 && jx < nx)) continue;
+        double v[BS];
+        if (s == 4) {
+#pragma unroll
+          for (int c = 0; c < BS; c++) v[c] = dg[c];
+        } else {
+          draw(s, v);
+        }
+        put_half_row<BS>(sl.vals16 + sl_row(j), v, inv);
+        j++;
+      }
+      const double z[BS] = {};
+      for (; j < sl.j_slots; j++) put_half_row<BS>(sl.vals16 + sl_row(j), z, inv);
+    }
   }
 }
 
@@ -398,7 +443,8 @@ __global__ __launch_bounds__(256) void k_gen_stencil_rows_c(int nx, int ny, int 
 // sizes the request loop serves (its BCSR-order FP64 stores coalesced through
 // LDS for 4, 6 and 8), the generic one otherwise.  `vals` null: no BCSR-order
 // values.  `scale` (FP16 block-row scales) is written for bs 4, 6 and 8 only;
-// `sl` (pass-sliced FP64 and / or FP32 copies) for bs 1, 2, 3, 4, 6, 8.
+// `sl` (pass-sliced FP64 and / or FP32 copies) for bs 1, 2, 3, 4, 6, 8, its
+// FP16 copy for bs 4, 6 and 8.
 inline void launch_gen_stencil(int nx, int ny, int bs, int kind, uint64_t seed, const int *rp, int *cols,
                                double *vals, float *vals32, hipStream_t s, float *scale = nullptr,
                                SlicedOut sl = SlicedOut{})

@@ -186,16 +186,19 @@ def test_jacobi_preconditioned_requests():
     assert np.array_equal(xa, oj[ij[2]][1])
 
 
-@pytest.mark.parametrize("bs,kind", [(4, 1), (1, 0), (2, 1), (3, 0), (6, 1), (8, 0)])
-def test_device_generator_matches_host_generator(bs, kind):
+@pytest.mark.parametrize("bs,kind,f16_min_bs", [(4, 1, 6), (1, 0, 6), (2, 1, 6), (3, 0, 6), (6, 1, 6), (6, 0, 6),
+                                                (8, 0, 6), (8, 1, 6), (4, 0, 4)])
+def test_device_generator_matches_host_generator(bs, kind, f16_min_bs):
     # explicit arrays from the host generator vs the on-device generator (FP64
-    # and FP32 values written at admission): identical matrices and
-    # right-hand sides give bitwise identical solutions
+    # values and the FP32 or FP16+scale inner copy written at admission; FP16
+    # for bs >= f16_min_bs): identical matrices and right-hand sides give
+    # bitwise identical solutions.  14 x 11 nodes leave the last pass of 64/bs
+    # nodes partly past the grid.
     nx, ny = 14, 11
     rp, cols, vals = bm.gen_stencil9(nx, ny, bs, kind, seed=5)
     n = nx * ny * bs
     b = oracle_rhs(3, 1, n)
-    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8, inner_f16_min_bs=f16_min_bs)
     i1 = S.submit(S.stencil_request(nx, ny, bs, kind=kind, seed=5, nrhs=1, rhs_seed=3))
     i2 = S.submit(S.bcsr_request(rp, cols, vals, nx * ny, bs, nrhs=1, b=b))
     out = S.drain()
