@@ -88,8 +88,15 @@ struct ReqDesc {
 
 struct RhsState {
   int status, outer, inner_it, inner_total;
-  double norm_b2, rel, rho, alpha, omega, beta, tt, inner_rr0;
+  double norm_b2, rel, rho, alpha, omega, beta, tt, inner_rr0, inner_tol;
 };
+
+// A refinement step whose outer residual rel is already close to the outer
+// tolerance tol stops its inner solve at SERVE_LAST_STEP_MARGIN * tol / rel
+// when that is looser than the inner tolerance: the next residual is then
+// about (that + the inner precision's own error) * rel, below tol, and the
+// step reads the inner matrix for fewer iterations.
+constexpr double SERVE_LAST_STEP_MARGIN = 0.25;
 
 // `outer` counts the refinement steps of the live right-hand sides and
 // `inner_it` the iterations of the current inner solve of the active ones
@@ -850,6 +857,7 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
         if (keep) {
           // the inner solve starts: ri = rh = r32, rho_0 = (rh, ri)
           R.inner_rr0 = a1;
+          R.inner_tol = fmax(D.inner_tol, rel > 0.0 ? SERVE_LAST_STEP_MARGIN * D.tol / rel : 0.0);
           R.inner_it = 0;
           R.rho = a1;
           R.alpha = 1.0;
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(256) void k_fin(const ReqDesc *__restrict__ rd, Req
         // a0 = (rh, ri), a1 = ||ri||^2 after the update
         R.inner_it = it + 1;
         const double rel_in = R.inner_rr0 > 0.0 ? sqrt(a1) / sqrt(R.inner_rr0) : 0.0;
-        const bool stop = !(rel_in >= D.inner_tol) || it + 1 >= D.max_inner || R.tt == 0.0 || R.omega == 0.0 ||
+        const bool stop = !(rel_in >= R.inner_tol) || it + 1 >= D.max_inner || R.tt == 0.0 || R.omega == 0.0 ||
                           a0 == 0.0 || !isfinite(a0);
         if (stop) {
           R.inner_total += it + 1;
