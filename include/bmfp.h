@@ -366,8 +366,12 @@ int bmfp_server_request_bytes(const bmfp_request *req, size_t *bytes);
 /* Queues a request (copies explicit host arrays).  BMFP_ERR_BUDGET when it
  * could never fit the budget. */
 int bmfp_server_submit(bmfp_server s, const bmfp_request *req, uint64_t *id);
-/* Admits what fits, runs one round of batched iterations, retires finished
- * requests.  Synchronous; reports active systems and waiting requests. */
+/* Waits for the round of batched iterations the previous call launched and
+ * retires the requests it finished (bmfp_server_poll then returns them),
+ * admits what fits, launches the next round and returns while it runs, so
+ * the caller's work until the next call overlaps it.  Reports the active
+ * systems (those in the launched round) and the waiting requests; 0 and 0
+ * when nothing is left to run.  Destroying the server waits for the round. */
 int bmfp_server_step(bmfp_server s, int *n_active, int *n_queued);
 /* Pops one finished request (FIFO by completion): result and, if x is not
  * NULL, its solutions (nrhs * n doubles, right-hand side major).  A finished
