@@ -451,7 +451,7 @@ def plain_one_at_a_time(reqs, max_iter, stream, use_jacobi=False):
     return total, np.array(per_req), xs, iters, moved, gen_s
 
 
-SERVE_F16_MIN_BS = 6  # the server's default inner_f16_min_bs
+SERVE_F16_MIN_BS = 4  # the server's default inner_f16_min_bs
 
 
 SERVE_SLOTS = "XRPASBC"

@@ -318,7 +318,7 @@ typedef struct {
                                  the tables each round); 0: tuned                     */
   double inner_tol;           /* FP32 BiCGStab relative tolerance per refinement step;
                                  a step whose outer residual rel is near the request's
-                                 tol stops at max(inner_tol, 0.25 tol / rel) instead  */
+                                 tol stops at max(inner_tol, 0.4 tol / rel) instead   */
   int max_outer;              /* refinement steps per right-hand side                  */
   int max_inner;              /* inner iterations per refinement step                  */
   int chunks_per_tile;        /* tile height in 256-row chunks; 0: tuned per bs        */
@@ -326,7 +326,8 @@ typedef struct {
                                  the inner solves, as gpu_bicgstab(A, true); 0: none   */
   int inner_f16_min_bs;       /* requests with bs >= this (4, 6 or 8) keep their inner
                                  matrix in FP16 with one FP32 scale per block row
-                                 instead of FP32; 0: FP32 for every request           */
+                                 instead of FP32 (default 4); 0: FP32 for every
+                                 request                                              */
   double inner_tol_f16;       /* inner tolerance of FP16 requests; 0: 1e-3 for bs 8,
                                  1e-4 below                                            */
 } bmfp_server_config;
