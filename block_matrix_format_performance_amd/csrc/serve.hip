@@ -96,7 +96,7 @@ struct RhsState {
 // when that is looser than the inner tolerance: the next residual is then
 // about (that + the inner precision's own error) * rel, below tol, and the
 // step reads the inner matrix for fewer iterations.
-constexpr double SERVE_LAST_STEP_MARGIN = 0.4;
+constexpr double SERVE_LAST_STEP_MARGIN = 0.6;
 
 // `outer` counts the refinement steps of the live right-hand sides and
 // `inner_it` the iterations of the current inner solve of the active ones

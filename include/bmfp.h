@@ -318,7 +318,7 @@ typedef struct {
                                  the tables each round); 0: tuned                     */
   double inner_tol;           /* FP32 BiCGStab relative tolerance per refinement step;
                                  a step whose outer residual rel is near the request's
-                                 tol stops at max(inner_tol, 0.4 tol / rel) instead   */
+                                 tol stops at max(inner_tol, 0.6 tol / rel) instead   */
   int max_outer;              /* refinement steps per right-hand side                  */
   int max_inner;              /* inner iterations per refinement step                  */
   int chunks_per_tile;        /* tile height in 256-row chunks; 0: tuned per bs        */
