@@ -637,7 +637,8 @@ def serving_comparison(n_requests, n_low, n_plain, lambdas, max_iter, stream, cf
     return {
         "workload": "20,000-request stream (seed 2): side U{16..512}, bs in {2,3,4,6,8}, 1-4 RHS uniform[-1,1], "
                     "tol 1e-10, diagonally dominant 9-point stencils generated on the device at admission; "
-                    "served by FP64 refinement around FP32 BiCGStab",
+                    "served by FP64 refinement around FP32-vector BiCGStab on the FP32 (bs 2, 3) or "
+                    "FP16+scale (bs 4-8) inner matrix",
         "replayed": {"lambda_ge_500": n_requests, "lambda_lt_500": n_low, "plain_sample": n_plain},
         "config": {k: v for k, v in cfg.items()},
         "plain_fp64": plain_res,
