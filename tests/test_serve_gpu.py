@@ -309,6 +309,34 @@ def test_results_independent_of_tile_schedule(groups, chunks):
         assert np.array_equal(base[a][1], out[b][1])
 
 
+def test_step_returns_while_its_round_runs():
+    # bmfp_server_step launches a round and returns while it runs: a request
+    # finished in it is returned after the following step; destroying the
+    # server with a round in flight waits for that round
+    spec = dict(nx=48, ny=40, bs=4, kind=0, seed=21, nrhs=2, rhs_seed=8)
+    ref = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    ref.submit(ref.stencil_request(**spec))
+    r0, x0 = next(iter(ref.drain().values()))
+    ref.destroy()
+    S = bm.Server(memory_budget_bytes=1 << 30, max_systems=8)
+    rid = S.submit(S.stencil_request(**spec))
+    got, steps = None, 0
+    while got is None:
+        a, q = S.step()
+        steps += 1
+        assert steps < 100000
+        got = S.poll()
+        if got is None:
+            assert a == 2 and q == 0  # its round (2 right-hand sides) is in flight
+    assert got[0].id == rid and got[0].status == bm.OK
+    assert np.array_equal(got[1], x0)
+    assert S.step() == (0, 0)
+    # a round in flight when the server is destroyed
+    S.submit(S.stencil_request(**spec))
+    assert S.step() == (2, 0)
+    S.destroy()
+
+
 def test_autotuned_schedule_is_reported():
     S = bm.Server(memory_budget_bytes=8 << 30, max_systems=8)
     groups, xcd_run, per_bs = S.schedule()
